@@ -1,0 +1,155 @@
+"""ctypes mirror of include/crmech.h (structs and enums only, no library loading)."""
+from __future__ import annotations
+
+import ctypes as C
+
+CRM_MAX_SPECIES = 4
+CRM_MAX_ROD_VERTICES = 16
+
+# crm_status
+CRM_OK = 0
+CRM_CALC_ERROR = 1
+CRM_BOUNDARY_ERROR = 2
+CRM_INDEX_ERROR = 3
+CRM_RNG_ERROR = 4
+CRM_SETUP_ERROR = 5
+CRM_DEVICE_ERROR = 6
+CRM_INVALID_ARGUMENT = 7
+
+STATUS_NAMES = {
+    CRM_OK: "Ok",
+    CRM_CALC_ERROR: "CalcError",
+    CRM_BOUNDARY_ERROR: "BoundaryError",
+    CRM_INDEX_ERROR: "IndexError",
+    CRM_RNG_ERROR: "RngError",
+    CRM_SETUP_ERROR: "SetupError",
+    CRM_DEVICE_ERROR: "DeviceError",
+    CRM_INVALID_ARGUMENT: "InvalidArgument",
+}
+
+# crm_potential_kind
+POTENTIAL_MORSE = 0
+POTENTIAL_MIE = 1
+POTENTIAL_BOUND_LENNARD_JONES = 2
+POTENTIAL_MORSE_SPECIES = 3
+POTENTIAL_CELL_SORTING = 4
+POTENTIAL_NONE = 5
+
+# crm_mechanics_kind
+MECHANICS_NEWTON_DAMPED = 0
+MECHANICS_LANGEVIN = 1
+MECHANICS_BROWNIAN = 2
+MECHANICS_ROD = 3
+
+PRECISION = {"fp64": 0, "mixed": 1}
+WEIGHTING = {"reference": 0, "physical": 1}
+HISTORY_ORDER = {"oldest_first": 0, "newest_first": 1}
+NOISE = {"none": 0, "buffer": 1, "philox": 2}
+
+FLAG_GRAPH = 1
+FLAG_TIMING = 2
+
+KERNEL_CLASSES = ("force", "integrate", "scan", "scatter", "fixup", "gather")
+
+
+class Domain(C.Structure):
+    _fields_ = [
+        ("min", C.c_double * 3),
+        ("max", C.c_double * 3),
+        ("interaction_range", C.c_double),
+        ("n_voxels", C.c_uint32 * 3),
+    ]
+
+
+class Potential(C.Structure):
+    _fields_ = [
+        ("kind", C.c_uint32),
+        ("n_species", C.c_uint32),
+        ("radius", C.c_double),
+        ("potential_stiffness", C.c_double),
+        ("cutoff", C.c_double),
+        ("strength", C.c_double),
+        ("bound", C.c_double),
+        ("en", C.c_double),
+        ("em", C.c_double),
+        ("epsilon", C.c_double),
+        ("sigma", C.c_double),
+        ("relative_interaction_range", C.c_double),
+        ("sign", C.c_double),
+        ("species_matrix", C.c_double * (CRM_MAX_SPECIES * CRM_MAX_SPECIES)),
+    ]
+
+
+class Mechanics(C.Structure):
+    _fields_ = [
+        ("kind", C.c_uint32),
+        ("n_vertices", C.c_uint32),
+        ("mass", C.c_double),
+        ("damping", C.c_double),
+        ("kb_temperature", C.c_double),
+        ("diffusion_constant", C.c_double),
+        ("spring_tension", C.c_double),
+        ("rigidity", C.c_double),
+        ("spring_length", C.c_double),
+    ]
+
+
+class Instance(C.Structure):
+    _fields_ = [("domain", Domain), ("potential", Potential), ("mechanics", Mechanics)]
+
+
+class Config(C.Structure):
+    _fields_ = [
+        ("dim", C.c_uint32),
+        ("solver_order", C.c_uint32),
+        ("precision", C.c_uint32),
+        ("weighting", C.c_uint32),
+        ("history_order", C.c_uint32),
+        ("noise_mode", C.c_uint32),
+        ("seed", C.c_uint64),
+        ("dt", C.c_double),
+        ("device", C.c_int32),
+        ("flags", C.c_uint32),
+    ]
+
+
+class Info(C.Structure):
+    _fields_ = [
+        ("n_agents", C.c_uint64),
+        ("n_voxels_total", C.c_uint64),
+        ("step", C.c_uint64),
+        ("n_instances", C.c_uint32),
+        ("n_vertices", C.c_uint32),
+        ("dim", C.c_uint32),
+        ("error_step_valid", C.c_uint32),
+        ("error_step", C.c_uint64),
+        ("error_agent", C.c_uint64),
+    ]
+
+
+class Timing(C.Structure):
+    _fields_ = [("total_ms", C.c_double * 6), ("launches", C.c_uint64 * 6)]
+
+
+# Every symbol include/crmech.h declares, with (restype, argtypes).
+P = C.c_void_p
+DP = C.POINTER(C.c_double)
+U32P = C.POINTER(C.c_uint32)
+U64P = C.POINTER(C.c_uint64)
+I32P = C.POINTER(C.c_int32)
+
+SIGNATURES = {
+    "crm_grid": (C.c_int, [C.POINTER(Domain), C.c_uint32, U64P, DP]),
+    "crm_create": (C.c_int, [C.POINTER(Config), C.POINTER(Instance), C.c_uint32, C.POINTER(P)]),
+    "crm_set_agents": (C.c_int, [P, U64P, DP, DP, I32P]),
+    "crm_step": (C.c_int, [P, C.c_uint32, DP]),
+    "crm_compute_forces": (C.c_int, [P, DP]),
+    "crm_get_state": (C.c_int, [P, DP, DP]),
+    "crm_get_history": (C.c_int, [P, DP, DP, U32P]),
+    "crm_get_voxels": (C.c_int, [P, U32P, U32P]),
+    "crm_get_info": (C.c_int, [P, C.POINTER(Info)]),
+    "crm_get_timing": (C.c_int, [P, C.POINTER(Timing)]),
+    "crm_reset_timing": (C.c_int, [P]),
+    "crm_last_error": (C.c_size_t, [P, C.c_char_p, C.c_size_t]),
+    "crm_destroy": (None, [P]),
+}

@@ -1,0 +1,853 @@
+// crmech.hip -- host runtime and C ABI of the gfx950 mechanics backend.
+//
+// One context = one HIP stream, one set of device buffers, one or more
+// independent instances (CartesianCuboid domains) advanced together.
+// The drop-in boundary is include/crmech.h; see there for the reference
+// interfaces each entry point replaces.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "crmech.h"
+#include "kernels_point.hpp"
+
+namespace crm {
+
+#define CRM_HIP(expr)                                                                        \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess) {                                                              \
+            ctx->err = std::string("HIP error: ") + hipGetErrorString(e_) + " at " #expr;     \
+            return CRM_DEVICE_ERROR;                                                         \
+        }                                                                                    \
+    } while (0)
+
+struct HostGrid {
+    double min[3], max[3], dx[3];
+    uint64_t n[3];  // 3-D layout, axis 0 extent 1 for 2-D
+    uint64_t V;
+};
+
+// num-traits to_usize on f64 (truncation, -1 < x < 2^64, NaN fails)
+static bool to_usize(double x, uint64_t& out) {
+    if (std::isnan(x) || !(x > -1.0) || !(x < 18446744073709551616.0)) return false;
+    double t = std::trunc(x);
+    out = t <= 0.0 ? 0 : (uint64_t)t;
+    return true;
+}
+
+// CartesianCuboid::from_boundaries_and_interaction_range / _n_voxels
+// (cartesian_cuboid_n.rs:149-162,183-256).
+static crm_status host_grid(const crm_domain& d, uint32_t D, HostGrid& g, std::string& err) {
+    if (D < 2 || D > 3) {
+        err = "dim must be 2 or 3";
+        return CRM_INVALID_ARGUMENT;
+    }
+    for (uint32_t i = 0; i < D; i++)
+        if (d.min[i] >= d.max[i]) {
+            err = "BoundaryError: Min must be smaller than Max for domain boundaries";
+            return CRM_BOUNDARY_ERROR;
+        }
+    for (int a = 0; a < 3; a++) {
+        g.min[a] = 0.0;
+        g.max[a] = 1.0;
+        g.dx[a] = 1.0;
+        g.n[a] = 1;
+    }
+    for (uint32_t i = 0; i < D; i++) {
+        const int a = (int)(i + 3 - D);
+        g.min[a] = d.min[i];
+        g.max[a] = d.max[i];
+        if (d.n_voxels[0] == 0) {
+            double n = std::floor((d.max[i] - d.min[i]) / d.interaction_range);
+            uint64_t m;
+            if (!to_usize(n, m)) {
+                err = "BoundaryError: Cannot convert float to usize (conversion error during domain setup)";
+                return CRM_BOUNDARY_ERROR;
+            }
+            g.n[a] = std::max<uint64_t>(m, 1);
+            g.dx[a] = (d.max[i] - d.min[i]) / n;
+        } else {
+            if (d.n_voxels[i] == 0) {
+                err = "BoundaryError: n_voxels must be positive";
+                return CRM_BOUNDARY_ERROR;
+            }
+            g.n[a] = d.n_voxels[i];
+            g.dx[a] = (d.max[i] - d.min[i]) / (double)d.n_voxels[i];
+        }
+    }
+    g.V = g.n[0] * g.n[1] * g.n[2];
+    return CRM_OK;
+}
+
+static int exact_int_exponent(double e) {
+    if (e >= 0.0 && e <= 64.0 && std::floor(e) == e) return (int)e;
+    return -1;
+}
+
+template <typename T>
+static void fill_pot(PotParams<T>& q, const crm_potential& p) {
+    const double R2 = p.radius + p.radius;
+    q.radius = (T)p.radius;
+    q.two_radius = (T)R2;
+    q.stiffness = (T)p.potential_stiffness;
+    q.cutoff = (T)p.cutoff;
+    q.strength = (T)p.strength;
+    q.bound = (T)p.bound;
+    q.en = (T)p.en;
+    q.em = (T)p.em;
+    // MiePotential: sigma = (em/en)^(1/(en-em)) * (R_own + R_ext) and
+    // C = en/(en-em) * (en/em)^(em/(en-em)) (interaction.rs:363-367,390-392)
+    q.mie_sigma = (T)(std::pow(p.em / p.en, 1.0 / (p.en - p.em)) * R2);
+    q.mie_constant = (T)(p.en / (p.en - p.em) * std::pow(p.en / p.em, p.em / (p.en - p.em)));
+    q.epsilon = (T)p.epsilon;
+    q.sigma = (T)p.sigma;
+    q.rel_range = (T)p.relative_interaction_range;
+    q.sign = (T)(p.sign < 0 ? -1.0 : 1.0);
+    for (int i = 0; i < CRM_MAX_SPECIES * CRM_MAX_SPECIES; i++) q.species_matrix[i] = (T)p.species_matrix[i];
+    q.ien1 = exact_int_exponent(p.en + 1.0);
+    q.iem1 = exact_int_exponent(p.em + 1.0);
+    q.pad0 = q.pad1 = 0;
+}
+
+enum { kNumKernelClasses = CRM_KERNEL_COUNT };
+
+}  // namespace crm
+
+using namespace crm;
+
+struct crm_ctx {
+    crm_config cfg;
+    uint32_t D = 3, nv = 1, K = 3;
+    uint32_t n_inst = 0;
+    std::vector<crm_instance> inst;
+    std::vector<HostGrid> grids;
+    std::vector<InstParams> hparams;
+    uint64_t Vt = 0;
+    uint32_t pot_kind = 0, mech_kind = 0;
+    bool symmetric = true;
+    std::string err;
+
+    int device = 0;
+    hipStream_t stream = nullptr;
+
+    // device memory
+    InstParams* d_ip = nullptr;
+    BlockWork* d_work = nullptr;
+    uint32_t n_work = 0;
+    Control* d_ctrl = nullptr;
+    uint32_t* d_start = nullptr;
+    uint32_t* d_count = nullptr;
+    uint32_t* d_bsum = nullptr;
+    uint32_t n_tiles = 0;
+    uint32_t* d_key = nullptr;
+    uint32_t* d_rank = nullptr;
+    uint32_t* d_perm = nullptr;
+    double* d_force = nullptr;
+    double4* d_rec64 = nullptr;
+    float4* d_rec32 = nullptr;
+    StateBuf buf[2] = {};
+    int cur = 0;
+    double* d_noise = nullptr;
+    size_t noise_cap = 0;
+
+    uint64_t N = 0;
+    std::vector<uint64_t> counts;
+    bool have_agents = false;
+    bool halted = false;
+    uint64_t step = 0;
+
+    // graphs for the two buffer parities
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+
+    // timing
+    hipEvent_t ev_a = nullptr, ev_b = nullptr;
+    double t_ms[CRM_KERNEL_COUNT] = {};
+    uint64_t t_n[CRM_KERNEL_COUNT] = {};
+};
+
+namespace {
+
+// Message of the last failed crm_create on this thread (crm_last_error(NULL, ...)).
+thread_local std::string g_create_error;
+
+void free_device(crm_ctx* ctx) {
+    void* ptrs[] = {ctx->d_work, ctx->d_start, ctx->d_count, ctx->d_bsum, ctx->d_key, ctx->d_rank,
+                    ctx->d_perm, ctx->d_force, ctx->d_rec64, ctx->d_rec32, ctx->d_noise};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    for (auto& b : ctx->buf) {
+        void* q[] = {b.pos, b.vel, b.hdx, b.hdv, b.species, b.id, b.vox};
+        for (void* p : q)
+            if (p) (void)hipFree(p);
+        b = StateBuf{};
+    }
+    ctx->d_work = nullptr;
+    ctx->d_start = ctx->d_count = ctx->d_bsum = ctx->d_key = ctx->d_rank = ctx->d_perm = nullptr;
+    ctx->d_force = nullptr;
+    ctx->d_rec64 = nullptr;
+    ctx->d_rec32 = nullptr;
+    ctx->d_noise = nullptr;
+    ctx->noise_cap = 0;
+    for (auto& g : ctx->graph)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+}
+
+Arrays arrays(crm_ctx* ctx) {
+    Arrays A;
+    A.ip = ctx->d_ip;
+    A.work = ctx->d_work;
+    A.ctrl = ctx->d_ctrl;
+    A.start = ctx->d_start;
+    A.count = ctx->d_count;
+    A.key = ctx->d_key;
+    A.rank = ctx->d_rank;
+    A.perm = ctx->d_perm;
+    A.force = ctx->d_force;
+    A.rec64 = ctx->d_rec64;
+    A.rec32 = ctx->d_rec32;
+    A.N = ctx->N;
+    A.Vt = (uint32_t)ctx->Vt;
+    return A;
+}
+
+bool timing_on(crm_ctx* ctx) { return (ctx->cfg.flags & CRM_FLAG_TIMING) != 0; }
+
+crm_status t_begin(crm_ctx* ctx) {
+    if (timing_on(ctx)) CRM_HIP(hipEventRecord(ctx->ev_a, ctx->stream));
+    return CRM_OK;
+}
+
+crm_status t_end(crm_ctx* ctx, int cls) {
+    CRM_HIP(hipGetLastError());
+    if (timing_on(ctx)) {
+        CRM_HIP(hipEventRecord(ctx->ev_b, ctx->stream));
+        CRM_HIP(hipEventSynchronize(ctx->ev_b));
+        float ms = 0.f;
+        CRM_HIP(hipEventElapsedTime(&ms, ctx->ev_a, ctx->ev_b));
+        ctx->t_ms[cls] += ms;
+        ctx->t_n[cls] += 1;
+    }
+    return CRM_OK;
+}
+
+#define CRM_TRY(expr)                  \
+    do {                               \
+        crm_status s_ = (expr);        \
+        if (s_ != CRM_OK) return s_;   \
+    } while (0)
+
+template <int D, int POT>
+crm_status launch_force_dp(crm_ctx* ctx, StateBuf S) {
+    Arrays A = arrays(ctx);
+    const uint32_t phys = ctx->cfg.weighting == CRM_WEIGHT_PHYSICAL;
+    const uint32_t sym = ctx->symmetric;
+    if (ctx->cfg.precision == CRM_PRECISION_MIXED)
+        hipLaunchKernelGGL((k_force_point<D, POT, true>), dim3(ctx->n_work), dim3(kBlock), 0, ctx->stream, A, S,
+                           phys, sym);
+    else
+        hipLaunchKernelGGL((k_force_point<D, POT, false>), dim3(ctx->n_work), dim3(kBlock), 0, ctx->stream, A, S,
+                           phys, sym);
+    return CRM_OK;
+}
+
+template <int D>
+crm_status launch_force_d(crm_ctx* ctx, StateBuf S) {
+    switch (ctx->pot_kind) {
+        case CRM_POTENTIAL_MORSE: return launch_force_dp<D, CRM_POTENTIAL_MORSE>(ctx, S);
+        case CRM_POTENTIAL_MIE: return launch_force_dp<D, CRM_POTENTIAL_MIE>(ctx, S);
+        case CRM_POTENTIAL_BOUND_LENNARD_JONES: return launch_force_dp<D, CRM_POTENTIAL_BOUND_LENNARD_JONES>(ctx, S);
+        case CRM_POTENTIAL_MORSE_SPECIES: return launch_force_dp<D, CRM_POTENTIAL_MORSE_SPECIES>(ctx, S);
+        case CRM_POTENTIAL_CELL_SORTING: return launch_force_dp<D, CRM_POTENTIAL_CELL_SORTING>(ctx, S);
+        case CRM_POTENTIAL_NONE: return launch_force_dp<D, CRM_POTENTIAL_NONE>(ctx, S);
+    }
+    return CRM_INVALID_ARGUMENT;
+}
+
+crm_status launch_force(crm_ctx* ctx) {
+    CRM_TRY(t_begin(ctx));
+    StateBuf S = ctx->buf[ctx->cur];
+    crm_status st = ctx->D == 2 ? launch_force_d<2>(ctx, S) : launch_force_d<3>(ctx, S);
+    if (st) return st;
+    return t_end(ctx, CRM_KERNEL_FORCE);
+}
+
+template <int D>
+crm_status launch_integrate_d(crm_ctx* ctx, StateBuf S, const StepArgs& sa) {
+    Arrays A = arrays(ctx);
+    switch (ctx->mech_kind) {
+        case CRM_MECHANICS_NEWTON_DAMPED:
+            hipLaunchKernelGGL((k_integrate_point<D, CRM_MECHANICS_NEWTON_DAMPED>), dim3(ctx->n_work), dim3(kBlock), 0,
+                               ctx->stream, A, S, sa);
+            return CRM_OK;
+        case CRM_MECHANICS_LANGEVIN:
+            hipLaunchKernelGGL((k_integrate_point<D, CRM_MECHANICS_LANGEVIN>), dim3(ctx->n_work), dim3(kBlock), 0,
+                               ctx->stream, A, S, sa);
+            return CRM_OK;
+        case CRM_MECHANICS_BROWNIAN:
+            hipLaunchKernelGGL((k_integrate_point<D, CRM_MECHANICS_BROWNIAN>), dim3(ctx->n_work), dim3(kBlock), 0,
+                               ctx->stream, A, S, sa);
+            return CRM_OK;
+    }
+    return CRM_INVALID_ARGUMENT;
+}
+
+crm_status launch_integrate(crm_ctx* ctx, const double* noise_step) {
+    StepArgs sa;
+    sa.noise = noise_step;
+    sa.seed = ctx->cfg.seed;
+    sa.dt = ctx->cfg.dt;
+    sa.sqrt_dt = std::sqrt(ctx->cfg.dt);
+    sa.noise_mode = ctx->cfg.noise_mode;
+    sa.cap = ctx->cfg.solver_order;
+    sa.newest_first = ctx->cfg.history_order == CRM_HISTORY_NEWEST_FIRST;
+    sa.pad = 0;
+    CRM_TRY(t_begin(ctx));
+    StateBuf S = ctx->buf[ctx->cur];
+    crm_status st = ctx->D == 2 ? launch_integrate_d<2>(ctx, S, sa) : launch_integrate_d<3>(ctx, S, sa);
+    if (st) return st;
+    return t_end(ctx, CRM_KERNEL_INTEGRATE);
+}
+
+// scan + scatter + fixup + gather: the counting sort that rebuilds the voxel
+// lists from d_key / d_rank / d_count and moves the state to the other buffer.
+crm_status launch_sort(crm_ctx* ctx, bool advance) {
+    Arrays A = arrays(ctx);
+    const uint32_t Vt = (uint32_t)ctx->Vt;
+    CRM_TRY(t_begin(ctx));
+    hipLaunchKernelGGL(k_scan_local, dim3(ctx->n_tiles), dim3(kBlock), 0, ctx->stream, ctx->d_count, ctx->d_start,
+                       ctx->d_bsum, Vt);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->d_bsum, ctx->n_tiles, ctx->d_start,
+                       Vt, ctx->d_ctrl, (uint32_t)(advance ? 1 : 0));
+    hipLaunchKernelGGL(k_scan_add, dim3((Vt + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, ctx->d_start,
+                       ctx->d_bsum, Vt);
+    CRM_TRY(t_end(ctx, CRM_KERNEL_SCAN));
+    CRM_TRY(t_begin(ctx));
+    hipLaunchKernelGGL(k_scatter, dim3((uint32_t)((ctx->N + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream, A);
+    CRM_TRY(t_end(ctx, CRM_KERNEL_SCATTER));
+    CRM_TRY(t_begin(ctx));
+    hipLaunchKernelGGL(k_fixup, dim3((Vt + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, A);
+    CRM_TRY(t_end(ctx, CRM_KERNEL_FIXUP));
+    CRM_TRY(t_begin(ctx));
+    StateBuf src = ctx->buf[ctx->cur], dst = ctx->buf[1 - ctx->cur];
+    const uint32_t cap = ctx->cfg.solver_order;
+    const bool mixed = ctx->cfg.precision == CRM_PRECISION_MIXED;
+    if (ctx->D == 2) {
+        if (mixed)
+            hipLaunchKernelGGL((k_gather_point<2, true>), dim3(ctx->n_work), dim3(kBlock), 0, ctx->stream, A, src, dst,
+                               cap);
+        else
+            hipLaunchKernelGGL((k_gather_point<2, false>), dim3(ctx->n_work), dim3(kBlock), 0, ctx->stream, A, src,
+                               dst, cap);
+    } else {
+        if (mixed)
+            hipLaunchKernelGGL((k_gather_point<3, true>), dim3(ctx->n_work), dim3(kBlock), 0, ctx->stream, A, src, dst,
+                               cap);
+        else
+            hipLaunchKernelGGL((k_gather_point<3, false>), dim3(ctx->n_work), dim3(kBlock), 0, ctx->stream, A, src,
+                               dst, cap);
+    }
+    CRM_TRY(t_end(ctx, CRM_KERNEL_GATHER));
+    ctx->cur = 1 - ctx->cur;
+    return CRM_OK;
+}
+
+crm_status launch_step(crm_ctx* ctx, const double* noise_step) {
+    CRM_TRY(launch_force(ctx));
+    CRM_TRY(launch_integrate(ctx, noise_step));
+    CRM_TRY(launch_sort(ctx, true));
+    return CRM_OK;
+}
+
+crm_status read_control(crm_ctx* ctx, Control& c) {
+    CRM_HIP(hipMemcpyAsync(&c, ctx->d_ctrl, sizeof(Control), hipMemcpyDeviceToHost, ctx->stream));
+    CRM_HIP(hipStreamSynchronize(ctx->stream));
+    return CRM_OK;
+}
+
+// Maps the device error word to a status and message.
+crm_status check_errors(crm_ctx* ctx, const char* phase) {
+    Control c;
+    CRM_TRY(read_control(ctx, c));
+    if (!c.error_flags) return CRM_OK;
+    crm_status kind = CRM_CALC_ERROR;
+    const char* msg = "CalcError: identical position for two objects. Cannot Calculate force in Mie potential";
+    if (c.error_flags & (1u << CRM_CALC_ERROR)) {
+    } else if (c.error_flags & (1u << CRM_BOUNDARY_ERROR)) {
+        kind = CRM_BOUNDARY_ERROR;
+        msg = "BoundaryError: Particle is out of domain";
+    } else if (c.error_flags & (1u << CRM_INDEX_ERROR)) {
+        kind = CRM_INDEX_ERROR;
+        msg = "DecomposeError: could not find voxel index";
+    } else {
+        kind = CRM_DEVICE_ERROR;
+        msg = "unknown device error";
+    }
+    char buf[256];
+    std::snprintf(buf, sizeof(buf), "%s (%s, step %llu, agent %llu)", msg, phase,
+                  (unsigned long long)(c.error_key >> 32), (unsigned long long)(c.error_key & 0xffffffffu));
+    ctx->err = buf;
+    ctx->halted = true;
+    return kind;
+}
+
+crm_status reset_control(crm_ctx* ctx, uint64_t step) {
+    Control c;
+    std::memset(&c, 0, sizeof(c));
+    c.step = step;
+    c.error_key = ~0ull;
+    CRM_HIP(hipMemcpyAsync(ctx->d_ctrl, &c, sizeof(Control), hipMemcpyHostToDevice, ctx->stream));
+    CRM_HIP(hipStreamSynchronize(ctx->stream));
+    return CRM_OK;
+}
+
+template <typename T>
+crm_status dalloc(crm_ctx* ctx, T*& p, size_t n) {
+    CRM_HIP(hipMalloc((void**)&p, std::max<size_t>(n, 1) * sizeof(T)));
+    return CRM_OK;
+}
+
+// Downloads a [C][N] SoA array and reorders it to agent index (rows of C).
+crm_status download_by_id(crm_ctx* ctx, const double* d_src, uint32_t C, double* out,
+                          const std::vector<uint32_t>& id) {
+    std::vector<double> tmp((size_t)C * ctx->N);
+    CRM_HIP(hipMemcpyAsync(tmp.data(), d_src, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    CRM_HIP(hipStreamSynchronize(ctx->stream));
+    for (uint64_t s = 0; s < ctx->N; s++)
+        for (uint32_t k = 0; k < C; k++) out[(uint64_t)id[s] * C + k] = tmp[(uint64_t)k * ctx->N + s];
+    return CRM_OK;
+}
+
+crm_status download_ids(crm_ctx* ctx, std::vector<uint32_t>& id) {
+    id.resize(ctx->N);
+    CRM_HIP(hipMemcpyAsync(id.data(), ctx->buf[ctx->cur].id, ctx->N * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    CRM_HIP(hipStreamSynchronize(ctx->stream));
+    return CRM_OK;
+}
+
+crm_status build_graph(crm_ctx* ctx, int parity) {
+    // Two steps return the buffers to the same parity, so one captured
+    // two-step sequence replays for any even number of steps.
+    int saved = ctx->cur;
+    ctx->cur = parity;
+    CRM_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    crm_status st = launch_step(ctx, nullptr);
+    if (st == CRM_OK) st = launch_step(ctx, nullptr);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    ctx->cur = saved;
+    if (st) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (e != hipSuccess) {
+        ctx->err = std::string("HIP error: ") + hipGetErrorString(e) + " at hipStreamEndCapture";
+        return CRM_DEVICE_ERROR;
+    }
+    e = hipGraphInstantiate(&ctx->graph[parity], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        ctx->err = std::string("HIP error: ") + hipGetErrorString(e) + " at hipGraphInstantiate";
+        return CRM_DEVICE_ERROR;
+    }
+    return CRM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+crm_status crm_grid(const crm_domain* domain, uint32_t dim, uint64_t n_voxels_out[3], double dx_out[3]) {
+    if (!domain || !n_voxels_out || !dx_out) return CRM_INVALID_ARGUMENT;
+    HostGrid g;
+    std::string err;
+    crm_status st = host_grid(*domain, dim, g, err);
+    if (st) return st;
+    for (uint32_t i = 0; i < 3; i++) {
+        n_voxels_out[i] = i < dim ? g.n[i + 3 - dim] : 1;
+        dx_out[i] = i < dim ? g.dx[i + 3 - dim] : 0.0;
+    }
+    return CRM_OK;
+}
+
+crm_status crm_create(const crm_config* cfg, const crm_instance* instances, uint32_t n_instances, crm_ctx** out) {
+    if (!out) return CRM_INVALID_ARGUMENT;
+    *out = nullptr;
+    g_create_error.clear();
+    if (!cfg || !instances || n_instances == 0) {
+        g_create_error = "cfg, instances and n_instances > 0 are required";
+        return CRM_INVALID_ARGUMENT;
+    }
+    crm_ctx* ctx = new crm_ctx();
+    ctx->cfg = *cfg;
+    auto fail = [&](crm_status st, const std::string& msg) {
+        g_create_error = msg;
+        crm_destroy(ctx);
+        return st;
+    };
+    if (cfg->dim < 2 || cfg->dim > 3) return fail(CRM_INVALID_ARGUMENT, "dim must be 2 or 3");
+    if (cfg->solver_order > 2) return fail(CRM_INVALID_ARGUMENT, "mechanics_solver_order must be 0, 1 or 2");
+    if (cfg->precision > CRM_PRECISION_MIXED || cfg->weighting > CRM_WEIGHT_PHYSICAL ||
+        cfg->history_order > CRM_HISTORY_NEWEST_FIRST || cfg->noise_mode > CRM_NOISE_PHILOX)
+        return fail(CRM_INVALID_ARGUMENT, "unknown precision / weighting / history order / noise mode");
+    if (!(cfg->dt > 0.0)) return fail(CRM_INVALID_ARGUMENT, "dt must be positive");
+    ctx->D = cfg->dim;
+    ctx->n_inst = n_instances;
+    ctx->inst.assign(instances, instances + n_instances);
+    ctx->pot_kind = instances[0].potential.kind;
+    ctx->mech_kind = instances[0].mechanics.kind;
+    if (ctx->pot_kind > CRM_POTENTIAL_NONE) return fail(CRM_INVALID_ARGUMENT, "unknown potential kind");
+    if (ctx->mech_kind > CRM_MECHANICS_BROWNIAN)
+        return fail(CRM_INVALID_ARGUMENT, "unsupported mechanics kind for point agents");
+    ctx->nv = 1;
+    ctx->K = ctx->nv * ctx->D;
+    uint64_t voff = 0;
+    for (uint32_t i = 0; i < n_instances; i++) {
+        const crm_instance& I = instances[i];
+        if (I.potential.kind != ctx->pot_kind || I.mechanics.kind != ctx->mech_kind)
+            return fail(CRM_INVALID_ARGUMENT, "all instances must share potential and mechanics kinds");
+        if ((I.potential.kind == CRM_POTENTIAL_MORSE_SPECIES || I.potential.kind == CRM_POTENTIAL_CELL_SORTING) &&
+            (I.potential.n_species == 0 || I.potential.n_species > CRM_MAX_SPECIES))
+            return fail(CRM_INVALID_ARGUMENT, "n_species must be 1..4");
+        HostGrid g;
+        crm_status st = host_grid(I.domain, ctx->D, g, ctx->err);
+        if (st) return fail(st, ctx->err);
+        ctx->grids.push_back(g);
+        InstParams P;
+        std::memset(&P, 0, sizeof(P));
+        for (int a = 0; a < 3; a++) {
+            P.min[a] = g.min[a];
+            P.max[a] = g.max[a];
+            P.dx[a] = g.dx[a];
+            P.dxf[a] = (float)g.dx[a];
+            P.n[a] = (uint32_t)g.n[a];
+        }
+        P.voxel_offset = (uint32_t)voff;
+        P.n_voxels = (uint32_t)g.V;
+        fill_pot<double>(P.pd, I.potential);
+        fill_pot<float>(P.pf, I.potential);
+        const crm_mechanics& M = I.mechanics;
+        P.mass = M.mass;
+        P.damping = M.damping;
+        P.kb_temperature = M.kb_temperature;
+        P.diffusion_constant = M.diffusion_constant;
+        P.spring_tension = M.spring_tension;
+        P.rigidity = M.rigidity;
+        P.spring_length = M.spring_length;
+        P.inv_mass = 1.0 / M.mass;
+        P.langevin_coeff = std::sqrt(2.0 * M.damping * M.kb_temperature / M.mass);
+        P.brownian_q = M.diffusion_constant / M.kb_temperature;
+        P.brownian_coeff = std::sqrt(2.0 * M.diffusion_constant);
+        ctx->hparams.push_back(P);
+        voff += g.V;
+        if (I.potential.kind == CRM_POTENTIAL_MORSE_SPECIES) {
+            for (uint32_t a = 0; a < I.potential.n_species; a++)
+                for (uint32_t b = 0; b < I.potential.n_species; b++)
+                    if (I.potential.species_matrix[a * CRM_MAX_SPECIES + b] !=
+                        I.potential.species_matrix[b * CRM_MAX_SPECIES + a])
+                        ctx->symmetric = false;
+        }
+    }
+    if (voff >= 0xffffffffull) return fail(CRM_INVALID_ARGUMENT, "too many voxels (more than 2^32 - 1)");
+    ctx->Vt = voff;
+    ctx->device = cfg->device;
+    hipError_t e = hipSetDevice(cfg->device);
+    if (e != hipSuccess) return fail(CRM_DEVICE_ERROR, std::string("hipSetDevice: ") + hipGetErrorString(e));
+    e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return fail(CRM_DEVICE_ERROR, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    if (hipEventCreate(&ctx->ev_a) != hipSuccess || hipEventCreate(&ctx->ev_b) != hipSuccess)
+        return fail(CRM_DEVICE_ERROR, "hipEventCreate failed");
+    if (hipMalloc((void**)&ctx->d_ip, sizeof(InstParams) * n_instances) != hipSuccess ||
+        hipMalloc((void**)&ctx->d_ctrl, sizeof(Control)) != hipSuccess)
+        return fail(CRM_DEVICE_ERROR, "hipMalloc failed");
+    e = hipMemcpy(ctx->d_ip, ctx->hparams.data(), sizeof(InstParams) * n_instances, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail(CRM_DEVICE_ERROR, std::string("hipMemcpy: ") + hipGetErrorString(e));
+    *out = ctx;
+    return CRM_OK;
+}
+
+crm_status crm_set_agents(crm_ctx* ctx, const uint64_t* counts, const double* pos, const double* vel,
+                          const int32_t* species) {
+    if (!ctx || !counts || !pos) return CRM_INVALID_ARGUMENT;
+    CRM_HIP(hipSetDevice(ctx->device));
+    uint64_t N = 0;
+    for (uint32_t i = 0; i < ctx->n_inst; i++) N += counts[i];
+    if (N == 0) {
+        ctx->err = "at least one agent is required";
+        return CRM_INVALID_ARGUMENT;
+    }
+    if (N >= 0xffffffffull) {
+        ctx->err = "too many agents (more than 2^32 - 1)";
+        return CRM_INVALID_ARGUMENT;
+    }
+    const uint32_t nsp = ctx->inst[0].potential.n_species;
+    if (species && (ctx->pot_kind == CRM_POTENTIAL_MORSE_SPECIES || ctx->pot_kind == CRM_POTENTIAL_CELL_SORTING))
+        for (uint64_t a = 0; a < N; a++)
+            if (species[a] < 0 || (uint32_t)species[a] >= nsp) {
+                ctx->err = "species out of range (agent " + std::to_string(a) + ")";
+                return CRM_INVALID_ARGUMENT;
+            }
+    CRM_HIP(hipStreamSynchronize(ctx->stream));
+    free_device(ctx);
+    ctx->N = N;
+    ctx->counts.assign(counts, counts + ctx->n_inst);
+    ctx->have_agents = false;
+    ctx->halted = false;
+    ctx->step = 0;
+    ctx->err.clear();
+    const uint32_t K = ctx->K;
+
+    // block work table
+    std::vector<BlockWork> work;
+    uint64_t first = 0;
+    for (uint32_t i = 0; i < ctx->n_inst; i++) {
+        ctx->hparams[i].agent_offset = first;
+        ctx->hparams[i].n_agents = counts[i];
+        for (uint64_t b = 0; b < counts[i]; b += kBlock) {
+            BlockWork w;
+            w.inst = i;
+            w.first = first + b;
+            w.count = (uint32_t)std::min<uint64_t>(kBlock, counts[i] - b);
+            work.push_back(w);
+        }
+        first += counts[i];
+    }
+    ctx->n_work = (uint32_t)work.size();
+    CRM_HIP(hipMemcpy(ctx->d_ip, ctx->hparams.data(), sizeof(InstParams) * ctx->n_inst, hipMemcpyHostToDevice));
+
+    const uint64_t Vt = ctx->Vt;
+    ctx->n_tiles = (uint32_t)((Vt + kScanTile - 1) / kScanTile);
+    CRM_TRY(dalloc(ctx, ctx->d_work, work.size()));
+    CRM_TRY(dalloc(ctx, ctx->d_start, Vt + 1));
+    CRM_TRY(dalloc(ctx, ctx->d_count, Vt));
+    CRM_TRY(dalloc(ctx, ctx->d_bsum, ctx->n_tiles));
+    CRM_TRY(dalloc(ctx, ctx->d_key, N));
+    CRM_TRY(dalloc(ctx, ctx->d_rank, N));
+    CRM_TRY(dalloc(ctx, ctx->d_perm, N));
+    CRM_TRY(dalloc(ctx, ctx->d_force, (size_t)K * N));
+    if (ctx->cfg.precision == CRM_PRECISION_MIXED) CRM_TRY(dalloc(ctx, ctx->d_rec32, N));
+    else CRM_TRY(dalloc(ctx, ctx->d_rec64, N));
+    for (auto& b : ctx->buf) {
+        CRM_TRY(dalloc(ctx, b.pos, (size_t)K * N));
+        CRM_TRY(dalloc(ctx, b.vel, (size_t)K * N));
+        CRM_TRY(dalloc(ctx, b.hdx, (size_t)2 * K * N));
+        CRM_TRY(dalloc(ctx, b.hdv, (size_t)2 * K * N));
+        CRM_TRY(dalloc(ctx, b.species, N));
+        CRM_TRY(dalloc(ctx, b.id, N));
+        CRM_TRY(dalloc(ctx, b.vox, N));
+    }
+    CRM_HIP(hipMemcpy(ctx->d_work, work.data(), sizeof(BlockWork) * work.size(), hipMemcpyHostToDevice));
+    CRM_HIP(hipMemsetAsync(ctx->d_count, 0, sizeof(uint32_t) * std::max<uint64_t>(Vt, 1), ctx->stream));
+
+    // upload in agent-index order (slot = agent index before the first sort)
+    ctx->cur = 0;
+    StateBuf& S = ctx->buf[0];
+    {
+        std::vector<double> tp((size_t)K * N), tv((size_t)K * N, 0.0);
+        for (uint64_t a = 0; a < N; a++)
+            for (uint32_t k = 0; k < K; k++) {
+                tp[(uint64_t)k * N + a] = pos[a * K + k];
+                if (vel) tv[(uint64_t)k * N + a] = vel[a * K + k];
+            }
+        std::vector<int32_t> sp(N, 0);
+        std::vector<uint32_t> id(N);
+        for (uint64_t a = 0; a < N; a++) {
+            sp[a] = species ? species[a] : 0;
+            id[a] = (uint32_t)a;
+        }
+        CRM_HIP(hipMemcpyAsync(S.pos, tp.data(), tp.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        CRM_HIP(hipMemcpyAsync(S.vel, tv.data(), tv.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        CRM_HIP(hipMemcpyAsync(S.species, sp.data(), N * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+        CRM_HIP(hipMemcpyAsync(S.id, id.data(), N * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+        CRM_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    CRM_TRY(reset_control(ctx, 0));
+    Arrays A = arrays(ctx);
+    if (ctx->D == 2)
+        hipLaunchKernelGGL(k_initial_key<2>, dim3(ctx->n_work), dim3(kBlock), 0, ctx->stream, A, S, ctx->nv);
+    else
+        hipLaunchKernelGGL(k_initial_key<3>, dim3(ctx->n_work), dim3(kBlock), 0, ctx->stream, A, S, ctx->nv);
+    CRM_HIP(hipGetLastError());
+    CRM_TRY(check_errors(ctx, "initial placement"));
+    CRM_TRY(launch_sort(ctx, false));
+    CRM_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->have_agents = true;
+    return CRM_OK;
+}
+
+crm_status crm_step(crm_ctx* ctx, uint32_t n_steps, const double* noise) {
+    if (!ctx) return CRM_INVALID_ARGUMENT;
+    if (!ctx->have_agents) {
+        ctx->err = "crm_set_agents must be called first";
+        return CRM_INVALID_ARGUMENT;
+    }
+    if (ctx->halted) return CRM_INVALID_ARGUMENT;
+    CRM_HIP(hipSetDevice(ctx->device));
+    const bool use_noise_buf = ctx->cfg.noise_mode == CRM_NOISE_BUFFER && noise != nullptr;
+    const size_t per_step = (size_t)ctx->N * ctx->K;
+    if (use_noise_buf) {
+        size_t need = per_step * n_steps;
+        if (need > ctx->noise_cap) {
+            if (ctx->d_noise) CRM_HIP(hipFree(ctx->d_noise));
+            ctx->d_noise = nullptr;
+            CRM_TRY(dalloc(ctx, ctx->d_noise, need));
+            ctx->noise_cap = need;
+        }
+        CRM_HIP(hipMemcpyAsync(ctx->d_noise, noise, need * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    }
+    const bool graphs = (ctx->cfg.flags & CRM_FLAG_GRAPH) && !timing_on(ctx) && !use_noise_buf;
+    uint32_t s = 0;
+    if (graphs) {
+        while (s + 2 <= n_steps) {
+            if (!ctx->graph[ctx->cur]) CRM_TRY(build_graph(ctx, ctx->cur));
+            CRM_HIP(hipGraphLaunch(ctx->graph[ctx->cur], ctx->stream));
+            s += 2;
+        }
+    }
+    for (; s < n_steps; s++) CRM_TRY(launch_step(ctx, use_noise_buf ? ctx->d_noise + per_step * s : nullptr));
+    CRM_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->step += n_steps;
+    return check_errors(ctx, "step");
+}
+
+crm_status crm_compute_forces(crm_ctx* ctx, double* force) {
+    if (!ctx || !force) return CRM_INVALID_ARGUMENT;
+    if (!ctx->have_agents) return CRM_INVALID_ARGUMENT;
+    CRM_HIP(hipSetDevice(ctx->device));
+    CRM_TRY(launch_force(ctx));
+    CRM_HIP(hipStreamSynchronize(ctx->stream));
+    crm_status st = check_errors(ctx, "forces");
+    if (st) return st;
+    std::vector<uint32_t> id;
+    CRM_TRY(download_ids(ctx, id));
+    return download_by_id(ctx, ctx->d_force, ctx->K, force, id);
+}
+
+crm_status crm_get_state(crm_ctx* ctx, double* pos, double* vel) {
+    if (!ctx || !ctx->have_agents) return CRM_INVALID_ARGUMENT;
+    CRM_HIP(hipSetDevice(ctx->device));
+    std::vector<uint32_t> id;
+    CRM_TRY(download_ids(ctx, id));
+    if (pos) CRM_TRY(download_by_id(ctx, ctx->buf[ctx->cur].pos, ctx->K, pos, id));
+    if (vel) CRM_TRY(download_by_id(ctx, ctx->buf[ctx->cur].vel, ctx->K, vel, id));
+    return CRM_OK;
+}
+
+crm_status crm_get_history(crm_ctx* ctx, double* dx_hist, double* dv_hist, uint32_t* n_stored) {
+    if (!ctx || !ctx->have_agents) return CRM_INVALID_ARGUMENT;
+    CRM_HIP(hipSetDevice(ctx->device));
+    const uint32_t cap = ctx->cfg.solver_order;
+    const uint64_t t = ctx->step;
+    const uint32_t size = cap == 0 ? 0u : (uint32_t)std::min<uint64_t>(t, cap);
+    const uint32_t oldest = (size < cap || cap == 0) ? 0u : (uint32_t)(t % cap);
+    if (n_stored) *n_stored = size;
+    std::vector<uint32_t> id;
+    CRM_TRY(download_ids(ctx, id));
+    const uint64_t slot_elems = (uint64_t)ctx->N * ctx->K;
+    for (uint32_t i = 0; i < 2; i++) {
+        if (i < size) {
+            uint32_t slot = (oldest + i) % cap;
+            if (dx_hist)
+                CRM_TRY(download_by_id(ctx, ctx->buf[ctx->cur].hdx + slot * slot_elems, ctx->K,
+                                       dx_hist + i * slot_elems, id));
+            if (dv_hist)
+                CRM_TRY(download_by_id(ctx, ctx->buf[ctx->cur].hdv + slot * slot_elems, ctx->K,
+                                       dv_hist + i * slot_elems, id));
+        } else {
+            if (dx_hist) std::fill(dx_hist + i * slot_elems, dx_hist + (i + 1) * slot_elems, 0.0);
+            if (dv_hist) std::fill(dv_hist + i * slot_elems, dv_hist + (i + 1) * slot_elems, 0.0);
+        }
+    }
+    return CRM_OK;
+}
+
+crm_status crm_get_voxels(crm_ctx* ctx, uint32_t* voxel_of_agent, uint32_t* voxel_start) {
+    if (!ctx || !ctx->have_agents) return CRM_INVALID_ARGUMENT;
+    CRM_HIP(hipSetDevice(ctx->device));
+    if (voxel_of_agent) {
+        std::vector<uint32_t> id, vox(ctx->N);
+        CRM_TRY(download_ids(ctx, id));
+        CRM_HIP(hipMemcpyAsync(vox.data(), ctx->buf[ctx->cur].vox, ctx->N * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                               ctx->stream));
+        CRM_HIP(hipStreamSynchronize(ctx->stream));
+        for (uint64_t s = 0; s < ctx->N; s++) voxel_of_agent[id[s]] = vox[s];
+    }
+    if (voxel_start) {
+        CRM_HIP(hipMemcpyAsync(voxel_start, ctx->d_start, (ctx->Vt + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                               ctx->stream));
+        CRM_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    return CRM_OK;
+}
+
+crm_status crm_get_info(crm_ctx* ctx, crm_info* info) {
+    if (!ctx || !info) return CRM_INVALID_ARGUMENT;
+    std::memset(info, 0, sizeof(*info));
+    info->n_agents = ctx->N;
+    info->n_voxels_total = ctx->Vt;
+    info->step = ctx->step;
+    info->n_instances = ctx->n_inst;
+    info->n_vertices = ctx->nv;
+    info->dim = ctx->D;
+    if (ctx->have_agents && ctx->d_ctrl) {
+        Control c;
+        CRM_TRY(read_control(ctx, c));
+        if (c.error_flags) {
+            info->error_step_valid = 1;
+            info->error_step = c.error_key >> 32;
+            info->error_agent = c.error_key & 0xffffffffu;
+        }
+    }
+    return CRM_OK;
+}
+
+crm_status crm_get_timing(crm_ctx* ctx, crm_timing* timing) {
+    if (!ctx || !timing) return CRM_INVALID_ARGUMENT;
+    for (int i = 0; i < CRM_KERNEL_COUNT; i++) {
+        timing->total_ms[i] = ctx->t_ms[i];
+        timing->launches[i] = ctx->t_n[i];
+    }
+    return CRM_OK;
+}
+
+crm_status crm_reset_timing(crm_ctx* ctx) {
+    if (!ctx) return CRM_INVALID_ARGUMENT;
+    for (int i = 0; i < CRM_KERNEL_COUNT; i++) {
+        ctx->t_ms[i] = 0.0;
+        ctx->t_n[i] = 0;
+    }
+    return CRM_OK;
+}
+
+size_t crm_last_error(crm_ctx* ctx, char* buf, size_t len) {
+    const std::string& e = ctx ? ctx->err : g_create_error;
+    if (buf && len) {
+        size_t n = std::min(len - 1, e.size());
+        std::memcpy(buf, e.data(), n);
+        buf[n] = 0;
+    }
+    return e.size();
+}
+
+void crm_destroy(crm_ctx* ctx) {
+    if (!ctx) return;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    free_device(ctx);
+    if (ctx->d_ip) (void)hipFree(ctx->d_ip);
+    if (ctx->d_ctrl) (void)hipFree(ctx->d_ctrl);
+    if (ctx->ev_a) (void)hipEventDestroy(ctx->ev_a);
+    if (ctx->ev_b) (void)hipEventDestroy(ctx->ev_b);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+}  // extern "C"

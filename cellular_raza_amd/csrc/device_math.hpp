@@ -1,0 +1,279 @@
+// device_math.hpp -- per-pair and per-agent arithmetic shared by the gfx950 kernels.
+//
+// Every function restates one reference routine; the FP64 instantiations keep the
+// reference's operation order (the translation unit is compiled with
+// -ffp-contract=off), the float instantiations are the mixed-precision pair path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "params.hpp"
+
+namespace crm {
+
+template <typename T>
+__device__ __forceinline__ T dsqrt(T x);
+template <>
+__device__ __forceinline__ double dsqrt<double>(double x) { return sqrt(x); }
+template <>
+__device__ __forceinline__ float dsqrt<float>(float x) { return sqrtf(x); }
+
+template <typename T>
+__device__ __forceinline__ T dexp(T x);
+template <>
+__device__ __forceinline__ double dexp<double>(double x) { return exp(x); }
+template <>
+__device__ __forceinline__ float dexp<float>(float x) { return expf(x); }
+
+template <typename T>
+__device__ __forceinline__ T dpow(T x, T y);
+template <>
+__device__ __forceinline__ double dpow<double>(double x, double y) { return pow(x, y); }
+template <>
+__device__ __forceinline__ float dpow<float>(float x, float y) { return powf(x, y); }
+
+template <typename T>
+__device__ __forceinline__ T dmin(T a, T b);  // Rust f{32,64}::min: NaN-ignoring
+template <>
+__device__ __forceinline__ double dmin<double>(double a, double b) { return fmin(a, b); }
+template <>
+__device__ __forceinline__ float dmin<float>(float a, float b) { return fminf(a, b); }
+template <typename T>
+__device__ __forceinline__ T dmax(T a, T b);
+template <>
+__device__ __forceinline__ double dmax<double>(double a, double b) { return fmax(a, b); }
+template <>
+__device__ __forceinline__ float dmax<float>(float a, float b) { return fmaxf(a, b); }
+
+// x^e for a small non-negative integer e, by binary powering.
+template <typename T>
+__device__ __forceinline__ T ipow(T x, int e) {
+    T r = T(1);
+    T b = x;
+    while (e > 0) {
+        if (e & 1) r = r * b;
+        b = b * b;
+        e >>= 1;
+    }
+    return r;
+}
+
+template <typename T>
+__device__ __forceinline__ T powe(T x, T e, int ie) {
+    return ie >= 0 ? ipow<T>(x, ie) : dpow<T>(x, e);
+}
+
+// ---------------------------------------------------------------------------
+// Point potentials.  z = own - ext.  Writes the force on `own`; the force on
+// `ext` is its exact negation for every potential below (IEEE negation is exact
+// and the reference returns (dir * F, -dir * F)).  Returns true on CalcError.
+// ---------------------------------------------------------------------------
+
+// calculate_morse_interaction, interaction.rs:151-182
+template <typename T, int D>
+__device__ __forceinline__ void morse(const T* z, T two_radius, T cutoff, T strength, T stiffness, T* f) {
+    T d2 = T(0);
+#pragma unroll
+    for (int k = 0; k < D; k++) d2 = d2 + z[k] * z[k];
+    T dist = dsqrt<T>(d2);
+    if (dist > cutoff || dist == T(0)) {
+#pragma unroll
+        for (int k = 0; k < D; k++) f[k] = T(0);
+        return;
+    }
+    T e = dexp<T>((-stiffness) * (dist - two_radius));
+    T force = ((((T(-2)) * strength) * stiffness) * e) * (T(1) - e);
+#pragma unroll
+    for (int k = 0; k < D; k++) f[k] = (z[k] / dist) * force;
+}
+
+// MiePotential::calculate_force_between, interaction.rs:345-376.
+// sigma and the Mie constant are pair-independent and precomputed on the host
+// with the reference's expressions.
+template <typename T, int D>
+__device__ __forceinline__ bool mie(const T* z, const PotParams<T>& P, T* f) {
+    T d2 = T(0);
+#pragma unroll
+    for (int k = 0; k < D; k++) d2 = d2 + z[k] * z[k];
+    T r = dsqrt<T>(d2);
+    if (r == T(0)) {
+#pragma unroll
+        for (int k = 0; k < D; k++) f[k] = T(0);
+        return true;
+    }
+    if (r > P.cutoff) {
+#pragma unroll
+        for (int k = 0; k < D; k++) f[k] = T(0);
+        return false;
+    }
+    T sr = P.mie_sigma / r;
+    T part = P.en / P.mie_sigma * powe<T>(sr, P.en + T(1), P.ien1) -
+             P.em / P.mie_sigma * powe<T>(sr, P.em + T(1), P.iem1);
+    T force = dmin<T>(P.mie_constant * P.strength * part, P.bound);
+#pragma unroll
+    for (int k = 0; k < D; k++) f[k] = (z[k] / r) * force;
+    return false;
+}
+
+// BoundLennardJones::calculate_force_between, interaction.rs:122-138
+template <typename T, int D>
+__device__ __forceinline__ void bound_lj(const T* z, const PotParams<T>& P, T* f) {
+    T d2 = T(0);
+#pragma unroll
+    for (int k = 0; k < D; k++) d2 = d2 + z[k] * z[k];
+    T r = dsqrt<T>(d2);
+    T sr = P.sigma / r;
+    T val = T(4) * P.epsilon / r * (T(12) * ipow<T>(sr, 11) - T(6) * ipow<T>(sr, 5));
+    T mx = P.bound / r;
+    T q = (P.cutoff >= r) ? T(1) : T(0);
+    T m = dmin<T>(mx, val);
+#pragma unroll
+    for (int k = 0; k < D; k++) f[k] = ((-(z[k] / r)) * q) * m;
+}
+
+// CellSpecificInteraction, benchmarks/src/cell_sorting.rs:31-94 (sign +1) and
+// examples/cell_sorting/src/main.rs:32-103 (sign -1).  `same` = equal species.
+template <typename T, int D>
+__device__ __forceinline__ void cell_sorting(const T* z, bool same, const PotParams<T>& P, T* f) {
+    const T mrd = T(0.3162277660168379);
+    T d2 = T(0);
+#pragma unroll
+    for (int k = 0; k < D; k++) d2 = d2 + z[k] * z[k];
+    T nz = dsqrt<T>(d2);
+    T r;
+    if (!(nz < P.radius * mrd)) {
+        r = nz;
+    } else {
+        bool zero = true;
+#pragma unroll
+        for (int k = 0; k < D; k++) zero = zero && (z[k] == T(0));
+        if (zero) {
+#pragma unroll
+            for (int k = 0; k < D; k++) f[k] = T(0);
+            return;
+        }
+        r = P.radius * mrd;
+    }
+    T sigma = r / P.two_radius;
+    T bound = T(4) + T(1) / sigma;
+    T sgn_arg = P.rel_range * P.two_radius - r;
+    T signum = (sgn_arg != sgn_arg) ? sgn_arg : copysign(T(1), sgn_arg);
+    T spatial_cutoff = (T(1) + signum) * T(0.5);
+    T inv = T(1) / sigma;
+    T inv2 = inv * inv;
+    T strength = P.strength * dmax<T>(dmin<T>(inv2 - inv2 * inv2, bound), -bound);
+    T smax = dmax<T>(strength, T(0)), smin = dmin<T>(strength, T(0));
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+        T dir = z[k] / nz;
+        T att = (dir * smax) * spatial_cutoff;
+        T rep = (dir * smin) * spatial_cutoff;
+        T o = same ? rep + att : rep;
+        f[k] = P.sign < T(0) ? -o : o;
+    }
+}
+
+template <int POT, typename T, int D>
+__device__ __forceinline__ bool point_force(const T* z, int sp_own, int sp_ext, const PotParams<T>& P, T* f) {
+    if constexpr (POT == CRM_POTENTIAL_MORSE) {
+        morse<T, D>(z, P.two_radius, P.cutoff, P.strength, P.stiffness, f);
+        return false;
+    } else if constexpr (POT == CRM_POTENTIAL_MORSE_SPECIES) {
+        T s = P.strength * P.species_matrix[sp_own * CRM_MAX_SPECIES + sp_ext];
+        morse<T, D>(z, P.two_radius, P.cutoff, s, P.stiffness, f);
+        return false;
+    } else if constexpr (POT == CRM_POTENTIAL_MIE) {
+        return mie<T, D>(z, P, f);
+    } else if constexpr (POT == CRM_POTENTIAL_BOUND_LENNARD_JONES) {
+        bound_lj<T, D>(z, P, f);
+        return false;
+    } else if constexpr (POT == CRM_POTENTIAL_NONE) {
+#pragma unroll
+        for (int k = 0; k < D; k++) f[k] = T(0);
+        return false;
+    } else {
+        cell_sorting<T, D>(z, sp_own == sp_ext, P, f);
+        return false;
+    }
+}
+
+// nearest_point_from_point_to_line, interaction.rs:497-509 (t and the point x).
+template <typename T, int D>
+__device__ __forceinline__ void nearest_point(const T* p, const T* a, const T* b, T* x, T& t) {
+    T ab[D], ap[D];
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+        ab[k] = b[k] - a[k];
+        ap[k] = p[k] - a[k];
+    }
+    T num = T(0), den = T(0);
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+        num = num + ab[k] * ap[k];
+        den = den + ab[k] * ab[k];
+    }
+    T tt = num / den;
+    if (tt < T(0)) tt = T(0);
+    if (tt > T(1)) tt = T(1);
+    t = tt;
+#pragma unroll
+    for (int k = 0; k < D; k++) x[k] = a[k] * (T(1) - tt) + b[k] * tt;
+}
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) and Box-Muller normals.
+// Counter {agent_lo, agent_hi, step_lo, (step_hi << 16) | block}, key = seed.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        uint32_t lo0 = 0xD2511F53u * c[0];
+        uint32_t hi0 = __umulhi(0xD2511F53u, c[0]);
+        uint32_t lo1 = 0xCD9E8D57u * c[2];
+        uint32_t hi1 = __umulhi(0xCD9E8D57u, c[2]);
+        uint32_t n0 = hi1 ^ c[1] ^ k0;
+        uint32_t n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0;
+        c[1] = lo1;
+        c[2] = n2;
+        c[3] = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+// Writes the two normals of block b for (seed, agent, step).
+__device__ __forceinline__ void gaussian_pair(uint64_t seed, uint64_t agent, uint64_t step, uint32_t b,
+                                              double& z0, double& z1) {
+    uint32_t c[4] = {(uint32_t)agent, (uint32_t)(agent >> 32), (uint32_t)step,
+                     (uint32_t)(((step >> 32) << 16) | (uint64_t)b)};
+    philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    uint64_t a = ((uint64_t)c[1] << 32) | c[0];
+    uint64_t d = ((uint64_t)c[3] << 32) | c[2];
+    const double inv53 = 1.0 / 9007199254740992.0;
+    double u1 = (double)((a >> 11) + 1) * inv53;
+    double u2 = (double)(d >> 11) * inv53;
+    double rad = sqrt(-2.0 * log(u1));
+    double th = 6.283185307179586 * u2;
+    z0 = rad * cos(th);
+    z1 = rad * sin(th);
+}
+
+// ---------------------------------------------------------------------------
+// Grid helpers (cartesian_cuboid_n.rs:469-491, 594-631).
+// ---------------------------------------------------------------------------
+
+// num-traits to_usize on (p - min) / dx: ok iff -1 < x (NaN fails); trunc.
+__device__ __forceinline__ bool axis_index(double p, double mn, double dx, uint32_t n, uint32_t& out) {
+    double x = (p - mn) / dx;
+    if (!(x > -1.0)) return false;
+    if (!(x < 4294967296.0)) return false;
+    double t = trunc(x);
+    uint32_t i = t <= 0.0 ? 0u : (uint32_t)t;
+    if (i >= n) return false;  // no such voxel: the cell stays where it is (update_mechanics.rs:430-453)
+    out = i;
+    return true;
+}
+
+}  // namespace crm

@@ -1,0 +1,64 @@
+// params.hpp -- per-instance constants and the device control block.
+//
+// Plain structs copied once to device memory at crm_create.  Grids are stored
+// as 3-D with a leading extent of 1 for 2-D domains, so plain indices are the
+// same as the reference's lexicographic ones (datastructures.rs:113-120):
+// plain = (i0 * n1 + i1) * n2 + i2, and a 2-D point (x, y) sits on axes 1, 2.
+#pragma once
+
+#include <stdint.h>
+
+#include "crmech.h"
+
+namespace crm {
+
+template <typename T>
+struct PotParams {
+    T radius, two_radius, stiffness, cutoff, strength, bound;
+    T en, em, mie_sigma, mie_constant;
+    T epsilon, sigma;
+    T rel_range, sign;
+    T species_matrix[CRM_MAX_SPECIES * CRM_MAX_SPECIES];
+    int ien1, iem1;  // integer exponents en+1, em+1 when exact, else -1
+    int pad0, pad1;
+};
+
+struct InstParams {
+    // grid, 3-D layout (axis 0 has extent 1 in 2-D)
+    double min[3], max[3], dx[3];
+    float dxf[3];
+    uint32_t n[3];
+    uint32_t voxel_offset;  // first global voxel of this instance
+    uint32_t n_voxels;
+    uint64_t agent_offset;  // first agent slot of this instance (slots never change instance)
+    uint64_t n_agents;
+    PotParams<double> pd;
+    PotParams<float> pf;
+    // mechanics
+    double mass, damping, kb_temperature, diffusion_constant;
+    double spring_tension, rigidity, spring_length;
+    double inv_mass;        // 1 / mass (Langevin, mechanics.rs:457)
+    double langevin_coeff;  // sqrt(2 * damping * kT / mass) (mechanics.rs:439-445)
+    double brownian_q;      // diffusion / kT (mechanics.rs:352)
+    double brownian_coeff;  // sqrt(2 * diffusion) (mechanics.rs:341)
+};
+
+// Work unit of the agent-parallel kernels: a block of consecutive slots inside
+// one instance, so per-instance parameters are wave-uniform.
+struct BlockWork {
+    uint32_t inst;
+    uint32_t count;
+    uint64_t first;
+};
+
+// Device control block.  `step` is the number of completed steps; the step
+// kernels read it, so a captured graph replays correctly.
+struct Control {
+    uint64_t step;
+    uint64_t error_key;     // min over (step << 32 | agent index) of failing agents
+    uint32_t error_flags;   // bit (1 << crm_status) per error kind seen
+    uint32_t pad;
+    uint64_t out_of_grid;   // agents that kept their voxel because the new index had no voxel
+};
+
+}  // namespace crm

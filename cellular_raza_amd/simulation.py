@@ -1,0 +1,267 @@
+"""Host-side mirror of the reference's simulation entry points for the mechanics path.
+
+`MechanicsBackend` is one context of the C ABI (include/crmech.h): it owns the
+device buffers of one or more instances and advances them with the gfx950
+kernels.  `run_simulation` mirrors chili's run_simulation! for the
+[Mechanics, Interaction] aspects (cellular_raza-core-proc-macro/src/run_sim.rs:220-409)
+with `FixedStepsize` save points (cellular_raza-core/src/time.rs:93-265).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+from . import _abi
+from ._native import load
+from .building_blocks import CartesianCuboid, instance_struct
+
+
+class SimulationError(RuntimeError):
+    """A non-Ok status of the C ABI; `kind` is the reference error name."""
+
+    def __init__(self, status: int, message: str):
+        self.status = status
+        self.kind = _abi.STATUS_NAMES.get(status, f"status {status}")
+        super().__init__(f"{self.kind}: {message}")
+
+
+def _dp(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(_abi.DP)
+
+
+@dataclass
+class InstanceSpec:
+    domain: CartesianCuboid
+    interaction: object
+    mechanics: object
+
+
+class MechanicsBackend:
+    """One simulation context on one GPU (ensemble of >= 1 instances)."""
+
+    def __init__(
+        self,
+        instances: Sequence[InstanceSpec],
+        dt: float,
+        mechanics_solver_order: int = 2,
+        precision: str = "fp64",
+        weighting: str = "reference",
+        history_order: str = "oldest_first",
+        noise: str = "none",
+        seed: int = 0,
+        device: int = 0,
+        graph: bool = True,
+        timing: bool = False,
+    ):
+        self._lib = load()
+        if not instances:
+            raise ValueError("at least one instance")
+        dim = instances[0].domain.dim
+        self.dim = dim
+        self.n_instances = len(instances)
+        self.n_vertices = getattr(instances[0].mechanics, "n_vertices", 1)
+        cfg = _abi.Config()
+        cfg.dim = dim
+        cfg.solver_order = mechanics_solver_order
+        cfg.precision = _abi.PRECISION[precision]
+        cfg.weighting = _abi.WEIGHTING[weighting]
+        cfg.history_order = _abi.HISTORY_ORDER[history_order]
+        cfg.noise_mode = _abi.NOISE[noise]
+        cfg.seed = seed
+        cfg.dt = dt
+        cfg.device = device
+        cfg.flags = (_abi.FLAG_GRAPH if graph else 0) | (_abi.FLAG_TIMING if timing else 0)
+        arr = (_abi.Instance * len(instances))()
+        for i, spec in enumerate(instances):
+            if spec.domain.dim != dim:
+                raise ValueError("all instances must have the same dimension")
+            arr[i] = instance_struct(spec.domain, spec.interaction, spec.mechanics)
+        self._cfg = cfg
+        handle = C.c_void_p()
+        st = self._lib.crm_create(C.byref(cfg), arr, len(instances), C.byref(handle))
+        if st != _abi.CRM_OK:
+            raise SimulationError(st, self._message(None))
+        self._h = handle
+        self.n_agents = 0
+        self.counts: List[int] = []
+
+    # -- helpers -----------------------------------------------------------
+    def _message(self, h) -> str:
+        buf = C.create_string_buffer(1024)
+        self._lib.crm_last_error(h, buf, len(buf))
+        return buf.value.decode(errors="replace")
+
+    def _check(self, st: int) -> None:
+        if st != _abi.CRM_OK:
+            raise SimulationError(st, self._message(self._h))
+
+    @property
+    def K(self) -> int:
+        return self.n_vertices * self.dim
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.crm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- API ---------------------------------------------------------------
+    def set_agents(self, pos, vel=None, species=None, counts=None) -> None:
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        n = pos.shape[0]
+        pos = pos.reshape(n, self.K)
+        if counts is None:
+            if self.n_instances != 1:
+                raise ValueError("counts is required for several instances")
+            counts = [n]
+        counts_a = np.ascontiguousarray(counts, dtype=np.uint64)
+        if int(counts_a.sum()) != n:
+            raise ValueError("sum(counts) must equal the number of agents")
+        vel_a = None if vel is None else np.ascontiguousarray(vel, dtype=np.float64).reshape(n, self.K)
+        sp_a = None if species is None else np.ascontiguousarray(species, dtype=np.int32).reshape(n)
+        st = self._lib.crm_set_agents(
+            self._h,
+            counts_a.ctypes.data_as(_abi.U64P),
+            _dp(pos),
+            _dp(vel_a),
+            None if sp_a is None else sp_a.ctypes.data_as(_abi.I32P),
+        )
+        self._check(st)
+        self.n_agents = n
+        self.counts = [int(c) for c in counts_a]
+
+    def step(self, n_steps: int = 1, noise: Optional[np.ndarray] = None) -> None:
+        nz = None
+        if noise is not None:
+            nz = np.ascontiguousarray(noise, dtype=np.float64)
+            if nz.size != n_steps * self.n_agents * self.K:
+                raise ValueError("noise must hold n_steps * n_agents * n_vertices * dim values")
+        self._check(self._lib.crm_step(self._h, n_steps, _dp(nz)))
+
+    def compute_forces(self) -> np.ndarray:
+        out = np.empty((self.n_agents, self.K), dtype=np.float64)
+        self._check(self._lib.crm_compute_forces(self._h, _dp(out)))
+        return out
+
+    def get_state(self):
+        pos = np.empty((self.n_agents, self.K), dtype=np.float64)
+        vel = np.empty((self.n_agents, self.K), dtype=np.float64)
+        self._check(self._lib.crm_get_state(self._h, _dp(pos), _dp(vel)))
+        return pos, vel
+
+    def get_history(self):
+        dx = np.empty((2, self.n_agents, self.K), dtype=np.float64)
+        dv = np.empty((2, self.n_agents, self.K), dtype=np.float64)
+        n = C.c_uint32()
+        self._check(self._lib.crm_get_history(self._h, _dp(dx), _dp(dv), C.byref(n)))
+        return dx, dv, int(n.value)
+
+    def info(self) -> _abi.Info:
+        inf = _abi.Info()
+        self._check(self._lib.crm_get_info(self._h, C.byref(inf)))
+        return inf
+
+    def get_voxels(self):
+        inf = self.info()
+        vox = np.empty(self.n_agents, dtype=np.uint32)
+        start = np.empty(int(inf.n_voxels_total) + 1, dtype=np.uint32)
+        self._check(
+            self._lib.crm_get_voxels(self._h, vox.ctypes.data_as(_abi.U32P), start.ctypes.data_as(_abi.U32P))
+        )
+        return vox, start
+
+    def timing(self) -> Dict[str, Dict[str, float]]:
+        t = _abi.Timing()
+        self._check(self._lib.crm_get_timing(self._h, C.byref(t)))
+        return {
+            name: {"total_ms": t.total_ms[i], "launches": int(t.launches[i])}
+            for i, name in enumerate(_abi.KERNEL_CLASSES)
+        }
+
+    def reset_timing(self) -> None:
+        self._check(self._lib.crm_reset_timing(self._h))
+
+
+class FixedStepsize:
+    """FixedStepsize::from_partial_save_steps (cellular_raza-core/src/time.rs:93-111,174-235)."""
+
+    def __init__(self, t0: float, dt: float, save_iterations: List[int], maximum_iterations: int):
+        self.t0 = t0
+        self.dt = dt
+        self.save_iterations = save_iterations
+        self.maximum_iterations = maximum_iterations
+
+    @classmethod
+    def from_partial_save_steps(cls, t0: float, dt: float, n_steps: int, save_interval: int):
+        max_save_points = -(-n_steps // save_interval)
+        points = [t0 + float(save_interval * n) * dt for n in range(max_save_points + 1)]
+        return cls.from_partial_save_points(t0, dt, points)
+
+    @classmethod
+    def from_partial_save_points(cls, t0: float, dt: float, save_points: Sequence[float]):
+        pts = sorted(save_points)
+        if any(t0 > x for x in pts):
+            raise ValueError("TimeError: Evaluation time point is before starting time point.")
+        if not pts:
+            raise ValueError("TimeError: No savepoints specified.")
+        maximum = int(round((pts[-1] - t0) / dt))
+        iters = sorted({int(round((t - t0) / dt)) for t in pts})
+        return cls(t0, dt, iters, maximum)
+
+
+@dataclass
+class Settings:
+    """chili::Settings (cellular_raza-core/src/backend/chili/setup.rs:17-27); n_threads is
+    accepted for signature parity, the GPU runs the single-subdomain semantics."""
+
+    time: FixedStepsize
+    n_threads: int = 1
+    progressbar: Optional[str] = None
+
+
+def run_simulation(
+    domain: CartesianCuboid,
+    interaction,
+    mechanics,
+    positions,
+    settings: Settings,
+    velocities=None,
+    species=None,
+    mechanics_solver_order: int = 2,
+    **backend_kwargs,
+) -> Dict[int, np.ndarray]:
+    """Runs the [Mechanics, Interaction] aspects and returns {iteration: positions}
+    at the save iterations (iteration 0 is the initial state, run_sim.rs:363-367)."""
+    be = MechanicsBackend(
+        [InstanceSpec(domain, interaction, mechanics)],
+        dt=settings.time.dt,
+        mechanics_solver_order=mechanics_solver_order,
+        **backend_kwargs,
+    )
+    try:
+        be.set_agents(positions, velocities, species)
+        out: Dict[int, np.ndarray] = {}
+        it = 0
+        for target in settings.time.save_iterations:
+            target = min(target, settings.time.maximum_iterations)
+            if target > it:
+                be.step(target - it)
+                it = target
+            out[it] = be.get_state()[0]
+        return out
+    finally:
+        be.close()

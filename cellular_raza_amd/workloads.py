@@ -1,0 +1,121 @@
+"""Seeded synthetic populations for the five benchmark configurations.
+
+Every generator takes a scale argument so that the same construction serves the
+full-size benchmark and the small parity cases.  Positions are uniform on the
+half-open box [0, L) like the reference examples' random_range(0.0..L)
+(cellular_raza-examples/cell_sorting/src/main.rs:119-123), nudged off the
+domain minimum because the initial placement requires min < p strictly
+(cartesian_cuboid_n.rs:149-162,320-337).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+from typing import List, Optional
+
+import numpy as np
+
+from .building_blocks import (
+    CartesianCuboid,
+    Langevin,
+    MiePotential,
+    MorsePotential,
+    NewtonDamped,
+    SpeciesMorsePotential,
+)
+from .simulation import InstanceSpec
+
+
+@dataclass
+class Workload:
+    name: str
+    instances: List[InstanceSpec]
+    dt: float
+    steps: int
+    solver_order: int
+    pos: np.ndarray
+    vel: Optional[np.ndarray]
+    species: Optional[np.ndarray]
+    counts: List[int]
+    noise: str = "none"
+
+    @property
+    def n_agents(self) -> int:
+        return int(sum(self.counts))
+
+    @property
+    def dim(self) -> int:
+        return self.instances[0].domain.dim
+
+
+def _uniform_open(rng, n, dim, L):
+    p = rng.uniform(0.0, L, size=(n, dim))
+    tiny = np.nextafter(0.0, 1.0)
+    return np.where(p <= 0.0, tiny, p)
+
+
+def c1_morse_2d(n: int = 1000, seed: int = 1, steps: int = 2000) -> Workload:
+    """2-D damped-Newton point cells, Morse (R 6, lambda 1/1.5, s 0.1, cutoff 12), Euler, dt 0.01.
+    Box side scales with sqrt(n) to keep the density of the 1000-agent / 250^2 case."""
+    L = 250.0 * math.sqrt(n / 1000.0)
+    dom = CartesianCuboid.from_boundaries_and_interaction_range([0.0, 0.0], [L, L], 12.0)
+    pot = MorsePotential(radius=6.0, potential_stiffness=1.0 / 1.5, cutoff=12.0, strength=0.1)
+    mech = NewtonDamped(damping_constant=1.0, mass=1.0)
+    rng = np.random.default_rng(seed)
+    pos = _uniform_open(rng, n, 2, L)
+    return Workload("C1 2-D Morse NewtonDamped", [InstanceSpec(dom, pot, mech)], 0.01, steps, 0, pos,
+                    np.zeros_like(pos), None, [n])
+
+
+def c2_sorting_3d(n: int = 1_000_000, seed: int = 2, steps: int = 1000) -> Workload:
+    """3-D two-species cell sorting, Langevin (gamma 2, kT 0.01, m 1), species-Morse
+    (R 1, lambda 1, cutoff 2.5, strength 0.5 x [[1, .2], [.2, 1]]), volume fraction 0.3, AB3."""
+    L = (n * 4.0 * math.pi / 3.0 / 0.3) ** (1.0 / 3.0)
+    dom = CartesianCuboid.from_boundaries_and_interaction_range([0.0] * 3, [L] * 3, 2.5)
+    pot = SpeciesMorsePotential(radius=1.0, potential_stiffness=1.0, cutoff=2.5, strength=0.5,
+                                species_matrix=[[1.0, 0.2], [0.2, 1.0]])
+    mech = Langevin(mass=1.0, damping=2.0, kb_temperature=0.01)
+    rng = np.random.default_rng(seed)
+    pos = _uniform_open(rng, n, 3, L)
+    species = (rng.random(n) < 0.5).astype(np.int32)
+    return Workload("C2 3-D cell sorting Langevin", [InstanceSpec(dom, pot, mech)], 0.005, steps, 2, pos,
+                    np.zeros_like(pos), species, [n], noise="philox")
+
+
+def c4_ensemble_2d(n_instances: int = 512, seed: int = 4, steps: int = 1000, n_min: int = 500,
+                   n_max: int = 20000) -> Workload:
+    """Ensemble of 2-D Morse runs: N_i log-uniform in [n_min, n_max], strength U[0.05, 0.5],
+    damping U[0.5, 2.0], box side sqrt(N_i / 0.02); Morse R 6, lambda 1/1.5, cutoff 12; Euler."""
+    rng = np.random.default_rng(seed)
+    counts = np.round(np.exp(rng.uniform(math.log(n_min), math.log(n_max), n_instances))).astype(int)
+    strengths = rng.uniform(0.05, 0.5, n_instances)
+    dampings = rng.uniform(0.5, 2.0, n_instances)
+    inst, pos = [], []
+    for i in range(n_instances):
+        L = math.sqrt(counts[i] / 0.02)
+        dom = CartesianCuboid.from_boundaries_and_interaction_range([0.0, 0.0], [L, L], 12.0)
+        pot = MorsePotential(radius=6.0, potential_stiffness=1.0 / 1.5, cutoff=12.0, strength=float(strengths[i]))
+        mech = NewtonDamped(damping_constant=float(dampings[i]), mass=1.0)
+        inst.append(InstanceSpec(dom, pot, mech))
+        pos.append(_uniform_open(rng, int(counts[i]), 2, L))
+    P = np.concatenate(pos)
+    return Workload("C4 2-D Morse ensemble", inst, 0.01, steps, 0, P, np.zeros_like(P), None,
+                    [int(c) for c in counts])
+
+
+def c5_mie_3d(n: int = 4_000_000, seed: int = 5, steps: int = 500, spacing: float = 2.0,
+              bound: float = 1000.0) -> Workload:
+    """3-D Mie (n 12, m 6, R 1, eps 1, cutoff 2.5) on a jittered cubic lattice (jitter
+    U[-0.2a, 0.2a]), NewtonDamped (gamma 1, m 1), dt 0.001, AB3."""
+    side = int(math.ceil(n ** (1.0 / 3.0)))
+    a = spacing
+    idx = np.arange(n)
+    ijk = np.stack([idx // (side * side), (idx // side) % side, idx % side], axis=1).astype(np.float64)
+    rng = np.random.default_rng(seed)
+    pos = (ijk + 0.5) * a + rng.uniform(-0.2 * a, 0.2 * a, size=(n, 3))
+    L = side * a
+    dom = CartesianCuboid.from_boundaries_and_interaction_range([0.0] * 3, [L] * 3, 2.5)
+    pot = MiePotential(radius=1.0, strength=1.0, bound=bound, cutoff=2.5, en=12.0, em=6.0)
+    mech = NewtonDamped(damping_constant=1.0, mass=1.0)
+    return Workload("C5 3-D Mie lattice", [InstanceSpec(dom, pot, mech)], 0.001, steps, 2, pos,
+                    np.zeros_like(pos), None, [n])

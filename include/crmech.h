@@ -1,0 +1,255 @@
+/*
+ * crmech.h -- C ABI of the MI355X (gfx950) mechanics backend for cellular_raza.
+ *
+ * The library advances agent-based mechanics simulations on one GPU:
+ *   voxel binning on a CartesianCuboid grid (integer counting sort by voxel),
+ *   neighbour search over the 3^D voxel stencil,
+ *   the pairwise Interaction::calculate_force_between loop,
+ *   the Mechanics update (NewtonDamped / Langevin / Brownian / rods) with the
+ *   chili Euler / Adams-Bashforth steppers, reflecting boundary, re-binning.
+ *
+ * Reference interfaces this ABI replaces (paths relative to the cellular_raza
+ * workspace):
+ *   - the per-step hook functions that run_simulation! calls,
+ *     cellular_raza-core-proc-macro/src/run_sim.rs:152-157,251-256
+ *     (update_mechanics_interaction_step_{1,2,3}), 259-270 (local mechanics
+ *     update), 293-297 (apply_boundary, sort_cells_in_voxels_step_{1,2});
+ *   - the native-backend FFI precedent cellular_raza-core/src/backend/cara/mod.rs:5-62
+ *     (opaque runner, extern "C" create/drop/update) and its build.rs:7-19.
+ *
+ * Conventions
+ *   - All host arrays are caller-owned and copied; the context owns device memory.
+ *   - "Agent index" is the position of an agent in the arrays passed to
+ *     crm_set_agents (instances concatenated in order).  Every array the library
+ *     returns is in agent-index order.
+ *   - Vectors are row-major: agent a, vertex v, component k at
+ *     ((a * n_vertices) + v) * dim + k.  Point agents have n_vertices == 1.
+ *   - Calls are synchronous with respect to the context's HIP stream.  One
+ *     context must not be used from two host threads at once.
+ *   - Status codes mirror cellular_raza-concepts/src/errors.rs:4-25,63-101 and
+ *     cellular_raza-core/src/backend/chili/errors.rs:73-121.
+ */
+#ifndef CRMECH_H
+#define CRMECH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CRM_MAX_DIM 3
+#define CRM_MAX_SPECIES 4
+#define CRM_MAX_ROD_VERTICES 16
+
+typedef enum crm_status {
+    CRM_OK = 0,
+    CRM_CALC_ERROR = 1,       /* CalcError      (errors.rs: e.g. Mie at r == 0, interaction.rs:355-359) */
+    CRM_BOUNDARY_ERROR = 2,   /* BoundaryError  (cartesian_cuboid_n.rs:149-162,615-625) */
+    CRM_INDEX_ERROR = 3,      /* IndexError     (update_mechanics.rs:433-451; initial decompose) */
+    CRM_RNG_ERROR = 4,        /* RngError       (mechanics.rs:211-226) */
+    CRM_SETUP_ERROR = 5,      /* DecomposeError / TimeError during setup */
+    CRM_DEVICE_ERROR = 6,     /* HIP runtime failure */
+    CRM_INVALID_ARGUMENT = 7  /* caller error: bad sizes, null pointers, unsupported combination */
+} crm_status;
+
+/* Interaction kinds (cellular_raza-building-blocks/src/cell_building_blocks/interaction.rs). */
+typedef enum crm_potential_kind {
+    CRM_POTENTIAL_MORSE = 0,              /* MorsePotential, interaction.rs:151-299 */
+    CRM_POTENTIAL_MIE = 1,                /* MiePotential, interaction.rs:301-434 */
+    CRM_POTENTIAL_BOUND_LENNARD_JONES = 2,/* BoundLennardJones, interaction.rs:84-147 */
+    CRM_POTENTIAL_MORSE_SPECIES = 3,      /* Morse with strength * species_matrix[own][ext];
+                                             composite, no single reference type */
+    CRM_POTENTIAL_CELL_SORTING = 4,       /* CellSpecificInteraction of
+                                             cellular_raza-benchmarks/src/cell_sorting.rs:31-94 (sign = +1)
+                                             and cellular_raza-examples/cell_sorting/src/main.rs:32-103 (sign = -1) */
+    CRM_POTENTIAL_NONE = 5                /* NoInteraction, interaction.rs:12-35 (Mechanics-only runs) */
+} crm_potential_kind;
+
+/* Mechanics kinds (cellular_raza-building-blocks/src/cell_building_blocks/mechanics.rs,
+ * bacterial_rods.rs). */
+typedef enum crm_mechanics_kind {
+    CRM_MECHANICS_NEWTON_DAMPED = 0,  /* NewtonDamped{2,3}D, mechanics.rs:14-204 */
+    CRM_MECHANICS_LANGEVIN = 1,       /* Langevin{2,3}D, mechanics.rs:389-571 */
+    CRM_MECHANICS_BROWNIAN = 2,       /* Brownian{2,3}D, mechanics.rs:228-387 */
+    CRM_MECHANICS_ROD = 3             /* RodMechanics + RodInteraction, bacterial_rods.rs:82-337 */
+} crm_mechanics_kind;
+
+typedef enum crm_precision {
+    CRM_PRECISION_FP64 = 0,   /* everything in double */
+    CRM_PRECISION_MIXED = 1   /* float pair forces on voxel-relative coordinates with
+                                 compensated per-agent sums; double state and integrator */
+} crm_precision;
+
+typedef enum crm_weighting {
+    /* chili with one subdomain: same-voxel pairs get 1/2 of each side's evaluation,
+     * adjacent-voxel pairs only 1/2 of the own evaluation
+     * (update_mechanics.rs:178-220, 63-109). */
+    CRM_WEIGHT_REFERENCE = 0,
+    /* every pair in the stencil weighted like a same-voxel pair (full force),
+     * which is what chili gives pairs that straddle two subdomains (update_mechanics.rs:285-382). */
+    CRM_WEIGHT_PHYSICAL = 1
+} crm_weighting;
+
+typedef enum crm_history_order {
+    CRM_HISTORY_OLDEST_FIRST = 0,  /* ring-buffer iteration oldest first (aux_storage.rs:859-877) */
+    CRM_HISTORY_NEWEST_FIRST = 1
+} crm_history_order;
+
+typedef enum crm_noise_mode {
+    CRM_NOISE_NONE = 0,     /* deterministic mechanics only (noise terms are zero) */
+    CRM_NOISE_BUFFER = 1,   /* caller passes standard normals per step, by agent index */
+    CRM_NOISE_PHILOX = 2    /* on-device Philox4x32-10 + Box-Muller keyed by (seed, agent index, step) */
+} crm_noise_mode;
+
+enum {
+    CRM_FLAG_GRAPH = 1u,    /* replay the per-step launch sequence as a captured hipGraph */
+    CRM_FLAG_TIMING = 2u    /* record HIP events around every kernel (disables graphs) */
+};
+
+/* CartesianCuboid, cartesian_cuboid_n.rs:65-72,183-256.
+ * n_voxels[0] == 0 selects from_boundaries_and_interaction_range, otherwise
+ * from_boundaries_and_n_voxels. */
+typedef struct crm_domain {
+    double min[3];
+    double max[3];
+    double interaction_range;
+    uint32_t n_voxels[3];
+} crm_domain;
+
+typedef struct crm_potential {
+    uint32_t kind;                    /* crm_potential_kind */
+    uint32_t n_species;               /* MORSE_SPECIES / CELL_SORTING: species are 0..n_species-1 */
+    double radius;                    /* Morse / Mie radius; CellSpecificInteraction cell_radius */
+    double potential_stiffness;       /* Morse lambda */
+    double cutoff;                    /* Morse / Mie / BoundLennardJones cutoff */
+    double strength;                  /* Morse V0, Mie epsilon ("strength"), CellSorting potential_strength */
+    double bound;                     /* Mie / BoundLennardJones bound */
+    double en, em;                    /* Mie exponents */
+    double epsilon, sigma;            /* BoundLennardJones */
+    double relative_interaction_range;/* CellSorting */
+    double sign;                      /* CellSorting: +1 benchmark variant, -1 example variant */
+    double species_matrix[CRM_MAX_SPECIES * CRM_MAX_SPECIES]; /* MORSE_SPECIES: row = own species */
+} crm_potential;
+
+typedef struct crm_mechanics {
+    uint32_t kind;             /* crm_mechanics_kind */
+    uint32_t n_vertices;       /* rods: vertices per rod (2..16); points: 1 */
+    double mass;               /* NewtonDamped / Langevin */
+    double damping;            /* NewtonDamped damping_constant, Langevin damping, Rod damping */
+    double kb_temperature;     /* Langevin / Brownian */
+    double diffusion_constant; /* Brownian / Rod */
+    double spring_tension;     /* Rod */
+    double rigidity;           /* Rod */
+    double spring_length;      /* Rod */
+} crm_mechanics;
+
+/* One independent simulation.  Several instances in one context advance together
+ * (ensemble batching); they share dim, kinds, dt and solver, and may differ in
+ * domain and parameters. */
+typedef struct crm_instance {
+    crm_domain domain;
+    crm_potential potential;
+    crm_mechanics mechanics;
+} crm_instance;
+
+typedef struct crm_config {
+    uint32_t dim;            /* 2 or 3 */
+    uint32_t solver_order;   /* run_simulation! mechanics_solver_order: 0 Euler (solvers.rs:23-55),
+                                1 mechanics_adams_bashforth_2 (247-300), 2 mechanics_adams_bashforth_3
+                                (156-231, the default, run_sim.rs:92) */
+    uint32_t precision;      /* crm_precision */
+    uint32_t weighting;      /* crm_weighting */
+    uint32_t history_order;  /* crm_history_order */
+    uint32_t noise_mode;     /* crm_noise_mode */
+    uint64_t seed;
+    double dt;
+    int32_t device;
+    uint32_t flags;          /* CRM_FLAG_* */
+} crm_config;
+
+typedef struct crm_info {
+    uint64_t n_agents;
+    uint64_t n_voxels_total;     /* sum over instances */
+    uint64_t step;               /* steps executed so far */
+    uint32_t n_instances;
+    uint32_t n_vertices;
+    uint32_t dim;
+    uint32_t error_step_valid;
+    uint64_t error_step;         /* first step that raised an error */
+    uint64_t error_agent;        /* agent index that raised it */
+} crm_info;
+
+/* Kernel classes reported by crm_get_timing, in this order. */
+enum {
+    CRM_KERNEL_FORCE = 0,
+    CRM_KERNEL_INTEGRATE = 1,
+    CRM_KERNEL_SCAN = 2,
+    CRM_KERNEL_SCATTER = 3,
+    CRM_KERNEL_FIXUP = 4,
+    CRM_KERNEL_GATHER = 5,
+    CRM_KERNEL_COUNT = 6
+};
+
+typedef struct crm_timing {
+    double total_ms[CRM_KERNEL_COUNT];  /* summed HIP-event time per kernel class */
+    uint64_t launches[CRM_KERNEL_COUNT];
+} crm_timing;
+
+typedef struct crm_ctx crm_ctx;
+
+/* Grid of a CartesianCuboid (cartesian_cuboid_n.rs:183-256); no device work. */
+crm_status crm_grid(const crm_domain* domain, uint32_t dim, uint64_t n_voxels_out[3], double dx_out[3]);
+
+/* Creates a context on cfg->device.  Replaces construct_simulation_runner
+ * (chili/datastructures.rs:51-217) for the mechanics aspects.  On failure *out
+ * is NULL and crm_last_error(NULL, ...) holds the reason. */
+crm_status crm_create(const crm_config* cfg, const crm_instance* instances, uint32_t n_instances,
+                      crm_ctx** out);
+
+/* Uploads agents and bins them (Domain::decompose + insert_initial_cells,
+ * cellular_raza-concepts-derive/src/domain.rs:292-421, datastructures.rs:278-299).
+ * counts[i] agents belong to instance i.  pos, vel: sum(counts) * n_vertices * dim doubles.
+ * vel may be NULL (zero).  species may be NULL (all 0).  Resets the step counter
+ * and the Adams-Bashforth history. */
+crm_status crm_set_agents(crm_ctx* ctx, const uint64_t* counts, const double* pos, const double* vel,
+                          const int32_t* species);
+
+/* Advances n_steps steps: forces -> integrator -> boundary -> re-binning
+ * (run_sim.rs:380-403).  noise: CRM_NOISE_BUFFER only, n_steps * n_agents *
+ * n_vertices * dim standard normals, step-major, by agent index. */
+crm_status crm_step(crm_ctx* ctx, uint32_t n_steps, const double* noise);
+
+/* Interaction force accumulated for the current state, by agent index
+ * (AuxStorageMechanics::current_force before integration, aux_storage.rs:41-130).
+ * Does not advance the simulation. */
+crm_status crm_compute_forces(crm_ctx* ctx, double* force);
+
+crm_status crm_get_state(crm_ctx* ctx, double* pos, double* vel);
+
+/* Adams-Bashforth ring contents, oldest slot first: dx_hist, dv_hist each
+ * 2 * n_agents * n_vertices * dim doubles (slot-major); n_stored in {0,1,2}. */
+crm_status crm_get_history(crm_ctx* ctx, double* dx_hist, double* dv_hist, uint32_t* n_stored);
+
+/* Voxel of every agent (instance voxel offset + plain index, plain index as in
+ * datastructures.rs:113-120) and the CSR offsets of the sorted agent list
+ * (n_voxels_total + 1 entries); either pointer may be NULL. */
+crm_status crm_get_voxels(crm_ctx* ctx, uint32_t* voxel_of_agent, uint32_t* voxel_start);
+
+crm_status crm_get_info(crm_ctx* ctx, crm_info* info);
+
+crm_status crm_get_timing(crm_ctx* ctx, crm_timing* timing);
+crm_status crm_reset_timing(crm_ctx* ctx);
+
+/* Copies the last error message (NUL-terminated) and returns its full length.
+ * ctx == NULL reports why the last crm_create on this thread failed. */
+size_t crm_last_error(crm_ctx* ctx, char* buf, size_t len);
+
+void crm_destroy(crm_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CRMECH_H */

@@ -1,0 +1,315 @@
+"""Parity of the gfx950 library (through the C ABI) against the CPU oracle.
+
+Tolerances (FP64 mode), from the specification of this backend:
+  per-step forces and positions   <= 1e-12 relative (max over agents of |dF| / max |F|)
+  100-step trajectories           <= 1e-9 relative
+  voxel assignment, neighbour sets bit-exact
+Mixed precision: <= 1e-5 relative force error against the FP64 oracle.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from cellular_raza_amd import (
+    BoundLennardJones,
+    Brownian,
+    CartesianCuboid,
+    CellSpecificInteraction,
+    InstanceSpec,
+    Langevin,
+    MechanicsBackend,
+    MiePotential,
+    MorsePotential,
+    NewtonDamped,
+    NoInteraction,
+    SimulationError,
+    SpeciesMorsePotential,
+    workloads,
+)
+from oracle.oracle import OracleError, OracleSim
+
+pytestmark = pytest.mark.gpu
+
+FORCE_TOL = 1e-12
+TRAJ_TOL = 1e-9
+MIXED_TOL = 1e-5
+
+
+def rel_err(a, b):
+    """max_i |a_i - b_i| / max_i |b_i| over agent vectors (norm-wise)."""
+    a = np.asarray(a).reshape(a.shape[0], -1)
+    b = np.asarray(b).reshape(b.shape[0], -1)
+    num = np.linalg.norm(a - b, axis=1).max()
+    den = np.linalg.norm(b, axis=1).max()
+    return num / den if den > 0 else num
+
+
+def pair(wl, **kw):
+    opts = dict(mechanics_solver_order=wl.solver_order)
+    g_kw = dict(opts)
+    g_kw.update({k: v for k, v in kw.items()})
+    o_kw = dict(opts)
+    o_kw.update({k: v for k, v in kw.items() if k in ("weighting", "history_order", "noise", "seed")})
+    g = MechanicsBackend(wl.instances, dt=wl.dt, **g_kw)
+    o = OracleSim(wl.instances, dt=wl.dt, **o_kw)
+    g.set_agents(wl.pos, wl.vel, wl.species, wl.counts)
+    o.set_agents(wl.pos, wl.vel, wl.species, wl.counts)
+    return g, o
+
+
+def voxel_reference(domain, pos):
+    """Voxel plain index by the reference formula trunc((p - min) / dx), computed in numpy."""
+    n = domain.get_n_voxels()
+    dx = domain.get_dx()
+    idx = [np.trunc((pos[:, k] - domain.min[k]) / dx[k]).astype(np.int64) for k in range(domain.dim)]
+    plain = np.zeros(len(pos), dtype=np.int64)
+    for k in range(domain.dim):
+        plain = plain * n[k] + idx[k]
+    return plain
+
+
+def small_workloads():
+    rng = np.random.default_rng(11)
+    out = {}
+    out["morse2d"] = workloads.c1_morse_2d(n=1500, seed=3, steps=50)
+    out["species3d"] = workloads.c2_sorting_3d(n=4000, seed=4, steps=50)
+    out["mie3d"] = workloads.c5_mie_3d(n=3000, seed=5, steps=50)
+    # bounded Lennard-Jones on a jittered lattice
+    w = workloads.c5_mie_3d(n=2000, seed=6, steps=20)
+    w.instances = [InstanceSpec(w.instances[0].domain, BoundLennardJones(epsilon=0.5, sigma=1.6, bound=5.0,
+                                                                         cutoff=2.5), w.instances[0].mechanics)]
+    out["blj3d"] = w
+    # cell-sorting interaction of the benchmark (sign +1) and of the example (sign -1)
+    for variant in ("benchmark", "example"):
+        n = 3000
+        L = 60.0
+        dom = CartesianCuboid.from_boundaries_and_interaction_range([0.0] * 3, [L] * 3, 1.5 * 6.0 * 2.0 / 4)
+        pot = CellSpecificInteraction(cell_radius=1.5, potential_strength=2.0, relative_interaction_range=1.5,
+                                      variant=variant)
+        pos = rng.uniform(1e-9, L, size=(n, 3))
+        sp = (rng.random(n) < 0.5).astype(np.int32)
+        out["sorting_" + variant] = workloads.Workload(
+            "cell sorting " + variant, [InstanceSpec(dom, pot, NewtonDamped(damping_constant=2.0, mass=1.0))],
+            0.01, 20, 2, pos, np.zeros_like(pos), sp, [n])
+    return out
+
+
+WL = small_workloads()
+
+
+@pytest.mark.parametrize("name", sorted(WL))
+@pytest.mark.parametrize("weighting", ["reference", "physical"])
+def test_forces_match_oracle(name, weighting):
+    wl = WL[name]
+    g, o = pair(wl, weighting=weighting)
+    fg, fo = g.compute_forces(), o.compute_forces()
+    assert np.abs(fo).max() > 0
+    assert rel_err(fg, fo) <= FORCE_TOL, (name, rel_err(fg, fo))
+
+
+@pytest.mark.parametrize("name", ["morse2d", "species3d", "mie3d"])
+def test_step_and_trajectory(name):
+    wl = WL[name]
+    noise = "philox" if wl.noise == "philox" else "none"
+    g, o = pair(wl, noise=noise, seed=7)
+    g.step(1)
+    o.step(1)
+    pg, vg = g.get_state()
+    po, vo = o.get_state()
+    assert rel_err(pg, po) <= FORCE_TOL
+    assert rel_err(vg, vo) <= FORCE_TOL * 10
+    g.step(99)
+    o.step(99)
+    pg, vg = g.get_state()
+    po, vo = o.get_state()
+    assert rel_err(pg, po) <= TRAJ_TOL, rel_err(pg, po)
+    assert rel_err(vg, vo) <= TRAJ_TOL * 100, rel_err(vg, vo)
+
+
+def test_noise_buffer_matches_oracle():
+    wl = workloads.c2_sorting_3d(n=3000, seed=8, steps=10)
+    g, o = pair(wl, noise="buffer")
+    rng = np.random.default_rng(1)
+    nz = rng.standard_normal((10, wl.n_agents, 3))
+    g.step(10, nz)
+    o.step(10, nz)
+    assert rel_err(g.get_state()[0], o.get_state()[0]) <= TRAJ_TOL
+
+
+@pytest.mark.parametrize("order,hist", [(0, "oldest_first"), (1, "oldest_first"), (2, "oldest_first"),
+                                        (2, "newest_first")])
+def test_solver_orders_and_history(order, hist):
+    wl = workloads.c5_mie_3d(n=1500, seed=9, steps=20)
+    wl.solver_order = order
+    g, o = pair(wl, history_order=hist)
+    for k in range(4):
+        dg, vg_, ng = g.get_history()
+        do, vo_, no = o.get_history()
+        assert ng == no == min(k, order)
+        for s in range(ng):
+            assert rel_err(dg[s], do[s]) <= FORCE_TOL and rel_err(vg_[s], vo_[s]) <= FORCE_TOL
+        g.step(1)
+        o.step(1)
+        assert rel_err(g.get_state()[0], o.get_state()[0]) <= FORCE_TOL
+
+
+def test_voxels_and_neighbour_sets_bit_exact():
+    wl = workloads.c2_sorting_3d(n=5000, seed=12, steps=10)
+    g, o = pair(wl, noise="philox", seed=3)
+    dom = wl.instances[0].domain
+    for _ in range(3):
+        vg, sg = g.get_voxels()
+        vo, so, order = o.get_voxels()
+        pg = g.get_state()[0]
+        # binning of the GPU positions by the reference formula, computed independently
+        assert np.array_equal(vg.astype(np.int64), voxel_reference(dom, pg))
+        # same agent sets per voxel as the oracle (order inside a voxel may differ)
+        assert np.array_equal(vg, vo)
+        assert np.array_equal(sg, so)
+        assert sg[-1] == wl.n_agents and np.all(np.diff(sg.astype(np.int64)) >= 0)
+        g.step(3)
+        o.step(3)
+
+
+def test_mixed_precision_force_error():
+    wl = workloads.c2_sorting_3d(n=20000, seed=13, steps=10)
+    g, o = pair(wl, precision="mixed")
+    err = rel_err(g.compute_forces(), o.compute_forces())
+    assert err <= MIXED_TOL, err
+    wl5 = workloads.c5_mie_3d(n=8000, seed=14, steps=10)
+    g5, o5 = pair(wl5, precision="mixed")
+    err5 = rel_err(g5.compute_forces(), o5.compute_forces())
+    assert err5 <= MIXED_TOL, err5
+
+
+def test_ensemble_matches_oracle():
+    wl = workloads.c4_ensemble_2d(n_instances=12, seed=15, steps=20, n_min=50, n_max=800)
+    g, o = pair(wl)
+    assert rel_err(g.compute_forces(), o.compute_forces()) <= FORCE_TOL
+    g.step(20)
+    o.step(20)
+    assert rel_err(g.get_state()[0], o.get_state()[0]) <= TRAJ_TOL
+
+
+def test_graph_and_eager_bitwise_equal_and_deterministic():
+    wl = workloads.c2_sorting_3d(n=6000, seed=16, steps=10)
+    runs = []
+    for graph in (True, False, True):
+        g = MechanicsBackend(wl.instances, dt=wl.dt, mechanics_solver_order=2, noise="philox", seed=5,
+                             precision="mixed", graph=graph)
+        g.set_agents(wl.pos, wl.vel, wl.species)
+        g.step(7)
+        runs.append(g.get_state())
+        g.close()
+    for r in runs[1:]:
+        assert np.array_equal(r[0], runs[0][0]) and np.array_equal(r[1], runs[0][1])
+
+
+def test_initial_placement_errors():
+    dom = CartesianCuboid.from_boundaries_and_interaction_range([0.0, 0.0], [10.0, 10.0], 2.0)
+    spec = [InstanceSpec(dom, MorsePotential(1.0, 1.0, 2.0, 1.0), NewtonDamped(1.0, 1.0))]
+    g = MechanicsBackend(spec, dt=0.1)
+    with pytest.raises(SimulationError) as e:  # p == min fails the strict check
+        g.set_agents(np.array([[0.0, 5.0], [3.0, 3.0]]))
+    assert e.value.kind == "BoundaryError"
+    with pytest.raises(SimulationError) as e:  # p == max has no voxel
+        g.set_agents(np.array([[10.0, 5.0], [3.0, 3.0]]))
+    assert e.value.kind == "IndexError"
+    g.set_agents(np.array([[1.0, 5.0], [3.0, 3.0]]))
+
+
+def test_boundary_error_same_step_as_oracle():
+    dom = CartesianCuboid.from_boundaries_and_interaction_range([0.0, 0.0], [10.0, 10.0], 2.0)
+    spec = [InstanceSpec(dom, MorsePotential(1.0, 1.0, 2.0, 1.0), NewtonDamped(0.0, 1.0))]
+    pos = np.array([[5.0, 5.0], [1.0, 1.0]])
+    vel = np.array([[0.0, 0.0], [-30.0, 0.0]])  # after reflection still outside
+    g = MechanicsBackend(spec, dt=0.1, mechanics_solver_order=0)
+    o = OracleSim(spec, dt=0.1, mechanics_solver_order=0)
+    g.set_agents(pos, vel)
+    o.set_agents(pos, vel)
+    with pytest.raises(SimulationError) as e:
+        g.step(3)
+    with pytest.raises(OracleError) as eo:
+        o.step(3)
+    assert e.value.kind == eo.value.kind == "BoundaryError"
+    inf = g.info()
+    assert inf.error_step_valid and inf.error_step == 0 and inf.error_agent == 1
+
+
+def test_mie_identical_positions_calc_error():
+    dom = CartesianCuboid.from_boundaries_and_interaction_range([0.0] * 3, [10.0] * 3, 2.5)
+    spec = [InstanceSpec(dom, MiePotential(1.0, 1.0, 100.0, 2.5, 12.0, 6.0), NewtonDamped(1.0, 1.0))]
+    g = MechanicsBackend(spec, dt=0.01)
+    g.set_agents(np.array([[5.0, 5.0, 5.0], [5.0, 5.0, 5.0], [1.0, 1.0, 1.0]]))
+    with pytest.raises(SimulationError) as e:
+        g.compute_forces()
+    assert e.value.kind == "CalcError"
+
+
+def _msd(pos, x0):
+    d2 = ((pos - x0) ** 2).sum(axis=1)
+    return d2.mean(), d2.std()
+
+
+@pytest.mark.parametrize("dim,damping,seed", [(3, 10.0, 1), (3, 1.0, 4), (3, 0.1, 5), (2, 10.0, 6)])
+def test_langevin_msd_statistics(dim, damping, seed):
+    """brownian_diffusion_constant_approx.rs:260-283: 160 particles from the centre,
+    dt 1e-3, 200 steps, MSD within 3.5 standard deviations of the closed form."""
+    L, n, dt, steps, D = 200.0, 160, 1e-3, 200, 1.0
+    dom = CartesianCuboid.from_boundaries_and_n_voxels([0.0] * dim, [L] * dim, [3] * dim)
+    kT = D * damping * 1.0
+    spec = [InstanceSpec(dom, NoInteraction(), Langevin(mass=1.0, damping=damping, kb_temperature=kT))]
+    g = MechanicsBackend(spec, dt=dt, noise="philox", seed=seed)
+    x0 = np.full(dim, L / 2)
+    g.set_agents(np.tile(x0, (n, 1)))
+    for it in range(1, steps + 1):
+        g.step(1)
+        if it > 1 and it % 20 == 0:
+            t = it * dt
+            expected = (-dim * kT / damping ** 2 * (1 - math.exp(-damping * t)) * (3 - math.exp(-damping * t))
+                        + 2 * dim * kT * t / damping)
+            mean, sd = _msd(g.get_state()[0], x0)
+            assert abs(expected - mean) <= 3.5 * sd
+
+
+@pytest.mark.parametrize("dim,D,seed", [(3, 1.0, 1), (3, 0.5, 2), (2, 0.25, 3)])
+def test_brownian_msd_statistics(dim, D, seed):
+    """brownian_diffusion_constant_approx.rs:109-123: MSD = 2 d D t within 3.5 sigma."""
+    L, n, dt, steps = 200.0, 160, 1e-3, 200
+    dom = CartesianCuboid.from_boundaries_and_n_voxels([0.0] * dim, [L] * dim, [3] * dim)
+    spec = [InstanceSpec(dom, NoInteraction(), Brownian(diffusion_constant=D, kb_temperature=1.0))]
+    g = MechanicsBackend(spec, dt=dt, noise="philox", seed=seed)
+    x0 = np.full(dim, L / 2)
+    g.set_agents(np.tile(x0, (n, 1)))
+    for it in range(20, steps + 1, 20):
+        g.step(20)
+        mean, sd = _msd(g.get_state()[0], x0)
+        assert abs(2 * dim * D * it * dt - mean) <= 3.5 * sd
+
+
+def test_free_drift_closed_form():
+    """chili/mod.rs:134-156: no forces, no damping: x = x0 + k * dt * v0."""
+    dom = CartesianCuboid.from_boundaries_and_interaction_range([-100.0] * 3, [100.0] * 3, 10.0)
+    spec = [InstanceSpec(dom, NoInteraction(), NewtonDamped(damping_constant=0.0, mass=1.0))]
+    g = MechanicsBackend(spec, dt=0.1)
+    x0 = np.array([[1.0, 2.0, 3.0], [-4.0, 5.0, -6.0]])
+    v0 = np.array([[0.5, -0.25, 1.0], [2.0, 1.0, -0.5]])
+    g.set_agents(x0, v0)
+    g.step(10)
+    assert np.abs(g.get_state()[0] - (x0 + 10 * 0.1 * v0)).max() < 1e-5
+
+
+def test_full_size_c2_invariants():
+    """1M-agent cell-sorting configuration: size-independent checks after a few steps."""
+    wl = workloads.c2_sorting_3d(n=1_000_000, seed=2, steps=3)
+    g = MechanicsBackend(wl.instances, dt=wl.dt, mechanics_solver_order=2, noise="philox", seed=1,
+                         precision="mixed")
+    g.set_agents(wl.pos, wl.vel, wl.species)
+    g.step(3)
+    vox, start = g.get_voxels()
+    pos = g.get_state()[0]
+    assert start[-1] == wl.n_agents
+    assert np.array_equal(np.bincount(vox, minlength=len(start) - 1), np.diff(start.astype(np.int64)))
+    assert np.array_equal(vox.astype(np.int64), voxel_reference(wl.instances[0].domain, pos))
+    assert np.isfinite(pos).all()
