@@ -150,7 +150,7 @@ struct crm_ctx {
     uint32_t* d_perm = nullptr;
     double* d_force = nullptr;
     double4* d_rec64 = nullptr;
-    float4* d_rec32 = nullptr;
+    int4* d_recq = nullptr;
     StateBuf buf[2] = {};
     int cur = 0;
     double* d_noise = nullptr;
@@ -178,7 +178,7 @@ thread_local std::string g_create_error;
 
 void free_device(crm_ctx* ctx) {
     void* ptrs[] = {ctx->d_work, ctx->d_start, ctx->d_count, ctx->d_bsum, ctx->d_key, ctx->d_rank,
-                    ctx->d_perm, ctx->d_force, ctx->d_rec64, ctx->d_rec32, ctx->d_noise};
+                    ctx->d_perm, ctx->d_force, ctx->d_rec64, ctx->d_recq, ctx->d_noise};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& b : ctx->buf) {
@@ -191,7 +191,7 @@ void free_device(crm_ctx* ctx) {
     ctx->d_start = ctx->d_count = ctx->d_bsum = ctx->d_key = ctx->d_rank = ctx->d_perm = nullptr;
     ctx->d_force = nullptr;
     ctx->d_rec64 = nullptr;
-    ctx->d_rec32 = nullptr;
+    ctx->d_recq = nullptr;
     ctx->d_noise = nullptr;
     ctx->noise_cap = 0;
     for (auto& g : ctx->graph)
@@ -213,7 +213,7 @@ Arrays arrays(crm_ctx* ctx) {
     A.perm = ctx->d_perm;
     A.force = ctx->d_force;
     A.rec64 = ctx->d_rec64;
-    A.rec32 = ctx->d_rec32;
+    A.recq = ctx->d_recq;
     A.N = ctx->N;
     A.Vt = (uint32_t)ctx->Vt;
     return A;
@@ -529,6 +529,8 @@ crm_status crm_create(const crm_config* cfg, const crm_instance* instances, uint
             P.max[a] = g.max[a];
             P.dx[a] = g.dx[a];
             P.dxf[a] = (float)g.dx[a];
+            P.qinv[a] = 1073741824.0 / g.dx[a];
+            P.qunit[a] = (float)(g.dx[a] / 1073741824.0);
             P.n[a] = (uint32_t)g.n[a];
         }
         P.voxel_offset = (uint32_t)voff;
@@ -634,7 +636,7 @@ crm_status crm_set_agents(crm_ctx* ctx, const uint64_t* counts, const double* po
     CRM_TRY(dalloc(ctx, ctx->d_rank, N));
     CRM_TRY(dalloc(ctx, ctx->d_perm, N));
     CRM_TRY(dalloc(ctx, ctx->d_force, (size_t)K * N));
-    if (ctx->cfg.precision == CRM_PRECISION_MIXED) CRM_TRY(dalloc(ctx, ctx->d_rec32, N));
+    if (ctx->cfg.precision == CRM_PRECISION_MIXED) CRM_TRY(dalloc(ctx, ctx->d_recq, N));
     else CRM_TRY(dalloc(ctx, ctx->d_rec64, N));
     for (auto& b : ctx->buf) {
         CRM_TRY(dalloc(ctx, b.pos, (size_t)K * N));

@@ -68,16 +68,22 @@ __device__ __forceinline__ T powe(T x, T e, int ie) {
 // Point potentials.  z = own - ext.  Writes the force on `own`; the force on
 // `ext` is its exact negation for every potential below (IEEE negation is exact
 // and the reference returns (dir * F, -dir * F)).  Returns true on CalcError.
+//
+// `cut` overrides the cutoff test: -1 decides from z, 0 forces "beyond the
+// cutoff", 1 forces "inside".  The float pair path uses it when the FP64
+// decision was taken from the double positions (the force jumps at the cutoff,
+// so the decision has to be the reference's).
 // ---------------------------------------------------------------------------
 
 // calculate_morse_interaction, interaction.rs:151-182
 template <typename T, int D>
-__device__ __forceinline__ void morse(const T* z, T two_radius, T cutoff, T strength, T stiffness, T* f) {
+__device__ __forceinline__ void morse(const T* z, T two_radius, T cutoff, T strength, T stiffness, int cut, T* f) {
     T d2 = T(0);
 #pragma unroll
     for (int k = 0; k < D; k++) d2 = d2 + z[k] * z[k];
     T dist = dsqrt<T>(d2);
-    if (dist > cutoff || dist == T(0)) {
+    const bool beyond = cut < 0 ? (dist > cutoff) : (cut == 0);
+    if (beyond || dist == T(0)) {
 #pragma unroll
         for (int k = 0; k < D; k++) f[k] = T(0);
         return;
@@ -92,7 +98,7 @@ __device__ __forceinline__ void morse(const T* z, T two_radius, T cutoff, T stre
 // sigma and the Mie constant are pair-independent and precomputed on the host
 // with the reference's expressions.
 template <typename T, int D>
-__device__ __forceinline__ bool mie(const T* z, const PotParams<T>& P, T* f) {
+__device__ __forceinline__ bool mie(const T* z, const PotParams<T>& P, int cut, T* f) {
     T d2 = T(0);
 #pragma unroll
     for (int k = 0; k < D; k++) d2 = d2 + z[k] * z[k];
@@ -102,7 +108,8 @@ __device__ __forceinline__ bool mie(const T* z, const PotParams<T>& P, T* f) {
         for (int k = 0; k < D; k++) f[k] = T(0);
         return true;
     }
-    if (r > P.cutoff) {
+    const bool beyond = cut < 0 ? (r > P.cutoff) : (cut == 0);
+    if (beyond) {
 #pragma unroll
         for (int k = 0; k < D; k++) f[k] = T(0);
         return false;
@@ -118,7 +125,7 @@ __device__ __forceinline__ bool mie(const T* z, const PotParams<T>& P, T* f) {
 
 // BoundLennardJones::calculate_force_between, interaction.rs:122-138
 template <typename T, int D>
-__device__ __forceinline__ void bound_lj(const T* z, const PotParams<T>& P, T* f) {
+__device__ __forceinline__ void bound_lj(const T* z, const PotParams<T>& P, int cut, T* f) {
     T d2 = T(0);
 #pragma unroll
     for (int k = 0; k < D; k++) d2 = d2 + z[k] * z[k];
@@ -126,7 +133,8 @@ __device__ __forceinline__ void bound_lj(const T* z, const PotParams<T>& P, T* f
     T sr = P.sigma / r;
     T val = T(4) * P.epsilon / r * (T(12) * ipow<T>(sr, 11) - T(6) * ipow<T>(sr, 5));
     T mx = P.bound / r;
-    T q = (P.cutoff >= r) ? T(1) : T(0);
+    const bool inside = cut < 0 ? (P.cutoff >= r) : (cut == 1);
+    T q = inside ? T(1) : T(0);
     T m = dmin<T>(mx, val);
 #pragma unroll
     for (int k = 0; k < D; k++) f[k] = ((-(z[k] / r)) * q) * m;
@@ -134,8 +142,9 @@ __device__ __forceinline__ void bound_lj(const T* z, const PotParams<T>& P, T* f
 
 // CellSpecificInteraction, benchmarks/src/cell_sorting.rs:31-94 (sign +1) and
 // examples/cell_sorting/src/main.rs:32-103 (sign -1).  `same` = equal species.
+// Its discontinuity is the signum at relative_interaction_range * (R + R).
 template <typename T, int D>
-__device__ __forceinline__ void cell_sorting(const T* z, bool same, const PotParams<T>& P, T* f) {
+__device__ __forceinline__ void cell_sorting(const T* z, bool same, const PotParams<T>& P, int cut, T* f) {
     const T mrd = T(0.3162277660168379);
     T d2 = T(0);
 #pragma unroll
@@ -157,9 +166,14 @@ __device__ __forceinline__ void cell_sorting(const T* z, bool same, const PotPar
     }
     T sigma = r / P.two_radius;
     T bound = T(4) + T(1) / sigma;
-    T sgn_arg = P.rel_range * P.two_radius - r;
-    T signum = (sgn_arg != sgn_arg) ? sgn_arg : copysign(T(1), sgn_arg);
-    T spatial_cutoff = (T(1) + signum) * T(0.5);
+    T spatial_cutoff;
+    if (cut < 0) {
+        T sgn_arg = P.rel_range * P.two_radius - r;
+        T signum = (sgn_arg != sgn_arg) ? sgn_arg : copysign(T(1), sgn_arg);
+        spatial_cutoff = (T(1) + signum) * T(0.5);
+    } else {
+        spatial_cutoff = cut ? T(1) : T(0);
+    }
     T inv = T(1) / sigma;
     T inv2 = inv * inv;
     T strength = P.strength * dmax<T>(dmin<T>(inv2 - inv2 * inv2, bound), -bound);
@@ -174,26 +188,47 @@ __device__ __forceinline__ void cell_sorting(const T* z, bool same, const PotPar
     }
 }
 
+// Radius at which the potential is discontinuous (the cutoff test).
+template <int POT, typename T>
+__device__ __forceinline__ T cut_radius(const PotParams<T>& P) {
+    if constexpr (POT == CRM_POTENTIAL_CELL_SORTING) return P.rel_range * P.two_radius;
+    else return P.cutoff;
+}
+
+// Reference cutoff decision from FP64 distance r: 1 inside, 0 beyond.
+template <int POT>
+__device__ __forceinline__ int cut_decision(const PotParams<double>& P, double r) {
+    if constexpr (POT == CRM_POTENTIAL_CELL_SORTING) {
+        double a = P.rel_range * P.two_radius - r;
+        return (a != a) ? 0 : (copysign(1.0, a) > 0.0 ? 1 : 0);
+    } else if constexpr (POT == CRM_POTENTIAL_BOUND_LENNARD_JONES) {
+        return (P.cutoff >= r) ? 1 : 0;
+    } else {
+        return (r > P.cutoff) ? 0 : 1;
+    }
+}
+
 template <int POT, typename T, int D>
-__device__ __forceinline__ bool point_force(const T* z, int sp_own, int sp_ext, const PotParams<T>& P, T* f) {
+__device__ __forceinline__ bool point_force(const T* z, int sp_own, int sp_ext, const PotParams<T>& P, int cut,
+                                            T* f) {
     if constexpr (POT == CRM_POTENTIAL_MORSE) {
-        morse<T, D>(z, P.two_radius, P.cutoff, P.strength, P.stiffness, f);
+        morse<T, D>(z, P.two_radius, P.cutoff, P.strength, P.stiffness, cut, f);
         return false;
     } else if constexpr (POT == CRM_POTENTIAL_MORSE_SPECIES) {
         T s = P.strength * P.species_matrix[sp_own * CRM_MAX_SPECIES + sp_ext];
-        morse<T, D>(z, P.two_radius, P.cutoff, s, P.stiffness, f);
+        morse<T, D>(z, P.two_radius, P.cutoff, s, P.stiffness, cut, f);
         return false;
     } else if constexpr (POT == CRM_POTENTIAL_MIE) {
-        return mie<T, D>(z, P, f);
+        return mie<T, D>(z, P, cut, f);
     } else if constexpr (POT == CRM_POTENTIAL_BOUND_LENNARD_JONES) {
-        bound_lj<T, D>(z, P, f);
+        bound_lj<T, D>(z, P, cut, f);
         return false;
     } else if constexpr (POT == CRM_POTENTIAL_NONE) {
 #pragma unroll
         for (int k = 0; k < D; k++) f[k] = T(0);
         return false;
     } else {
-        cell_sorting<T, D>(z, sp_own == sp_ext, P, f);
+        cell_sorting<T, D>(z, sp_own == sp_ext, P, cut, f);
         return false;
     }
 }

@@ -41,7 +41,7 @@ struct Arrays {
     uint32_t* perm;   // [N] sorted slot -> previous slot
     double* force;    // [K][N]
     double4* rec64;   // [N] (x, y, z, species) absolute, FP64 path
-    float4* rec32;    // [N] (x, y, z, species) relative to the voxel origin, mixed path
+    int4* recq;       // [N] (qx, qy, qz, species | outside << 31) fixed point in the voxel, mixed path
     uint64_t N;
     uint32_t Vt;
 };
@@ -78,11 +78,32 @@ struct Kahan {
 //   adjacent voxel: 1/2 f(i<-j by i)                        (update_mechanics.rs:90-106, 205-220)
 // The physical mode gives adjacent pairs the same-voxel weight.  With a
 // symmetric interaction both evaluations are equal and one is computed.
+//
+// Mixed precision: positions are 30-bit fixed point relative to the voxel
+// origin (q = floor((p - origin) * 2^30 / dx)), so the difference of two agents
+// in neighbouring voxels is an exact int32 and converts to float with one
+// rounding: the direction is accurate to ~1e-7 however close the pair is.  A
+// pair whose float distance lies in a narrow band around the cutoff radius (the
+// forces jump there), a zero distance, or an agent flagged as outside its
+// voxel's fixed-point range, is re-evaluated from the FP64 positions so that
+// the cutoff decision is the reference's.
 // ---------------------------------------------------------------------------
+
+// blockIdx -> work block, so that each XCD (blocks are dealt round-robin over 8)
+// walks one contiguous slab of agents and its L2 holds that slab's records.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
+    const uint32_t per = nb / 8u, rem = nb % 8u;
+    const uint32_t x = b % 8u, k = b / 8u;
+    return x < rem ? x * (per + 1u) + k : rem * (per + 1u) + (x - rem) * per + k;
+}
+
+constexpr int kQBits = 30;
+constexpr int kQOne = 1 << kQBits;
+
 template <int D, int POT, bool MIXED>
 __global__ __launch_bounds__(kBlock) void k_force_point(Arrays A, StateBuf S, uint32_t physical,
                                                           uint32_t symmetric) {
-    const BlockWork w = A.work[blockIdx.x];
+    const BlockWork w = A.work[xcd_block(blockIdx.x, gridDim.x)];
     if (threadIdx.x >= w.count) return;
     const uint64_t s = w.first + threadIdx.x;
     const InstParams& P = A.ip[w.inst];
@@ -100,19 +121,27 @@ __global__ __launch_bounds__(kBlock) void k_force_point(Arrays A, StateBuf S, ui
     }
     using T = typename std::conditional<MIXED, float, double>::type;
     const PotParams<T>& PP = pot_of<T>(P);
-    T me[D];
+    const T rc = cut_radius<POT, T>(PP);
+    const float rc2 = (float)(rc * rc);
+    const float band = 2e-6f * rc2;
+    const uint64_t N = A.N;
+
+    int qi[3] = {0, 0, 0};
+    double me[3] = {0.0, 0.0, 0.0};
     int sp_i;
+    uint32_t flag_i = 0;
     if constexpr (MIXED) {
-        float4 r = A.rec32[s];
-        me[0] = r.x;
-        me[1] = r.y;
-        if constexpr (D == 3) me[2] = r.z;
-        sp_i = __float_as_int(r.w);
+        int4 r = A.recq[s];
+        qi[0] = r.x;
+        qi[1] = r.y;
+        qi[2] = r.z;
+        sp_i = r.w & 0x7fffffff;
+        flag_i = (uint32_t)r.w >> 31;
     } else {
         double4 r = A.rec64[s];
         me[0] = r.x;
         me[1] = r.y;
-        if constexpr (D == 3) me[2] = r.z;
+        me[2] = r.z;
         sp_i = (int)r.w;
     }
     double accd[D];
@@ -134,37 +163,49 @@ __global__ __launch_bounds__(kBlock) void k_force_point(Arrays A, StateBuf S, ui
                 const uint32_t je = A.start[vv + 1];
                 if (jb != je) {
                     const bool full = physical || vv == v;
-                    T shift[D];
+                    // fixed-point position of agent i relative to the candidate voxel's origin
+                    int sh[D];
                     if constexpr (MIXED) {
                         const int off[3] = {(int)a - (int)idx[0], (int)b - (int)idx[1], (int)c - (int)idx[2]};
 #pragma unroll
-                        for (int k = 0; k < D; k++) {
-                            const int ax = k + 3 - D;
-                            shift[k] = me[k] - (float)off[ax] * P.dxf[ax];
-                        }
-                    } else {
-#pragma unroll
-                        for (int k = 0; k < D; k++) shift[k] = me[k];
+                        for (int k = 0; k < D; k++) sh[k] = qi[k] - off[k + 3 - D] * kQOne;
                     }
                     for (uint32_t j = jb; j < je; j++) {
                         if (j == s) continue;
                         T z[D];
                         int sp_j;
+                        int cut = -1;
                         if constexpr (MIXED) {
-                            float4 r = A.rec32[j];
-                            z[0] = shift[0] - r.x;
-                            z[1] = shift[1] - r.y;
-                            if constexpr (D == 3) z[2] = shift[2] - r.z;
-                            sp_j = __float_as_int(r.w);
+                            int4 r = A.recq[j];
+                            const int qj[3] = {r.x, r.y, r.z};
+                            sp_j = r.w & 0x7fffffff;
+                            float d2 = 0.0f;
+#pragma unroll
+                            for (int k = 0; k < D; k++) {
+                                z[k] = (float)(sh[k] - qj[k]) * P.qunit[k + 3 - D];
+                                d2 = d2 + z[k] * z[k];
+                            }
+                            const bool refine = (((uint32_t)r.w >> 31) | flag_i) || d2 == 0.0f ||
+                                                fabsf(d2 - rc2) <= band;
+                            if (refine) {
+                                double zd[D], dd = 0.0;
+#pragma unroll
+                                for (int k = 0; k < D; k++) {
+                                    zd[k] = S.pos[k * N + s] - S.pos[k * N + j];
+                                    dd = dd + zd[k] * zd[k];
+                                    z[k] = (float)zd[k];
+                                }
+                                cut = cut_decision<POT>(P.pd, sqrt(dd));
+                            }
                         } else {
                             double4 r = A.rec64[j];
-                            z[0] = shift[0] - r.x;
-                            z[1] = shift[1] - r.y;
-                            if constexpr (D == 3) z[2] = shift[2] - r.z;
+                            z[0] = me[0] - r.x;
+                            z[1] = me[1] - r.y;
+                            if constexpr (D == 3) z[2] = me[2] - r.z;
                             sp_j = (int)r.w;
                         }
                         T f[D];
-                        calc_err |= point_force<POT, T, D>(z, sp_i, sp_j, PP, f);
+                        calc_err |= point_force<POT, T, D>(z, sp_i, sp_j, PP, cut, f);
                         T wgt = T(0.5);
                         if (full) {
                             if (symmetric) {
@@ -173,7 +214,7 @@ __global__ __launch_bounds__(kBlock) void k_force_point(Arrays A, StateBuf S, ui
                                 T zj[D], g[D];
 #pragma unroll
                                 for (int k = 0; k < D; k++) zj[k] = -z[k];
-                                calc_err |= point_force<POT, T, D>(zj, sp_j, sp_i, PP, g);
+                                calc_err |= point_force<POT, T, D>(zj, sp_j, sp_i, PP, cut, g);
 #pragma unroll
                                 for (int k = 0; k < D; k++) {
                                     if constexpr (MIXED) acck[k].add(-g[k] * 0.5f);
@@ -198,7 +239,7 @@ __global__ __launch_bounds__(kBlock) void k_force_point(Arrays A, StateBuf S, ui
         double fk;
         if constexpr (MIXED) fk = (double)acck[k].s + (double)(-acck[k].c);
         else fk = accd[k];
-        A.force[(uint64_t)k * A.N + s] = fk;
+        A.force[(uint64_t)k * N + s] = fk;
     }
 }
 
@@ -577,14 +618,18 @@ __global__ __launch_bounds__(kBlock) void k_gather_point(Arrays A, StateBuf src,
         idx[2] = lv % P.n[2];
         idx[1] = (lv / P.n[2]) % P.n[1];
         idx[0] = lv / (P.n[2] * P.n[1]);
-        float r[3] = {0.f, 0.f, 0.f};
+        int q[3] = {0, 0, 0};
+        uint32_t outside = 0;
 #pragma unroll
         for (int k = 0; k < D; k++) {
             const int ax = k + 3 - D;
-            double origin = P.min[ax] + (double)idx[ax] * P.dx[ax];
-            r[k] = (float)(p[k] - origin);
+            const double origin = P.min[ax] + (double)idx[ax] * P.dx[ax];
+            const double t = floor((p[k] - origin) * P.qinv[ax]);
+            const bool ok = t >= 0.0 && t < (double)kQOne;  // false for NaN
+            q[k] = ok ? (int)t : (t < 0.0 ? 0 : kQOne - 1);
+            outside |= ok ? 0u : 1u;
         }
-        A.rec32[s] = make_float4(r[0], r[1], r[2], __int_as_float(sp));
+        A.recq[s] = make_int4(q[0], q[1], D == 3 ? q[2] : 0, (int)((uint32_t)sp | (outside << 31)));
     } else {
         A.rec64[s] = make_double4(p[0], p[1], D == 3 ? p[D - 1] : 0.0, (double)sp);
     }

@@ -26,6 +26,8 @@ struct PotParams {
 struct InstParams {
     // grid, 3-D layout (axis 0 has extent 1 in 2-D)
     double min[3], max[3], dx[3];
+    double qinv[3];   // 2^30 / dx: fixed-point scale of voxel-relative positions
+    float qunit[3];   // dx / 2^30
     float dxf[3];
     uint32_t n[3];
     uint32_t voxel_offset;  // first global voxel of this instance

@@ -85,7 +85,9 @@ def c2_sorting_3d(n: int = 1_000_000, seed: int = 2, steps: int = 1000) -> Workl
 def c4_ensemble_2d(n_instances: int = 512, seed: int = 4, steps: int = 1000, n_min: int = 500,
                    n_max: int = 20000) -> Workload:
     """Ensemble of 2-D Morse runs: N_i log-uniform in [n_min, n_max], strength U[0.05, 0.5],
-    damping U[0.5, 2.0], box side sqrt(N_i / 0.02); Morse R 6, lambda 1/1.5, cutoff 12; Euler."""
+    damping U[0.5, 2.0], box side sqrt(N_i / 0.02); Morse R 2, lambda 1/1.5, cutoff 12; Euler.
+    The radius is not specified for this ensemble; R = 2 keeps the area fraction at 0.25
+    (R = 6 at density 0.02 would overlap 2.3-fold and throw agents out of the box)."""
     rng = np.random.default_rng(seed)
     counts = np.round(np.exp(rng.uniform(math.log(n_min), math.log(n_max), n_instances))).astype(int)
     strengths = rng.uniform(0.05, 0.5, n_instances)
@@ -94,7 +96,7 @@ def c4_ensemble_2d(n_instances: int = 512, seed: int = 4, steps: int = 1000, n_m
     for i in range(n_instances):
         L = math.sqrt(counts[i] / 0.02)
         dom = CartesianCuboid.from_boundaries_and_interaction_range([0.0, 0.0], [L, L], 12.0)
-        pot = MorsePotential(radius=6.0, potential_stiffness=1.0 / 1.5, cutoff=12.0, strength=float(strengths[i]))
+        pot = MorsePotential(radius=2.0, potential_stiffness=1.0 / 1.5, cutoff=12.0, strength=float(strengths[i]))
         mech = NewtonDamped(damping_constant=float(dampings[i]), mass=1.0)
         inst.append(InstanceSpec(dom, pot, mech))
         pos.append(_uniform_open(rng, int(counts[i]), 2, L))
