@@ -48,6 +48,7 @@ NOISE = {"none": 0, "buffer": 1, "philox": 2}
 
 FLAG_GRAPH = 1
 FLAG_TIMING = 2
+FLAG_NO_LDS = 4
 
 KERNEL_CLASSES = ("force", "integrate", "scan", "scatter", "fixup", "gather")
 

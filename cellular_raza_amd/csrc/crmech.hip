@@ -140,6 +140,8 @@ struct crm_ctx {
     InstParams* d_ip = nullptr;
     BlockWork* d_work = nullptr;
     uint32_t n_work = 0;
+    ForceTile* d_tiles = nullptr;
+    uint32_t n_force_tiles = 0;
     Control* d_ctrl = nullptr;
     uint32_t* d_start = nullptr;
     uint32_t* d_count = nullptr;
@@ -177,7 +179,7 @@ namespace {
 thread_local std::string g_create_error;
 
 void free_device(crm_ctx* ctx) {
-    void* ptrs[] = {ctx->d_work, ctx->d_start, ctx->d_count, ctx->d_bsum, ctx->d_key, ctx->d_rank,
+    void* ptrs[] = {ctx->d_work, ctx->d_tiles, ctx->d_start, ctx->d_count, ctx->d_bsum, ctx->d_key, ctx->d_rank,
                     ctx->d_perm, ctx->d_force, ctx->d_rec64, ctx->d_recq, ctx->d_noise};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -188,6 +190,7 @@ void free_device(crm_ctx* ctx) {
         b = StateBuf{};
     }
     ctx->d_work = nullptr;
+    ctx->d_tiles = nullptr;
     ctx->d_start = ctx->d_count = ctx->d_bsum = ctx->d_key = ctx->d_rank = ctx->d_perm = nullptr;
     ctx->d_force = nullptr;
     ctx->d_rec64 = nullptr;
@@ -250,12 +253,13 @@ crm_status launch_force_dp(crm_ctx* ctx, StateBuf S) {
     Arrays A = arrays(ctx);
     const uint32_t phys = ctx->cfg.weighting == CRM_WEIGHT_PHYSICAL;
     const uint32_t sym = ctx->symmetric;
+    const uint32_t no_lds = (ctx->cfg.flags & CRM_FLAG_NO_LDS) ? 1u : 0u;
     if (ctx->cfg.precision == CRM_PRECISION_MIXED)
-        hipLaunchKernelGGL((k_force_point<D, POT, true>), dim3(ctx->n_work), dim3(kBlock), 0, ctx->stream, A, S,
-                           phys, sym);
+        hipLaunchKernelGGL((k_force_tiled<D, POT, true>), dim3(ctx->n_force_tiles), dim3(kBlock), 0, ctx->stream, A,
+                           S, ctx->d_tiles, phys, sym, no_lds);
     else
-        hipLaunchKernelGGL((k_force_point<D, POT, false>), dim3(ctx->n_work), dim3(kBlock), 0, ctx->stream, A, S,
-                           phys, sym);
+        hipLaunchKernelGGL((k_force_tiled<D, POT, false>), dim3(ctx->n_force_tiles), dim3(kBlock), 0, ctx->stream, A,
+                           S, ctx->d_tiles, phys, sym, no_lds);
     return CRM_OK;
 }
 
@@ -280,27 +284,7 @@ crm_status launch_force(crm_ctx* ctx) {
     return t_end(ctx, CRM_KERNEL_FORCE);
 }
 
-template <int D>
-crm_status launch_integrate_d(crm_ctx* ctx, StateBuf S, const StepArgs& sa) {
-    Arrays A = arrays(ctx);
-    switch (ctx->mech_kind) {
-        case CRM_MECHANICS_NEWTON_DAMPED:
-            hipLaunchKernelGGL((k_integrate_point<D, CRM_MECHANICS_NEWTON_DAMPED>), dim3(ctx->n_work), dim3(kBlock), 0,
-                               ctx->stream, A, S, sa);
-            return CRM_OK;
-        case CRM_MECHANICS_LANGEVIN:
-            hipLaunchKernelGGL((k_integrate_point<D, CRM_MECHANICS_LANGEVIN>), dim3(ctx->n_work), dim3(kBlock), 0,
-                               ctx->stream, A, S, sa);
-            return CRM_OK;
-        case CRM_MECHANICS_BROWNIAN:
-            hipLaunchKernelGGL((k_integrate_point<D, CRM_MECHANICS_BROWNIAN>), dim3(ctx->n_work), dim3(kBlock), 0,
-                               ctx->stream, A, S, sa);
-            return CRM_OK;
-    }
-    return CRM_INVALID_ARGUMENT;
-}
-
-crm_status launch_integrate(crm_ctx* ctx, const double* noise_step) {
+StepArgs step_args(crm_ctx* ctx, const double* noise_step) {
     StepArgs sa;
     sa.noise = noise_step;
     sa.seed = ctx->cfg.seed;
@@ -310,16 +294,60 @@ crm_status launch_integrate(crm_ctx* ctx, const double* noise_step) {
     sa.cap = ctx->cfg.solver_order;
     sa.newest_first = ctx->cfg.history_order == CRM_HISTORY_NEWEST_FIRST;
     sa.pad = 0;
+    return sa;
+}
+
+template <int D>
+crm_status launch_key_d(crm_ctx* ctx, StateBuf S, const StepArgs& sa) {
+    Arrays A = arrays(ctx);
+    switch (ctx->mech_kind) {
+        case CRM_MECHANICS_NEWTON_DAMPED:
+            hipLaunchKernelGGL((k_key_point<D, CRM_MECHANICS_NEWTON_DAMPED>), dim3(ctx->n_work), dim3(kBlock), 0,
+                               ctx->stream, A, S, sa);
+            return CRM_OK;
+        case CRM_MECHANICS_LANGEVIN:
+            hipLaunchKernelGGL((k_key_point<D, CRM_MECHANICS_LANGEVIN>), dim3(ctx->n_work), dim3(kBlock), 0,
+                               ctx->stream, A, S, sa);
+            return CRM_OK;
+        case CRM_MECHANICS_BROWNIAN:
+            hipLaunchKernelGGL((k_key_point<D, CRM_MECHANICS_BROWNIAN>), dim3(ctx->n_work), dim3(kBlock), 0,
+                               ctx->stream, A, S, sa);
+            return CRM_OK;
+    }
+    return CRM_INVALID_ARGUMENT;
+}
+
+crm_status launch_key(crm_ctx* ctx, const StepArgs& sa) {
     CRM_TRY(t_begin(ctx));
     StateBuf S = ctx->buf[ctx->cur];
-    crm_status st = ctx->D == 2 ? launch_integrate_d<2>(ctx, S, sa) : launch_integrate_d<3>(ctx, S, sa);
+    crm_status st = ctx->D == 2 ? launch_key_d<2>(ctx, S, sa) : launch_key_d<3>(ctx, S, sa);
     if (st) return st;
     return t_end(ctx, CRM_KERNEL_INTEGRATE);
 }
 
+template <int D, bool MIXED>
+crm_status launch_gather_integrate_dm(crm_ctx* ctx, StateBuf src, StateBuf dst, const StepArgs& sa) {
+    Arrays A = arrays(ctx);
+    switch (ctx->mech_kind) {
+        case CRM_MECHANICS_NEWTON_DAMPED:
+            hipLaunchKernelGGL((k_gather_integrate_point<D, MIXED, CRM_MECHANICS_NEWTON_DAMPED>), dim3(ctx->n_work),
+                               dim3(kBlock), 0, ctx->stream, A, src, dst, sa);
+            return CRM_OK;
+        case CRM_MECHANICS_LANGEVIN:
+            hipLaunchKernelGGL((k_gather_integrate_point<D, MIXED, CRM_MECHANICS_LANGEVIN>), dim3(ctx->n_work),
+                               dim3(kBlock), 0, ctx->stream, A, src, dst, sa);
+            return CRM_OK;
+        case CRM_MECHANICS_BROWNIAN:
+            hipLaunchKernelGGL((k_gather_integrate_point<D, MIXED, CRM_MECHANICS_BROWNIAN>), dim3(ctx->n_work),
+                               dim3(kBlock), 0, ctx->stream, A, src, dst, sa);
+            return CRM_OK;
+    }
+    return CRM_INVALID_ARGUMENT;
+}
+
 // scan + scatter + fixup + gather: the counting sort that rebuilds the voxel
 // lists from d_key / d_rank / d_count and moves the state to the other buffer.
-crm_status launch_sort(crm_ctx* ctx, bool advance) {
+crm_status launch_sort(crm_ctx* ctx, bool advance, const StepArgs* sa) {
     Arrays A = arrays(ctx);
     const uint32_t Vt = (uint32_t)ctx->Vt;
     CRM_TRY(t_begin(ctx));
@@ -340,7 +368,15 @@ crm_status launch_sort(crm_ctx* ctx, bool advance) {
     StateBuf src = ctx->buf[ctx->cur], dst = ctx->buf[1 - ctx->cur];
     const uint32_t cap = ctx->cfg.solver_order;
     const bool mixed = ctx->cfg.precision == CRM_PRECISION_MIXED;
-    if (ctx->D == 2) {
+    crm_status gst = CRM_OK;
+    if (advance) {
+        if (ctx->D == 2)
+            gst = mixed ? launch_gather_integrate_dm<2, true>(ctx, src, dst, *sa)
+                        : launch_gather_integrate_dm<2, false>(ctx, src, dst, *sa);
+        else
+            gst = mixed ? launch_gather_integrate_dm<3, true>(ctx, src, dst, *sa)
+                        : launch_gather_integrate_dm<3, false>(ctx, src, dst, *sa);
+    } else if (ctx->D == 2) {
         if (mixed)
             hipLaunchKernelGGL((k_gather_point<2, true>), dim3(ctx->n_work), dim3(kBlock), 0, ctx->stream, A, src, dst,
                                cap);
@@ -355,15 +391,17 @@ crm_status launch_sort(crm_ctx* ctx, bool advance) {
             hipLaunchKernelGGL((k_gather_point<3, false>), dim3(ctx->n_work), dim3(kBlock), 0, ctx->stream, A, src,
                                dst, cap);
     }
+    if (gst) return gst;
     CRM_TRY(t_end(ctx, CRM_KERNEL_GATHER));
     ctx->cur = 1 - ctx->cur;
     return CRM_OK;
 }
 
 crm_status launch_step(crm_ctx* ctx, const double* noise_step) {
+    const StepArgs sa = step_args(ctx, noise_step);
     CRM_TRY(launch_force(ctx));
-    CRM_TRY(launch_integrate(ctx, noise_step));
-    CRM_TRY(launch_sort(ctx, true));
+    CRM_TRY(launch_key(ctx, sa));
+    CRM_TRY(launch_sort(ctx, true, &sa));
     return CRM_OK;
 }
 
@@ -407,6 +445,36 @@ crm_status reset_control(crm_ctx* ctx, uint64_t step) {
     CRM_HIP(hipMemcpyAsync(ctx->d_ctrl, &c, sizeof(Control), hipMemcpyHostToDevice, ctx->stream));
     CRM_HIP(hipStreamSynchronize(ctx->stream));
     return CRM_OK;
+}
+
+// Force tiles of one instance (see k_force_tiled): whole rows when the expected
+// neighbourhood of a few rows fits the LDS budget, otherwise single rows cut
+// along their length.  Sized from the mean density with a 20 % margin; a tile
+// that overflows at run time falls back to global loads.
+void build_tiles(const InstParams& P, uint32_t inst, uint64_t n_agents, uint32_t cap, std::vector<ForceTile>& out) {
+    const uint32_t n0 = P.n[0], n1 = P.n[1], n2 = P.n[2];
+    const double rows = (double)n0 * n1;
+    const double apr = (double)n_agents / rows;
+    const double rho = (double)n_agents / ((double)rows * n2);
+    const double planes = n0 > 1 ? 3.0 : 1.0;
+    const double margin = 1.2;
+    uint32_t R = 0;
+    for (uint32_t r = 1; r <= (uint32_t)kTileRowsMax && r <= n1; r++) {
+        const bool fits = planes * (r + 2) * apr * margin <= cap &&
+                          3u * (r + 2) * (n2 + 1) <= (uint32_t)kLdsVoxelStarts && r * apr <= 320.0;
+        if (fits) R = r;
+    }
+    if (R >= 1) {
+        for (uint32_t a = 0; a < n0; a++)
+            for (uint32_t b = 0; b < n1; b += R) out.push_back(ForceTile{inst, a, b, std::min(b + R, n1), 0u, n2});
+        return;
+    }
+    double lz = std::floor(cap / (planes * 3.0 * std::max(rho, 1e-12) * margin)) - 2.0;
+    lz = std::min(lz, (double)kLdsVoxelStarts / 9.0 - 3.0);
+    const uint32_t L = (uint32_t)std::max(1.0, std::min(lz, (double)n2));
+    for (uint32_t a = 0; a < n0; a++)
+        for (uint32_t b = 0; b < n1; b++)
+            for (uint32_t c = 0; c < n2; c += L) out.push_back(ForceTile{inst, a, b, b + 1, c, std::min(c + L, n2)});
 }
 
 template <typename T>
@@ -624,6 +692,10 @@ crm_status crm_set_agents(crm_ctx* ctx, const uint64_t* counts, const double* po
         first += counts[i];
     }
     ctx->n_work = (uint32_t)work.size();
+    std::vector<ForceTile> tiles;
+    const uint32_t cap = ctx->cfg.precision == CRM_PRECISION_MIXED ? kLdsRecordsMixed : kLdsRecordsFP64;
+    for (uint32_t i = 0; i < ctx->n_inst; i++) build_tiles(ctx->hparams[i], i, counts[i], cap, tiles);
+    ctx->n_force_tiles = (uint32_t)tiles.size();
     CRM_HIP(hipMemcpy(ctx->d_ip, ctx->hparams.data(), sizeof(InstParams) * ctx->n_inst, hipMemcpyHostToDevice));
 
     const uint64_t Vt = ctx->Vt;
@@ -648,6 +720,8 @@ crm_status crm_set_agents(crm_ctx* ctx, const uint64_t* counts, const double* po
         CRM_TRY(dalloc(ctx, b.vox, N));
     }
     CRM_HIP(hipMemcpy(ctx->d_work, work.data(), sizeof(BlockWork) * work.size(), hipMemcpyHostToDevice));
+    CRM_TRY(dalloc(ctx, ctx->d_tiles, tiles.size()));
+    CRM_HIP(hipMemcpy(ctx->d_tiles, tiles.data(), sizeof(ForceTile) * tiles.size(), hipMemcpyHostToDevice));
     CRM_HIP(hipMemsetAsync(ctx->d_count, 0, sizeof(uint32_t) * std::max<uint64_t>(Vt, 1), ctx->stream));
 
     // upload in agent-index order (slot = agent index before the first sort)
@@ -680,7 +754,7 @@ crm_status crm_set_agents(crm_ctx* ctx, const uint64_t* counts, const double* po
         hipLaunchKernelGGL(k_initial_key<3>, dim3(ctx->n_work), dim3(kBlock), 0, ctx->stream, A, S, ctx->nv);
     CRM_HIP(hipGetLastError());
     CRM_TRY(check_errors(ctx, "initial placement"));
-    CRM_TRY(launch_sort(ctx, false));
+    CRM_TRY(launch_sort(ctx, false, nullptr));
     CRM_HIP(hipStreamSynchronize(ctx->stream));
     ctx->have_agents = true;
     return CRM_OK;

@@ -75,9 +75,44 @@ __device__ __forceinline__ T powe(T x, T e, int ie) {
 // so the decision has to be the reference's).
 // ---------------------------------------------------------------------------
 
+// Float pair path: distance and its reciprocal from one v_rsq_f32, exp from
+// v_exp_f32 (relative errors ~1e-7, far inside the mixed-precision budget).
+struct FastDist {
+    float d2, dist, inv;
+};
+template <int D>
+__device__ __forceinline__ FastDist fast_dist(const float* z) {
+    float d2 = 0.0f;
+#pragma unroll
+    for (int k = 0; k < D; k++) d2 = d2 + z[k] * z[k];
+    const float inv = __builtin_amdgcn_rsqf(d2);
+    return FastDist{d2, d2 * inv, inv};
+}
+
+template <int D>
+__device__ __forceinline__ void morse_fast(const float* z, float two_radius, float cutoff, float strength,
+                                           float stiffness, int cut, float* f) {
+    const FastDist q = fast_dist<D>(z);
+    const bool beyond = cut < 0 ? (q.dist > cutoff) : (cut == 0);
+    if (beyond || q.d2 == 0.0f) {
+#pragma unroll
+        for (int k = 0; k < D; k++) f[k] = 0.0f;
+        return;
+    }
+    const float e = __expf((-stiffness) * (q.dist - two_radius));
+    const float force = ((((-2.0f) * strength) * stiffness) * e) * (1.0f - e);
+    const float fi = force * q.inv;
+#pragma unroll
+    for (int k = 0; k < D; k++) f[k] = z[k] * fi;
+}
+
 // calculate_morse_interaction, interaction.rs:151-182
 template <typename T, int D>
 __device__ __forceinline__ void morse(const T* z, T two_radius, T cutoff, T strength, T stiffness, int cut, T* f) {
+    if constexpr (std::is_same<T, float>::value) {
+        morse_fast<D>(z, two_radius, cutoff, strength, stiffness, cut, f);
+        return;
+    }
     T d2 = T(0);
 #pragma unroll
     for (int k = 0; k < D; k++) d2 = d2 + z[k] * z[k];
@@ -208,15 +243,15 @@ __device__ __forceinline__ int cut_decision(const PotParams<double>& P, double r
     }
 }
 
+// One Interaction::calculate_force_between evaluation.  `strength` is the
+// pair's interaction strength (Morse: the potential's strength, or that times
+// the species-matrix entry, computed by the caller), `same` tells the
+// cell-sorting interaction whether both agents are of one species.
 template <int POT, typename T, int D>
-__device__ __forceinline__ bool point_force(const T* z, int sp_own, int sp_ext, const PotParams<T>& P, int cut,
+__device__ __forceinline__ bool point_force(const T* z, bool same, T strength, const PotParams<T>& P, int cut,
                                             T* f) {
-    if constexpr (POT == CRM_POTENTIAL_MORSE) {
-        morse<T, D>(z, P.two_radius, P.cutoff, P.strength, P.stiffness, cut, f);
-        return false;
-    } else if constexpr (POT == CRM_POTENTIAL_MORSE_SPECIES) {
-        T s = P.strength * P.species_matrix[sp_own * CRM_MAX_SPECIES + sp_ext];
-        morse<T, D>(z, P.two_radius, P.cutoff, s, P.stiffness, cut, f);
+    if constexpr (POT == CRM_POTENTIAL_MORSE || POT == CRM_POTENTIAL_MORSE_SPECIES) {
+        morse<T, D>(z, P.two_radius, P.cutoff, strength, P.stiffness, cut, f);
         return false;
     } else if constexpr (POT == CRM_POTENTIAL_MIE) {
         return mie<T, D>(z, P, cut, f);
@@ -228,9 +263,15 @@ __device__ __forceinline__ bool point_force(const T* z, int sp_own, int sp_ext, 
         for (int k = 0; k < D; k++) f[k] = T(0);
         return false;
     } else {
-        cell_sorting<T, D>(z, sp_own == sp_ext, P, cut, f);
+        cell_sorting<T, D>(z, same, P, cut, f);
         return false;
     }
+}
+
+// Strength of the (own, ext) species pair from a row of the matrix held in registers.
+template <typename T>
+__device__ __forceinline__ T pick4(const T* row, int k) {
+    return k == 0 ? row[0] : (k == 1 ? row[1] : (k == 2 ? row[2] : row[3]));
 }
 
 // nearest_point_from_point_to_line, interaction.rs:497-509 (t and the point x).

@@ -1,14 +1,15 @@
 // kernels_point.hpp -- gfx950 kernels of the per-step mechanics path for point agents.
 //
 // Step (run_sim.rs:380-403) as launched by the runtime:
-//   k_force_point      pair loop over the 3^D voxel stencil (update_mechanics.rs:153-238)
-//   k_integrate_point  Mechanics increment + Euler/AB step (solvers.rs:23-347), reflecting
-//                      boundary (cartesian_cuboid_n.rs:594-631), next voxel index
+//   k_force_tiled      pair loop over the 3^D voxel stencil (update_mechanics.rs:153-238)
+//   k_key_point        next position (solvers.rs:23-347 + apply_boundary,
+//                      cartesian_cuboid_n.rs:594-631) -> next voxel index
 //                      (cartesian_cuboid_n.rs:469-491) and its histogram
 //   k_scan_*           exclusive scan of the voxel histogram            } integer counting sort,
 //   k_scatter          slot -> sorted position                          } stable by previous slot
 //   k_fixup            per-voxel order by previous slot, histogram reset } (sort_cells_in_voxels,
-//   k_gather_point     permutes the state, writes force records          }  update_mechanics.rs:418-513)
+//   k_gather_integrate_point  the full mechanics update, written into   }  update_mechanics.rs:418-513)
+//                      the agent's new sorted slot, and the force records
 //
 // Agents live in SoA arrays sorted by voxel; a voxel's agents are the slots
 // [start[v], start[v+1]).
@@ -100,15 +101,79 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
 constexpr int kQBits = 30;
 constexpr int kQOne = 1 << kQBits;
 
-template <int D, int POT, bool MIXED>
-__global__ __launch_bounds__(kBlock) void k_force_point(Arrays A, StateBuf S, uint32_t physical,
-                                                          uint32_t symmetric) {
-    const BlockWork w = A.work[xcd_block(blockIdx.x, gridDim.x)];
-    if (threadIdx.x >= w.count) return;
-    const uint64_t s = w.first + threadIdx.x;
-    const InstParams& P = A.ip[w.inst];
-    const uint32_t v = S.vox[s];
-    const uint32_t lv = v - P.voxel_offset;
+template <bool MIXED>
+struct RecOf {
+    using type = double4;
+};
+template <>
+struct RecOf<true> {
+    using type = int4;
+};
+
+__device__ __forceinline__ const double4* rec_array(const Arrays& A, std::false_type) { return A.rec64; }
+__device__ __forceinline__ const int4* rec_array(const Arrays& A, std::true_type) { return A.recq; }
+
+// Candidate sources.  Within a stencil row (a, b) the voxels are consecutive in
+// the sorted order, so the candidates of voxels [c_lo, c_hi) of that row are one
+// contiguous index range; range() also returns the bounds of the agent's own
+// column c = i2 inside the row (b1, b2) so that each candidate's voxel offset
+// along the row is (j >= b1) + (j >= b2) - 1.
+struct RowRange {
+    uint32_t jb, je, b1, b2;
+};
+
+template <bool MIXED>
+struct GlobalSource {
+    using Rec = typename RecOf<MIXED>::type;
+    const Rec* rec;
+    const uint32_t* start;
+    uint32_t voff, n1, n2;
+    __device__ __forceinline__ RowRange range(uint32_t a, uint32_t b, uint32_t c_lo, uint32_t c_hi,
+                                              uint32_t i2) const {
+        const uint32_t row = voff + (a * n1 + b) * n2;
+        return RowRange{start[row + c_lo], start[row + c_hi], start[row + i2], start[row + i2 + 1]};
+    }
+    __device__ __forceinline__ Rec record(uint32_t j) const { return rec[j]; }
+    __device__ __forceinline__ uint32_t slot(uint32_t j) const { return j; }
+};
+
+template <bool MIXED>
+struct LdsSource {
+    using Rec = typename RecOf<MIXED>::type;
+    const Rec* lds;       // staged records
+    const uint32_t* vst;  // per candidate row: LDS index of the first agent of voxels cl .. ch
+    const uint32_t* gb;   // per candidate row: global slot of its first staged agent
+    const uint32_t* off;  // per candidate row: its first LDS index
+    uint32_t a0, b0, RW, W1, cl;
+    __device__ __forceinline__ uint32_t row_of(uint32_t a, uint32_t b) const { return (a + 1 - a0) * RW + (b + 1 - b0); }
+    __device__ __forceinline__ RowRange range(uint32_t a, uint32_t b, uint32_t c_lo, uint32_t c_hi,
+                                              uint32_t i2) const {
+        const uint32_t* r = vst + row_of(a, b) * W1 - cl;
+        return RowRange{r[c_lo], r[c_hi], r[i2], r[i2 + 1]};
+    }
+    __device__ __forceinline__ Rec record(uint32_t j) const { return lds[j]; }
+    // global slot of LDS index j (only on the rare FP64 re-evaluation path)
+    __device__ __forceinline__ uint32_t slot(uint32_t j) const {
+        uint32_t lo = 0, hi = 3 * RW;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (off[mid] <= j) lo = mid;
+            else hi = mid;
+        }
+        return gb[lo] + (j - off[lo]);
+    }
+};
+
+// Force on the agent in slot s (index `self` in the source's index space).
+// Candidate order: stencil rows (a, b) ascending, then index ascending, which is
+// voxel-ascending then slot-ascending -- the same for every source.
+template <int D, int POT, bool MIXED, typename Src>
+__device__ __forceinline__ void agent_force(const Arrays& A, const StateBuf& S, const InstParams& P, uint64_t s,
+                                            uint32_t self, uint32_t physical, uint32_t symmetric, const Src& src) {
+    using T = typename std::conditional<MIXED, float, double>::type;
+    using Rec = typename RecOf<MIXED>::type;
+    const uint64_t N = A.N;
+    const uint32_t lv = S.vox[s] - P.voxel_offset;
     uint32_t idx[3];
     idx[2] = lv % P.n[2];
     idx[1] = (lv / P.n[2]) % P.n[1];
@@ -119,30 +184,30 @@ __global__ __launch_bounds__(kBlock) void k_force_point(Arrays A, StateBuf S, ui
         lo[a] = idx[a] == 0 ? 0 : idx[a] - 1;
         hi[a] = min(idx[a] + 2, P.n[a]);
     }
-    using T = typename std::conditional<MIXED, float, double>::type;
     const PotParams<T>& PP = pot_of<T>(P);
     const T rc = cut_radius<POT, T>(PP);
     const float rc2 = (float)(rc * rc);
     const float band = 2e-6f * rc2;
-    const uint64_t N = A.N;
 
-    int qi[3] = {0, 0, 0};
-    double me[3] = {0.0, 0.0, 0.0};
+    const Rec me = src.record(self);
     int sp_i;
     uint32_t flag_i = 0;
     if constexpr (MIXED) {
-        int4 r = A.recq[s];
-        qi[0] = r.x;
-        qi[1] = r.y;
-        qi[2] = r.z;
-        sp_i = r.w & 0x7fffffff;
-        flag_i = (uint32_t)r.w >> 31;
+        sp_i = me.w & 0x7fffffff;
+        flag_i = (uint32_t)me.w >> 31;
     } else {
-        double4 r = A.rec64[s];
-        me[0] = r.x;
-        me[1] = r.y;
-        me[2] = r.z;
-        sp_i = (int)r.w;
+        sp_i = (int)me.w;
+    }
+    // species-matrix row (own species) and column (own species as ext) in registers
+    T srow[CRM_MAX_SPECIES], scol[CRM_MAX_SPECIES];
+#pragma unroll
+    for (int k = 0; k < CRM_MAX_SPECIES; k++) {
+        if constexpr (POT == CRM_POTENTIAL_MORSE_SPECIES) {
+            srow[k] = PP.strength * PP.species_matrix[sp_i * CRM_MAX_SPECIES + k];
+            scol[k] = PP.strength * PP.species_matrix[k * CRM_MAX_SPECIES + sp_i];
+        } else {
+            srow[k] = scol[k] = PP.strength;
+        }
     }
     double accd[D];
     Kahan acck[D];
@@ -153,85 +218,102 @@ __global__ __launch_bounds__(kBlock) void k_force_point(Arrays A, StateBuf S, ui
         acck[k].c = 0.0f;
     }
     bool calc_err = false;
-    if (POT == CRM_POTENTIAL_NONE) hi[0] = lo[0];  // NoInteraction: no pair work
-    for (uint32_t a = lo[0]; a < hi[0]; a++) {
-        for (uint32_t b = lo[1]; b < hi[1]; b++) {
-            const uint32_t row = P.voxel_offset + (a * P.n[1] + b) * P.n[2];
-            uint32_t jb = A.start[row + lo[2]];
-            for (uint32_t c = lo[2]; c < hi[2]; c++) {
-                const uint32_t vv = row + c;
-                const uint32_t je = A.start[vv + 1];
-                if (jb != je) {
-                    const bool full = physical || vv == v;
-                    // fixed-point position of agent i relative to the candidate voxel's origin
-                    int sh[D];
-                    if constexpr (MIXED) {
-                        const int off[3] = {(int)a - (int)idx[0], (int)b - (int)idx[1], (int)c - (int)idx[2]};
+    // Flattened candidate stream: one iteration per candidate across all stencil
+    // rows, so a wave runs max over lanes of the total count rather than the sum
+    // over rows of the per-row maximum.
+    uint32_t ra = lo[0], rb = lo[1], j = 0;
+    bool done = (POT == CRM_POTENTIAL_NONE);
+    RowRange rr{0u, 0u, 0u, 0u};
+    bool own_row = false;
+    int sh[D];
+    auto load_row = [&](uint32_t a, uint32_t b) {
+        rr = src.range(a, b, lo[2], hi[2], idx[2]);
+        own_row = (a == idx[0]) && (b == idx[1]);
+        if constexpr (MIXED) {
+            // fixed-point position of agent i relative to this row's voxel origins
+            const int off01[2] = {(int)a - (int)idx[0], (int)b - (int)idx[1]};
+            const int qi[3] = {me.x, me.y, me.z};
 #pragma unroll
-                        for (int k = 0; k < D; k++) sh[k] = qi[k] - off[k + 3 - D] * kQOne;
+            for (int k = 0; k < D - 1; k++) sh[k] = qi[k] - off01[k + 3 - D] * kQOne;
+            sh[D - 1] = qi[D - 1];
+        }
+        j = rr.jb;
+    };
+    if (!done) load_row(ra, rb);
+    while (true) {
+        while (j >= rr.je) {
+            if (++rb == hi[1]) {
+                rb = lo[1];
+                if (++ra == hi[0]) {
+                    done = true;
+                    break;
+                }
+            }
+            load_row(ra, rb);
+        }
+        if (done) break;
+        if (j != self) {
+            const Rec rj = src.record(j);
+            const bool in_col = (j >= rr.b1) && (j < rr.b2);
+            const bool full = physical || (own_row && in_col);
+            T z[D];
+            int sp_j;
+            int cut = -1;
+            if constexpr (MIXED) {
+                const int qj[3] = {rj.x, rj.y, rj.z};
+                sp_j = rj.w & 0x7fffffff;
+                const int offz = (int)(j >= rr.b1) + (int)(j >= rr.b2) - 1;
+                float d2 = 0.0f;
+#pragma unroll
+                for (int k = 0; k < D; k++) {
+                    const int sk = (k == D - 1) ? sh[k] - offz * kQOne : sh[k];
+                    z[k] = (float)(sk - qj[k]) * P.qunit[k + 3 - D];
+                    d2 = d2 + z[k] * z[k];
+                }
+                const bool refine = (((uint32_t)rj.w >> 31) | flag_i) || d2 == 0.0f || fabsf(d2 - rc2) <= band;
+                if (refine) {
+                    const uint32_t gj = src.slot(j);
+                    double zd[D], dd = 0.0;
+#pragma unroll
+                    for (int k = 0; k < D; k++) {
+                        zd[k] = S.pos[k * N + s] - S.pos[k * N + gj];
+                        dd = dd + zd[k] * zd[k];
+                        z[k] = (float)zd[k];
                     }
-                    for (uint32_t j = jb; j < je; j++) {
-                        if (j == s) continue;
-                        T z[D];
-                        int sp_j;
-                        int cut = -1;
-                        if constexpr (MIXED) {
-                            int4 r = A.recq[j];
-                            const int qj[3] = {r.x, r.y, r.z};
-                            sp_j = r.w & 0x7fffffff;
-                            float d2 = 0.0f;
+                    cut = cut_decision<POT>(P.pd, sqrt(dd));
+                }
+            } else {
+                z[0] = me.x - rj.x;
+                z[1] = me.y - rj.y;
+                if constexpr (D == 3) z[2] = me.z - rj.z;
+                sp_j = (int)rj.w;
+            }
+            T f[D];
+            const bool same_sp = sp_i == sp_j;
+            calc_err |= point_force<POT, T, D>(z, same_sp, pick4<T>(srow, sp_j), PP, cut, f);
+            T wgt = T(0.5);
+            if (full) {
+                if (symmetric) {
+                    wgt = T(1);
+                } else {
+                    T zj[D], g[D];
 #pragma unroll
-                            for (int k = 0; k < D; k++) {
-                                z[k] = (float)(sh[k] - qj[k]) * P.qunit[k + 3 - D];
-                                d2 = d2 + z[k] * z[k];
-                            }
-                            const bool refine = (((uint32_t)r.w >> 31) | flag_i) || d2 == 0.0f ||
-                                                fabsf(d2 - rc2) <= band;
-                            if (refine) {
-                                double zd[D], dd = 0.0;
+                    for (int k = 0; k < D; k++) zj[k] = -z[k];
+                    calc_err |= point_force<POT, T, D>(zj, same_sp, pick4<T>(scol, sp_j), PP, cut, g);
 #pragma unroll
-                                for (int k = 0; k < D; k++) {
-                                    zd[k] = S.pos[k * N + s] - S.pos[k * N + j];
-                                    dd = dd + zd[k] * zd[k];
-                                    z[k] = (float)zd[k];
-                                }
-                                cut = cut_decision<POT>(P.pd, sqrt(dd));
-                            }
-                        } else {
-                            double4 r = A.rec64[j];
-                            z[0] = me[0] - r.x;
-                            z[1] = me[1] - r.y;
-                            if constexpr (D == 3) z[2] = me[2] - r.z;
-                            sp_j = (int)r.w;
-                        }
-                        T f[D];
-                        calc_err |= point_force<POT, T, D>(z, sp_i, sp_j, PP, cut, f);
-                        T wgt = T(0.5);
-                        if (full) {
-                            if (symmetric) {
-                                wgt = T(1);
-                            } else {
-                                T zj[D], g[D];
-#pragma unroll
-                                for (int k = 0; k < D; k++) zj[k] = -z[k];
-                                calc_err |= point_force<POT, T, D>(zj, sp_j, sp_i, PP, cut, g);
-#pragma unroll
-                                for (int k = 0; k < D; k++) {
-                                    if constexpr (MIXED) acck[k].add(-g[k] * 0.5f);
-                                    else accd[k] = accd[k] + (-g[k]) * 0.5;
-                                }
-                            }
-                        }
-#pragma unroll
-                        for (int k = 0; k < D; k++) {
-                            if constexpr (MIXED) acck[k].add(f[k] * wgt);
-                            else accd[k] = accd[k] + f[k] * wgt;
-                        }
+                    for (int k = 0; k < D; k++) {
+                        if constexpr (MIXED) acck[k].add(-g[k] * 0.5f);
+                        else accd[k] = accd[k] + (-g[k]) * 0.5;
                     }
                 }
-                jb = je;
+            }
+#pragma unroll
+            for (int k = 0; k < D; k++) {
+                if constexpr (MIXED) acck[k].add(f[k] * wgt);
+                else accd[k] = accd[k] + f[k] * wgt;
             }
         }
+        ++j;
     }
     if (calc_err) raise_error(A.ctrl, CRM_CALC_ERROR, A.ctrl->step, S.id[s]);
 #pragma unroll
@@ -240,6 +322,99 @@ __global__ __launch_bounds__(kBlock) void k_force_point(Arrays A, StateBuf S, ui
         if constexpr (MIXED) fk = (double)acck[k].s + (double)(-acck[k].c);
         else fk = accd[k];
         A.force[(uint64_t)k * N + s] = fk;
+    }
+}
+
+// A force tile: rows b in [b0, b1) of plane a0 of one instance, voxels c in
+// [c0, c1) of those rows (either whole rows, or one row cut along its length).
+// Its own agents are one contiguous slot range; the agents of its 3^D stencil
+// are at most 3 * (rows + 2) contiguous row segments.
+constexpr int kTileRowsMax = 8;
+constexpr int kTileTable = 3 * (kTileRowsMax + 2);
+constexpr int kLdsRecordsMixed = 2048;  // 32 KiB of int4
+constexpr int kLdsRecordsFP64 = 1280;   // 40 KiB of double4
+constexpr int kLdsVoxelStarts = 1536;   // 6 KiB of voxel-start indices
+
+struct ForceTile {
+    uint32_t inst, a0, b0, b1, c0, c1;
+};
+
+// Workgroup per tile: stage the stencil's row segments and their voxel starts in
+// LDS with coalesced loads, then one lane per own agent sums its forces from
+// LDS.  A tile whose neighbourhood does not fit (local crowding) reads from
+// global memory instead; the arithmetic and its order are the same.
+template <int D, int POT, bool MIXED>
+__global__ __launch_bounds__(kBlock) void k_force_tiled(Arrays A, StateBuf S, const ForceTile* __restrict__ tiles,
+                                                          uint32_t physical, uint32_t symmetric, uint32_t no_lds) {
+    using Rec = typename RecOf<MIXED>::type;
+    constexpr int CAP = MIXED ? kLdsRecordsMixed : kLdsRecordsFP64;
+    __shared__ Rec lds[CAP];
+    __shared__ uint32_t vst[kLdsVoxelStarts];
+    __shared__ uint32_t t_gb[kTileTable];
+    __shared__ uint32_t t_off[kTileTable + 1];
+    const Rec* __restrict__ rec = rec_array(A, std::integral_constant<bool, MIXED>());
+    const ForceTile T = tiles[xcd_block(blockIdx.x, gridDim.x)];
+    const InstParams& P = A.ip[T.inst];
+    const uint32_t n0 = P.n[0], n1 = P.n[1], n2 = P.n[2], voff = P.voxel_offset;
+    const uint32_t RW = T.b1 - T.b0 + 2;
+    const uint32_t cl = T.c0 == 0 ? 0u : T.c0 - 1u, ch = min(T.c1 + 1u, n2);
+    const uint32_t W1 = ch - cl + 1;
+    const uint32_t n_rows = 3 * RW;
+    if (threadIdx.x < n_rows) {
+        const uint32_t pa = threadIdx.x / RW, rb = threadIdx.x % RW;
+        const int a = (int)T.a0 - 1 + (int)pa, b = (int)T.b0 - 1 + (int)rb;
+        uint32_t gb = 0, len = 0;
+        if (a >= 0 && a < (int)n0 && b >= 0 && b < (int)n1) {
+            const uint32_t base = voff + ((uint32_t)a * n1 + (uint32_t)b) * n2;
+            gb = A.start[base + cl];
+            len = A.start[base + ch] - gb;
+        }
+        t_gb[threadIdx.x] = gb;
+        t_off[threadIdx.x + 1] = len;  // lengths; prefix-summed below
+    }
+    if (threadIdx.x == 0) t_off[0] = 0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (uint32_t r = 1; r <= n_rows; r++) t_off[r] += t_off[r - 1];
+    }
+    __syncthreads();
+    const uint32_t total = t_off[n_rows];
+    const bool use_lds = !no_lds && total <= (uint32_t)CAP && n_rows * W1 <= (uint32_t)kLdsVoxelStarts;
+    const uint32_t own_b = A.start[voff + (T.a0 * n1 + T.b0) * n2 + T.c0];
+    const uint32_t own_e = A.start[voff + (T.a0 * n1 + T.b1 - 1) * n2 + T.c1];
+    if (use_lds) {
+        for (uint32_t e = threadIdx.x; e < n_rows * W1; e += kBlock) {
+            const uint32_t r = e / W1, c = cl + e % W1;
+            const uint32_t pa = r / RW, rb = r % RW;
+            const int a = (int)T.a0 - 1 + (int)pa, b = (int)T.b0 - 1 + (int)rb;
+            uint32_t x = t_off[r];
+            if (a >= 0 && a < (int)n0 && b >= 0 && b < (int)n1)
+                x = A.start[voff + ((uint32_t)a * n1 + (uint32_t)b) * n2 + c] - t_gb[r] + t_off[r];
+            vst[e] = x;
+        }
+        for (uint32_t i = threadIdx.x; i < total; i += kBlock) {
+            uint32_t lo = 0, hi = n_rows;  // largest lo with t_off[lo] <= i
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (t_off[mid] <= i) lo = mid;
+                else hi = mid;
+            }
+            lds[i] = rec[t_gb[lo] + (i - t_off[lo])];
+        }
+        __syncthreads();
+        LdsSource<MIXED> src{lds, vst, t_gb, t_off, T.a0, T.b0, RW, W1, cl};
+        const uint32_t own_row = RW;  // own rows are pa = 1
+        for (uint32_t s = own_b + threadIdx.x; s < own_e; s += kBlock) {
+            // own agent's row inside the tile: its slot lies in the staged row segment
+            const uint32_t lv = S.vox[s] - voff;
+            const uint32_t b = (lv / n2) % n1;
+            const uint32_t rt = own_row + (b + 1 - T.b0);
+            agent_force<D, POT, MIXED>(A, S, P, s, s - t_gb[rt] + t_off[rt], physical, symmetric, src);
+        }
+    } else {
+        GlobalSource<MIXED> src{rec, A.start, voff, n1, n2};
+        for (uint32_t s = own_b + threadIdx.x; s < own_e; s += kBlock)
+            agent_force<D, POT, MIXED>(A, S, P, s, s, physical, symmetric, src);
     }
 }
 
@@ -299,8 +474,126 @@ __device__ __forceinline__ void point_increment(const InstParams& P, const doubl
     }
 }
 
+// Ring bookkeeping for a step that starts after t completed steps.
+struct Ring {
+    uint32_t cap, wslot, n_read;
+    uint32_t it[2];  // slots yielded by the iterator after the push
+};
+
+__device__ __forceinline__ Ring ring_at(uint64_t t, uint32_t cap, uint32_t newest_first) {
+    Ring r;
+    r.cap = cap;
+    r.wslot = cap ? (uint32_t)(t % cap) : 0u;
+    const uint32_t size = (uint32_t)min<uint64_t>(t + 1, cap);
+    const uint32_t oldest = size < cap ? 0u : (uint32_t)((t + 1) % (cap ? cap : 1u));
+    r.n_read = (cap == 2 && size == 2) ? 2u : 1u;
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+        r.it[i] = cap == 0 ? 0u
+                           : (newest_first ? (oldest + size - 1 - i + 2 * cap) % cap : (oldest + i) % cap);
+    return r;
+}
+
+// Standard normals of this agent and step (buffer or Philox).
+template <int D>
+__device__ __forceinline__ bool point_noise(const StepArgs& sa, uint32_t id, uint64_t t, double* z) {
+    if (sa.noise_mode == CRM_NOISE_BUFFER && sa.noise) {
+#pragma unroll
+        for (int k = 0; k < D; k++) z[k] = sa.noise[(uint64_t)id * D + k];
+        return true;
+    }
+    if (sa.noise_mode == CRM_NOISE_PHILOX) {
+        double g[4];
+        gaussian_pair(sa.seed, id, t, 0, g[0], g[1]);
+        if (D > 2) gaussian_pair(sa.seed, id, t, 1, g[2], g[3]);
+#pragma unroll
+        for (int k = 0; k < D; k++) z[k] = g[k];
+        return true;
+    }
+#pragma unroll
+    for (int k = 0; k < D; k++) z[k] = 0.0;
+    return false;
+}
+
+// One chili mechanics update of a point agent (solvers.rs:23-55,156-347 via
+// local_mechanics_update, update_mechanics.rs:530-559) followed by
+// apply_boundary (cartesian_cuboid_n.rs:594-631).  hist(i, dv) reads the value
+// of the ring slot it[i] (never the slot written this step).  With
+// WITH_VEL == false only the position is produced (the velocity, increments
+// and, for NewtonDamped / Langevin, the noise do not influence it).
+template <int D, int MECH, bool WITH_VEL, typename Hist>
+__device__ __forceinline__ bool point_step(const InstParams& P, const StepArgs& sa, const Ring& rg, uint64_t t,
+                                           uint32_t id, const double* x, const double* v, const double* f,
+                                           Hist hist, double* nx, double* nv, double* dx, double* dv) {
+    double z[D];
+    const bool need_noise = WITH_VEL || MECH == CRM_MECHANICS_BROWNIAN;
+    bool have_noise = false;
+    if (need_noise) have_noise = point_noise<D>(sa, id, t, z);
+    else {
+#pragma unroll
+        for (int k = 0; k < D; k++) z[k] = 0.0;
+    }
+    double dxr[D], dvr[D];
+    point_increment<MECH, D>(P, x, v, f, z, have_noise, sa.dt, sa.sqrt_dt, dx, dv, dxr, dvr);
+    if (rg.cap == 0) {  // euler, solvers.rs:302-310
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+            nx[k] = (dx[k] * sa.dt + x[k]) * 1.0 + dxr[k] * sa.dt;
+            if (WITH_VEL) nv[k] = (dv[k] * sa.dt + v[k]) * 1.0 + dvr[k] * sa.dt;
+        }
+    } else if (rg.n_read == 2) {  // adams_bashforth_3, solvers.rs:312-329
+        const double f0 = 23.0 / 12.0, f1 = (-16.0) / 12.0, f2 = 5.0 / 12.0;
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+            const double h0 = rg.it[0] == rg.wslot ? dx[k] : hist(0, k, false);
+            const double h1 = rg.it[1] == rg.wslot ? dx[k] : hist(1, k, false);
+            const double a = (dx[k] * f0 + (h0 * f1 + h1 * f2)) * sa.dt + x[k];
+            nx[k] = a * 1.0 + dxr[k] * sa.dt;
+            if (WITH_VEL) {
+                const double g0 = rg.it[0] == rg.wslot ? dv[k] : hist(0, k, true);
+                const double g1 = rg.it[1] == rg.wslot ? dv[k] : hist(1, k, true);
+                const double b = (dv[k] * f0 + (g0 * f1 + g1 * f2)) * sa.dt + v[k];
+                nv[k] = b * 1.0 + dvr[k] * sa.dt;
+            }
+        }
+    } else {  // adams_bashforth_2, solvers.rs:331-347
+        const double g0c = 3.0 / 2.0, g1c = (-1.0) / 2.0;
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+            const double h0 = rg.it[0] == rg.wslot ? dx[k] : hist(0, k, false);
+            nx[k] = ((dx[k] * g0c + h0 * g1c) * sa.dt + x[k]) * 1.0 + dxr[k] * sa.dt;
+            if (WITH_VEL) {
+                const double q0 = rg.it[0] == rg.wslot ? dv[k] : hist(0, k, true);
+                nv[k] = ((dv[k] * g0c + q0 * g1c) * sa.dt + v[k]) * 1.0 + dvr[k] * sa.dt;
+            }
+        }
+    }
+    if (WITH_VEL && MECH == CRM_MECHANICS_BROWNIAN) {
+#pragma unroll
+        for (int k = 0; k < D; k++) nv[k] = v[k];  // Brownian::set_velocity is a no-op
+    }
+    bool out = false;
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+        const int ax = k + 3 - D;
+        if (nx[k] < P.min[ax]) {
+            nx[k] = 2.0 * P.min[ax] - nx[k];
+            if (WITH_VEL) nv[k] = fabs(nv[k]);
+        }
+        if (nx[k] > P.max[ax]) {
+            nx[k] = 2.0 * P.max[ax] - nx[k];
+            if (WITH_VEL) nv[k] = -fabs(nv[k]);
+        }
+        out |= (nx[k] < P.min[ax]) || (nx[k] > P.max[ax]);
+    }
+    return out;
+}
+
+// Next voxel of every agent (get_index_of, cartesian_cuboid_n.rs:469-491) from
+// the position it will have after this step, and its arrival rank there.  A
+// position whose index has no voxel keeps the old one (update_mechanics.rs:430-453).
 template <int D, int MECH>
-__global__ __launch_bounds__(kBlock) void k_integrate_point(Arrays A, StateBuf S, StepArgs sa) {
+__global__ __launch_bounds__(kBlock) void k_key_point(Arrays A, StateBuf S, StepArgs sa) {
     const BlockWork w = A.work[blockIdx.x];
     if (threadIdx.x >= w.count) return;
     const uint64_t s = w.first + threadIdx.x;
@@ -308,116 +601,17 @@ __global__ __launch_bounds__(kBlock) void k_integrate_point(Arrays A, StateBuf S
     const uint64_t N = A.N;
     const uint64_t t = A.ctrl->step;
     const uint32_t id = S.id[s];
-
-    double x[D], v[D], f[D], z[D];
+    const Ring rg = ring_at(t, sa.cap, sa.newest_first);
+    double x[D], v[D], f[D], nx[D], nv[D], dx[D], dv[D];
 #pragma unroll
     for (int k = 0; k < D; k++) {
         x[k] = S.pos[k * N + s];
-        v[k] = S.vel[k * N + s];
-        f[k] = A.force[k * N + s];
-        z[k] = 0.0;
+        v[k] = MECH == CRM_MECHANICS_BROWNIAN ? 0.0 : S.vel[k * N + s];
+        f[k] = MECH == CRM_MECHANICS_BROWNIAN ? A.force[k * N + s] : 0.0;
     }
-    bool have_noise = false;
-    if (sa.noise_mode == CRM_NOISE_BUFFER && sa.noise) {
-#pragma unroll
-        for (int k = 0; k < D; k++) z[k] = sa.noise[(uint64_t)id * D + k];
-        have_noise = true;
-    } else if (sa.noise_mode == CRM_NOISE_PHILOX) {
-        double g[4];
-        gaussian_pair(sa.seed, id, t, 0, g[0], g[1]);
-        if (D > 2) gaussian_pair(sa.seed, id, t, 1, g[2], g[3]);
-#pragma unroll
-        for (int k = 0; k < D; k++) z[k] = g[k];
-        have_noise = true;
-    }
-    double dx[D], dv[D], dxr[D], dvr[D];
-    point_increment<MECH, D>(P, x, v, f, z, have_noise, sa.dt, sa.sqrt_dt, dx, dv, dxr, dvr);
-
-    // ring push + iterator (see comment above)
-    const uint32_t cap = sa.cap;
-    double nx[D], nvel[D];
-    if (cap == 0) {
-#pragma unroll
-        for (int k = 0; k < D; k++) {
-            nx[k] = (dx[k] * sa.dt + x[k]) * 1.0 + dxr[k] * sa.dt;
-            nvel[k] = (dv[k] * sa.dt + v[k]) * 1.0 + dvr[k] * sa.dt;
-        }
-    } else {
-        const uint32_t wslot = (uint32_t)(t % cap);
-        const uint32_t size = (uint32_t)min<uint64_t>(t + 1, cap);
-        const uint32_t oldest = size < cap ? 0u : (uint32_t)((t + 1) % cap);
-        uint32_t it[2];
-#pragma unroll
-        for (int i = 0; i < 2; i++)
-            it[i] = sa.newest_first ? (oldest + size - 1 - i + 2 * cap) % cap : (oldest + i) % cap;
-        // which yielded element is the value pushed this step
-        double h[2][D], hv[2][D];
-        const uint32_t n_read = (cap == 2 && size == 2) ? 2u : 1u;
-        for (uint32_t i = 0; i < n_read; i++) {
-#pragma unroll
-            for (int k = 0; k < D; k++) {
-                if (it[i] == wslot) {
-                    h[i][k] = dx[k];
-                    hv[i][k] = dv[k];
-                } else {
-                    h[i][k] = S.hdx[((uint64_t)it[i] * D + k) * N + s];
-                    hv[i][k] = S.hdv[((uint64_t)it[i] * D + k) * N + s];
-                }
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < D; k++) {
-            S.hdx[((uint64_t)wslot * D + k) * N + s] = dx[k];
-            S.hdv[((uint64_t)wslot * D + k) * N + s] = dv[k];
-        }
-        if (n_read == 2) {  // adams_bashforth_3, solvers.rs:312-329
-            const double f0 = 23.0 / 12.0, f1 = (-16.0) / 12.0, f2 = 5.0 / 12.0;
-#pragma unroll
-            for (int k = 0; k < D; k++) {
-                double a = (dx[k] * f0 + (h[0][k] * f1 + h[1][k] * f2)) * sa.dt + x[k];
-                nx[k] = a * 1.0 + dxr[k] * sa.dt;
-                double b = (dv[k] * f0 + (hv[0][k] * f1 + hv[1][k] * f2)) * sa.dt + v[k];
-                nvel[k] = b * 1.0 + dvr[k] * sa.dt;
-            }
-        } else {  // adams_bashforth_2, solvers.rs:331-347
-            const double g0 = 3.0 / 2.0, g1 = (-1.0) / 2.0;
-#pragma unroll
-            for (int k = 0; k < D; k++) {
-                nx[k] = ((dx[k] * g0 + h[0][k] * g1) * sa.dt + x[k]) * 1.0 + dxr[k] * sa.dt;
-                nvel[k] = ((dv[k] * g0 + hv[0][k] * g1) * sa.dt + v[k]) * 1.0 + dvr[k] * sa.dt;
-            }
-        }
-    }
-    if (MECH == CRM_MECHANICS_BROWNIAN) {
-#pragma unroll
-        for (int k = 0; k < D; k++) nvel[k] = v[k];  // Brownian::set_velocity is a no-op
-    }
-
-    // apply_boundary (cartesian_cuboid_n.rs:594-631)
-    bool out = false;
-#pragma unroll
-    for (int k = 0; k < D; k++) {
-        const int ax = k + 3 - D;
-        if (nx[k] < P.min[ax]) {
-            nx[k] = 2.0 * P.min[ax] - nx[k];
-            nvel[k] = fabs(nvel[k]);
-        }
-        if (nx[k] > P.max[ax]) {
-            nx[k] = 2.0 * P.max[ax] - nx[k];
-            nvel[k] = -fabs(nvel[k]);
-        }
-        out |= (nx[k] < P.min[ax]) || (nx[k] > P.max[ax]);
-    }
+    auto hist = [&](int i, int k, bool) { return S.hdx[((uint64_t)rg.it[i] * D + k) * N + s]; };
+    const bool out = point_step<D, MECH, false>(P, sa, rg, t, id, x, v, f, hist, nx, nv, dx, dv);
     if (out) raise_error(A.ctrl, CRM_BOUNDARY_ERROR, t, id);
-
-#pragma unroll
-    for (int k = 0; k < D; k++) {
-        S.pos[k * N + s] = nx[k];
-        S.vel[k * N + s] = nvel[k];
-    }
-
-    // next voxel (get_index_of, cartesian_cuboid_n.rs:469-491); a point whose
-    // index has no voxel keeps its voxel (update_mechanics.rs:430-453)
     uint32_t idx[3] = {0, 0, 0};
     bool ok = true;
 #pragma unroll
@@ -582,8 +776,37 @@ __global__ __launch_bounds__(kBlock) void k_fixup(Arrays A) {
     }
 }
 
-// Permutes the state into sorted order and writes the force records.  The ring
-// holds min(step, cap) increments (k_scan_tiles has already advanced `step`).
+// Force record of slot s: absolute double coordinates (FP64), or 30-bit fixed
+// point relative to the voxel origin (mixed; the sign bit of w flags an agent
+// outside its voxel's fixed-point range, which sends its pairs to the FP64 path).
+template <int D, bool MIXED>
+__device__ __forceinline__ void write_record(const Arrays& A, const InstParams& P, uint64_t s, uint32_t key,
+                                             const double* p, int32_t sp) {
+    if constexpr (MIXED) {
+        const uint32_t lv = key - P.voxel_offset;
+        uint32_t idx[3];
+        idx[2] = lv % P.n[2];
+        idx[1] = (lv / P.n[2]) % P.n[1];
+        idx[0] = lv / (P.n[2] * P.n[1]);
+        int q[3] = {0, 0, 0};
+        uint32_t outside = 0;
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+            const int ax = k + 3 - D;
+            const double origin = P.min[ax] + (double)idx[ax] * P.dx[ax];
+            const double tq = floor((p[k] - origin) * P.qinv[ax]);
+            const bool ok = tq >= 0.0 && tq < (double)kQOne;  // false for NaN
+            q[k] = ok ? (int)tq : (tq < 0.0 ? 0 : kQOne - 1);
+            outside |= ok ? 0u : 1u;
+        }
+        A.recq[s] = make_int4(q[0], q[1], D == 3 ? q[2] : 0, (int)((uint32_t)sp | (outside << 31)));
+    } else {
+        A.rec64[s] = make_double4(p[0], p[1], D == 3 ? p[D - 1] : 0.0, (double)sp);
+    }
+}
+
+// Permutes the state into sorted order and writes the force records, without
+// integrating (initial placement).  The ring holds min(step, cap) increments.
 template <int D, bool MIXED>
 __global__ __launch_bounds__(kBlock) void k_gather_point(Arrays A, StateBuf src, StateBuf dst, uint32_t cap) {
     const BlockWork w = A.work[blockIdx.x];
@@ -612,27 +835,61 @@ __global__ __launch_bounds__(kBlock) void k_gather_point(Arrays A, StateBuf src,
     dst.id[s] = src.id[o];
     const uint32_t key = A.key[o];
     dst.vox[s] = key;
-    if constexpr (MIXED) {
-        const uint32_t lv = key - P.voxel_offset;
-        uint32_t idx[3];
-        idx[2] = lv % P.n[2];
-        idx[1] = (lv / P.n[2]) % P.n[1];
-        idx[0] = lv / (P.n[2] * P.n[1]);
-        int q[3] = {0, 0, 0};
-        uint32_t outside = 0;
+    write_record<D, MIXED>(A, P, s, key, p, sp);
+}
+
+// Gather + integrate: the agent that lands in sorted slot s comes from slot
+// o = perm[s]; its update (the same point_step as in k_key_point, so the same
+// position) is computed here and the new state written straight to slot s,
+// together with the force record for the next step.  k_scan_tiles has already
+// advanced the step counter, so this step started after step - 1 steps.
+template <int D, bool MIXED, int MECH>
+__global__ __launch_bounds__(kBlock) void k_gather_integrate_point(Arrays A, StateBuf src, StateBuf dst,
+                                                                     StepArgs sa) {
+    const BlockWork w = A.work[blockIdx.x];
+    if (threadIdx.x >= w.count) return;
+    const uint64_t s = w.first + threadIdx.x;
+    const InstParams& P = A.ip[w.inst];
+    const uint64_t N = A.N;
+    const uint64_t t = A.ctrl->step - 1;
+    const uint32_t o = A.perm[s];
+    const uint32_t id = src.id[o];
+    const Ring rg = ring_at(t, sa.cap, sa.newest_first);
+    double x[D], v[D], f[D], nx[D], nv[D], dx[D], dv[D];
 #pragma unroll
-        for (int k = 0; k < D; k++) {
-            const int ax = k + 3 - D;
-            const double origin = P.min[ax] + (double)idx[ax] * P.dx[ax];
-            const double t = floor((p[k] - origin) * P.qinv[ax]);
-            const bool ok = t >= 0.0 && t < (double)kQOne;  // false for NaN
-            q[k] = ok ? (int)t : (t < 0.0 ? 0 : kQOne - 1);
-            outside |= ok ? 0u : 1u;
-        }
-        A.recq[s] = make_int4(q[0], q[1], D == 3 ? q[2] : 0, (int)((uint32_t)sp | (outside << 31)));
-    } else {
-        A.rec64[s] = make_double4(p[0], p[1], D == 3 ? p[D - 1] : 0.0, (double)sp);
+    for (int k = 0; k < D; k++) {
+        x[k] = src.pos[k * N + o];
+        v[k] = src.vel[k * N + o];
+        f[k] = A.force[k * N + o];
     }
+    auto hist = [&](int i, int k, bool is_dv) {
+        const double* h = is_dv ? src.hdv : src.hdx;
+        return h[((uint64_t)rg.it[i] * D + k) * N + o];
+    };
+    point_step<D, MECH, true>(P, sa, rg, t, id, x, v, f, hist, nx, nv, dx, dv);
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+        dst.pos[k * N + s] = nx[k];
+        dst.vel[k * N + s] = nv[k];
+    }
+    // ring: the slot written this step gets (dx, dv), the other slot is carried over
+    if (rg.cap > 0) {
+        const uint32_t n_hist = (uint32_t)min<uint64_t>(t + 1, rg.cap);
+        for (uint32_t h = 0; h < n_hist; h++) {
+#pragma unroll
+            for (int k = 0; k < D; k++) {
+                const uint64_t e = ((uint64_t)h * D + k) * N;
+                dst.hdx[e + s] = h == rg.wslot ? dx[k] : src.hdx[e + o];
+                dst.hdv[e + s] = h == rg.wslot ? dv[k] : src.hdv[e + o];
+            }
+        }
+    }
+    const int32_t sp = src.species[o];
+    dst.species[s] = sp;
+    dst.id[s] = id;
+    const uint32_t key = A.key[o];
+    dst.vox[s] = key;
+    write_record<D, MIXED>(A, P, s, key, nx, sp);
 }
 
 }  // namespace crm

@@ -55,6 +55,7 @@ class MechanicsBackend:
         device: int = 0,
         graph: bool = True,
         timing: bool = False,
+        lds: bool = True,
     ):
         self._lib = load()
         if not instances:
@@ -73,7 +74,8 @@ class MechanicsBackend:
         cfg.seed = seed
         cfg.dt = dt
         cfg.device = device
-        cfg.flags = (_abi.FLAG_GRAPH if graph else 0) | (_abi.FLAG_TIMING if timing else 0)
+        cfg.flags = ((_abi.FLAG_GRAPH if graph else 0) | (_abi.FLAG_TIMING if timing else 0)
+                     | (0 if lds else _abi.FLAG_NO_LDS))
         arr = (_abi.Instance * len(instances))()
         for i, spec in enumerate(instances):
             if spec.domain.dim != dim:

@@ -105,7 +105,8 @@ typedef enum crm_noise_mode {
 
 enum {
     CRM_FLAG_GRAPH = 1u,    /* replay the per-step launch sequence as a captured hipGraph */
-    CRM_FLAG_TIMING = 2u    /* record HIP events around every kernel (disables graphs) */
+    CRM_FLAG_TIMING = 2u,   /* record HIP events around every kernel (disables graphs) */
+    CRM_FLAG_NO_LDS = 4u    /* force kernel reads candidates from global memory (testing) */
 };
 
 /* CartesianCuboid, cartesian_cuboid_n.rs:65-72,183-256.

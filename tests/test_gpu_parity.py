@@ -250,6 +250,33 @@ def test_mixed_precision_cutoff_band_and_close_pairs(potential):
     assert rel_err(fg, fo) <= MIXED_TOL, rel_err(fg, fo)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_lds_tiles_equal_global_path_and_crowded_tiles(precision):
+    """The force kernel stages each tile's neighbourhood in LDS; a tile whose
+    neighbourhood overflows LDS (a crowded cluster in a sparse domain) reads from
+    global memory.  Both paths do the same arithmetic in the same order."""
+    rng = np.random.default_rng(31)
+    L = 200.0
+    dom = CartesianCuboid.from_boundaries_and_interaction_range([0.0] * 3, [L] * 3, 2.5)
+    sparse = rng.uniform(1e-6, L, (20000, 3))
+    cluster = rng.normal(100.0, 1.5, (6000, 3))
+    pos = np.concatenate([sparse, cluster])
+    sp = (rng.random(len(pos)) < 0.5).astype(np.int32)
+    pot = SpeciesMorsePotential(1.0, 1.0, 2.5, 0.5, [[1.0, 0.2], [0.2, 1.0]])
+    spec = [InstanceSpec(dom, pot, Langevin(1.0, 2.0, 0.01))]
+    F = []
+    for lds in (True, False):
+        g = MechanicsBackend(spec, dt=0.005, precision=precision, lds=lds)
+        g.set_agents(pos, species=sp)
+        F.append(g.compute_forces())
+        g.close()
+    assert np.array_equal(F[0], F[1])
+    o = OracleSim(spec, dt=0.005)
+    o.set_agents(pos, species=sp)
+    tol = FORCE_TOL if precision == "fp64" else MIXED_TOL
+    assert rel_err(F[0], o.compute_forces()) <= tol
+
+
 def test_ensemble_matches_oracle():
     wl = workloads.c4_ensemble_2d(n_instances=12, seed=15, steps=20, n_min=50, n_max=800)
     g, o = pair(wl)
