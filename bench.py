@@ -38,6 +38,21 @@ def candidates_per_agent(wl, start):
     return float((stencil * cnt).sum() / cnt.sum())
 
 
+PROFILE = os.path.join(ROOT, "profiles", "c2_cell_sorting_1M.md")
+FORCE_KERNEL = "k_force_tiled<3, 3, true>"
+
+
+def measured_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (counters scaled by a copy kernel of known size), or None."""
+    try:
+        text = open(PROFILE).read()
+        block = text.split("```json", 1)[1].split("```", 1)[0]
+        return json.loads(block)["hbm_bytes_per_dispatch"].get(kernel)
+    except (OSError, IndexError, KeyError, ValueError):
+        return None
+
+
 def force_bytes_per_agent(cbar, record_bytes, dim, stencil=27):
     # own record + (start, end) per stencil voxel + candidate records + force write (double)
     return record_bytes + 8 * stencil + cbar * record_bytes + 8 * dim
@@ -91,10 +106,22 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 256 MiB device copy (reference for the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": src.numel() * 4, "copy_bytes_written": dst.numel() * 4}))
+        return
     if args.gpus != 1:
-        print(json.dumps({"error": "this backend runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
-                                   "described in DESIGN.md, not built)"}))
+        print(json.dumps({"error": "this backend runs on one GPU; --gpus must be 1 (slab sharding across "
+                                   "GPUs is designed, not built)"}))
         sys.exit(2)
     if int(os.environ.get("RANK", "0")) != 0:
         return
@@ -169,8 +196,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "kernel": "k_force_point<3, MORSE_SPECIES, mixed>",
+            "traffic": measured_traffic(FORCE_KERNEL),
+            "traffic_source": "profiles/c2_cell_sorting_1M.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+            "kernel": FORCE_KERNEL,
             "bytes_per_agent": bf,
             "candidates_per_agent": cbar,
         },

@@ -25,9 +25,11 @@
 // reference doctests and unit tests (nearest point, grid, decomposition,
 // stencil, Euler / AB bounds), free-drift and harmonic-spring closed forms,
 // Langevin MSD statistics, brute-force voxel assignment, and Philox KAT vectors.
-// The pair weighting and ring-buffer order are pinned only by reading the
-// reference code (no reference test holds them): "parity unpinned" beyond the
-// structural checks in tests/test_oracle_engine.py.
+// The pair weighting and the order in which a full ring buffer is iterated are
+// pinned by reading the reference code and by structural tests derived from it
+// (tests/test_oracle_kats.py: same-voxel / adjacent-voxel / cross-subdomain
+// weights, the AB recursion under both iteration orders); no reference test or
+// fixture holds their values, so beyond those checks they are parity unpinned.
 
 #include <algorithm>
 #include <atomic>
