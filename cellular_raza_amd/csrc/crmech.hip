@@ -14,7 +14,7 @@
 #include <vector>
 
 #include "crmech.h"
-#include "kernels_point.hpp"
+#include "kernels_rod.hpp"
 
 namespace crm {
 
@@ -157,6 +157,8 @@ struct crm_ctx {
     int cur = 0;
     double* d_noise = nullptr;
     size_t noise_cap = 0;
+    double* d_recrod = nullptr;  // rods: AoS vertex positions [N][K]
+    uint32_t rods_per_block = 0;
 
     uint64_t N = 0;
     std::vector<uint64_t> counts;
@@ -180,7 +182,7 @@ thread_local std::string g_create_error;
 
 void free_device(crm_ctx* ctx) {
     void* ptrs[] = {ctx->d_work, ctx->d_tiles, ctx->d_start, ctx->d_count, ctx->d_bsum, ctx->d_key, ctx->d_rank,
-                    ctx->d_perm, ctx->d_force, ctx->d_rec64, ctx->d_recq, ctx->d_noise};
+                    ctx->d_perm, ctx->d_force, ctx->d_rec64, ctx->d_recq, ctx->d_noise, ctx->d_recrod};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& b : ctx->buf) {
@@ -196,6 +198,7 @@ void free_device(crm_ctx* ctx) {
     ctx->d_rec64 = nullptr;
     ctx->d_recq = nullptr;
     ctx->d_noise = nullptr;
+    ctx->d_recrod = nullptr;
     ctx->noise_cap = 0;
     for (auto& g : ctx->graph)
         if (g) {
@@ -220,6 +223,16 @@ Arrays arrays(crm_ctx* ctx) {
     A.N = ctx->N;
     A.Vt = (uint32_t)ctx->Vt;
     return A;
+}
+
+bool is_rod(const crm_ctx* ctx) { return ctx->mech_kind == CRM_MECHANICS_ROD; }
+
+RodArgs rod_args(crm_ctx* ctx) {
+    RodArgs R;
+    R.rec = ctx->d_recrod;
+    R.nv = ctx->nv;
+    R.rpb = ctx->rods_per_block;
+    return R;
 }
 
 bool timing_on(crm_ctx* ctx) { return (ctx->cfg.flags & CRM_FLAG_TIMING) != 0; }
@@ -276,10 +289,39 @@ crm_status launch_force_d(crm_ctx* ctx, StateBuf S) {
     return CRM_INVALID_ARGUMENT;
 }
 
+template <int D>
+crm_status launch_force_rod_d(crm_ctx* ctx, StateBuf S) {
+    Arrays A = arrays(ctx);
+    const RodArgs R = rod_args(ctx);
+    const uint32_t phys = ctx->cfg.weighting == CRM_WEIGHT_PHYSICAL;
+    const dim3 g(ctx->n_work), b(kBlock);
+    switch (ctx->pot_kind) {
+        case CRM_POTENTIAL_MORSE:
+            hipLaunchKernelGGL((k_force_rod<D, CRM_POTENTIAL_MORSE>), g, b, 0, ctx->stream, A, S, R, phys);
+            return CRM_OK;
+        case CRM_POTENTIAL_MORSE_SPECIES:
+            hipLaunchKernelGGL((k_force_rod<D, CRM_POTENTIAL_MORSE_SPECIES>), g, b, 0, ctx->stream, A, S, R, phys);
+            return CRM_OK;
+        case CRM_POTENTIAL_MIE:
+            hipLaunchKernelGGL((k_force_rod<D, CRM_POTENTIAL_MIE>), g, b, 0, ctx->stream, A, S, R, phys);
+            return CRM_OK;
+        case CRM_POTENTIAL_BOUND_LENNARD_JONES:
+            hipLaunchKernelGGL((k_force_rod<D, CRM_POTENTIAL_BOUND_LENNARD_JONES>), g, b, 0, ctx->stream, A, S, R,
+                               phys);
+            return CRM_OK;
+        case CRM_POTENTIAL_NONE:
+            hipLaunchKernelGGL((k_force_rod<D, CRM_POTENTIAL_NONE>), g, b, 0, ctx->stream, A, S, R, phys);
+            return CRM_OK;
+    }
+    return CRM_INVALID_ARGUMENT;
+}
+
 crm_status launch_force(crm_ctx* ctx) {
     CRM_TRY(t_begin(ctx));
     StateBuf S = ctx->buf[ctx->cur];
-    crm_status st = ctx->D == 2 ? launch_force_d<2>(ctx, S) : launch_force_d<3>(ctx, S);
+    crm_status st;
+    if (is_rod(ctx)) st = ctx->D == 2 ? launch_force_rod_d<2>(ctx, S) : launch_force_rod_d<3>(ctx, S);
+    else st = ctx->D == 2 ? launch_force_d<2>(ctx, S) : launch_force_d<3>(ctx, S);
     if (st) return st;
     return t_end(ctx, CRM_KERNEL_FORCE);
 }
@@ -320,7 +362,17 @@ crm_status launch_key_d(crm_ctx* ctx, StateBuf S, const StepArgs& sa) {
 crm_status launch_key(crm_ctx* ctx, const StepArgs& sa) {
     CRM_TRY(t_begin(ctx));
     StateBuf S = ctx->buf[ctx->cur];
-    crm_status st = ctx->D == 2 ? launch_key_d<2>(ctx, S, sa) : launch_key_d<3>(ctx, S, sa);
+    crm_status st = CRM_OK;
+    if (is_rod(ctx)) {
+        Arrays A = arrays(ctx);
+        const RodArgs R = rod_args(ctx);
+        if (ctx->D == 2)
+            hipLaunchKernelGGL((k_key_rod<2>), dim3(ctx->n_work), dim3(kBlock), 0, ctx->stream, A, S, R, sa);
+        else
+            hipLaunchKernelGGL((k_key_rod<3>), dim3(ctx->n_work), dim3(kBlock), 0, ctx->stream, A, S, R, sa);
+    } else {
+        st = ctx->D == 2 ? launch_key_d<2>(ctx, S, sa) : launch_key_d<3>(ctx, S, sa);
+    }
     if (st) return st;
     return t_end(ctx, CRM_KERNEL_INTEGRATE);
 }
@@ -369,7 +421,18 @@ crm_status launch_sort(crm_ctx* ctx, bool advance, const StepArgs* sa) {
     const uint32_t cap = ctx->cfg.solver_order;
     const bool mixed = ctx->cfg.precision == CRM_PRECISION_MIXED;
     crm_status gst = CRM_OK;
-    if (advance) {
+    if (is_rod(ctx)) {
+        const RodArgs R = rod_args(ctx);
+        const StepArgs sa0 = sa ? *sa : step_args(ctx, nullptr);
+        const dim3 g(ctx->n_work), b(kBlock);
+        if (ctx->D == 2) {
+            if (advance) hipLaunchKernelGGL((k_gather_rod<2, true>), g, b, 0, ctx->stream, A, src, dst, R, sa0);
+            else hipLaunchKernelGGL((k_gather_rod<2, false>), g, b, 0, ctx->stream, A, src, dst, R, sa0);
+        } else {
+            if (advance) hipLaunchKernelGGL((k_gather_rod<3, true>), g, b, 0, ctx->stream, A, src, dst, R, sa0);
+            else hipLaunchKernelGGL((k_gather_rod<3, false>), g, b, 0, ctx->stream, A, src, dst, R, sa0);
+        }
+    } else if (advance) {
         if (ctx->D == 2)
             gst = mixed ? launch_gather_integrate_dm<2, true>(ctx, src, dst, *sa)
                         : launch_gather_integrate_dm<2, false>(ctx, src, dst, *sa);
@@ -574,14 +637,24 @@ crm_status crm_create(const crm_config* cfg, const crm_instance* instances, uint
     ctx->pot_kind = instances[0].potential.kind;
     ctx->mech_kind = instances[0].mechanics.kind;
     if (ctx->pot_kind > CRM_POTENTIAL_NONE) return fail(CRM_INVALID_ARGUMENT, "unknown potential kind");
-    if (ctx->mech_kind > CRM_MECHANICS_BROWNIAN)
-        return fail(CRM_INVALID_ARGUMENT, "unsupported mechanics kind for point agents");
+    if (ctx->mech_kind > CRM_MECHANICS_ROD) return fail(CRM_INVALID_ARGUMENT, "unknown mechanics kind");
     ctx->nv = 1;
+    if (ctx->mech_kind == CRM_MECHANICS_ROD) {
+        ctx->nv = instances[0].mechanics.n_vertices;
+        if (ctx->nv < 2 || ctx->nv > CRM_MAX_ROD_VERTICES)
+            return fail(CRM_INVALID_ARGUMENT, "rods need 2..16 vertices");
+        if (cfg->precision != CRM_PRECISION_FP64)
+            return fail(CRM_INVALID_ARGUMENT, "rods run in FP64 precision only");
+        if (ctx->pot_kind == CRM_POTENTIAL_CELL_SORTING)
+            return fail(CRM_INVALID_ARGUMENT, "rods take a point potential (Morse, Mie, BoundLennardJones)");
+        ctx->rods_per_block = kBlock / ctx->nv;
+    }
     ctx->K = ctx->nv * ctx->D;
     uint64_t voff = 0;
     for (uint32_t i = 0; i < n_instances; i++) {
         const crm_instance& I = instances[i];
-        if (I.potential.kind != ctx->pot_kind || I.mechanics.kind != ctx->mech_kind)
+        if (I.potential.kind != ctx->pot_kind || I.mechanics.kind != ctx->mech_kind ||
+            (ctx->mech_kind == CRM_MECHANICS_ROD && I.mechanics.n_vertices != ctx->nv))
             return fail(CRM_INVALID_ARGUMENT, "all instances must share potential and mechanics kinds");
         if ((I.potential.kind == CRM_POTENTIAL_MORSE_SPECIES || I.potential.kind == CRM_POTENTIAL_CELL_SORTING) &&
             (I.potential.n_species == 0 || I.potential.n_species > CRM_MAX_SPECIES))
@@ -682,11 +755,12 @@ crm_status crm_set_agents(crm_ctx* ctx, const uint64_t* counts, const double* po
     for (uint32_t i = 0; i < ctx->n_inst; i++) {
         ctx->hparams[i].agent_offset = first;
         ctx->hparams[i].n_agents = counts[i];
-        for (uint64_t b = 0; b < counts[i]; b += kBlock) {
+        const uint64_t per = is_rod(ctx) ? ctx->rods_per_block : (uint64_t)kBlock;
+        for (uint64_t b = 0; b < counts[i]; b += per) {
             BlockWork w;
             w.inst = i;
             w.first = first + b;
-            w.count = (uint32_t)std::min<uint64_t>(kBlock, counts[i] - b);
+            w.count = (uint32_t)std::min<uint64_t>(per, counts[i] - b);
             work.push_back(w);
         }
         first += counts[i];
@@ -708,7 +782,8 @@ crm_status crm_set_agents(crm_ctx* ctx, const uint64_t* counts, const double* po
     CRM_TRY(dalloc(ctx, ctx->d_rank, N));
     CRM_TRY(dalloc(ctx, ctx->d_perm, N));
     CRM_TRY(dalloc(ctx, ctx->d_force, (size_t)K * N));
-    if (ctx->cfg.precision == CRM_PRECISION_MIXED) CRM_TRY(dalloc(ctx, ctx->d_recq, N));
+    if (is_rod(ctx)) CRM_TRY(dalloc(ctx, ctx->d_recrod, (size_t)K * N));
+    else if (ctx->cfg.precision == CRM_PRECISION_MIXED) CRM_TRY(dalloc(ctx, ctx->d_recq, N));
     else CRM_TRY(dalloc(ctx, ctx->d_rec64, N));
     for (auto& b : ctx->buf) {
         CRM_TRY(dalloc(ctx, b.pos, (size_t)K * N));

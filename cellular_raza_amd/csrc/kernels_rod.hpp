@@ -1,0 +1,458 @@
+// kernels_rod.hpp -- gfx950 kernels of the mechanics step for bacterial rods.
+//
+// A rod is a chain of nv vertices (RodMechanics, bacterial_rods.rs:82-265); it is
+// binned by its vertex centroid (bacterial_rods.rs:354-373,536-555).  One lane
+// handles one (rod, vertex); the lanes of a rod are consecutive in a workgroup.
+// Step as launched by the runtime:
+//   k_force_rod              RodInteraction (bacterial_rods.rs:292-336) with the chili
+//                            pair weights (update_mechanics.rs:178-220)
+//   k_key_rod                next vertex positions, per-vertex reflection
+//                            (bacterial_rods.rs:489-533), centroid voxel + rank
+//   k_scan_* / k_scatter / k_fixup   as for point agents (slots are rods)
+//   k_gather_integrate_rod   RodMechanics::calculate_increment (bacterial_rods.rs:157-240)
+//                            + solver, written into the rod's new slot
+// State arrays are the point arrays with K = nv * dim components per slot; the
+// force kernel reads rods from an AoS copy (rec_rod[slot][K]) written by the gathers.
+#pragma once
+
+#include "kernels_point.hpp"
+
+namespace crm {
+
+struct RodArgs {
+    double* rec;   // [N][K] AoS vertex positions (force records)
+    uint32_t nv;   // vertices per rod
+    uint32_t rpb;  // rods per workgroup
+};
+
+// Lane -> (rod slot, vertex).  Returns false for lanes past the block's rods.
+__device__ __forceinline__ bool rod_lane(const BlockWork& w, uint32_t nv, uint64_t& s, uint32_t& v) {
+    const uint32_t r = threadIdx.x / nv;
+    v = threadIdx.x % nv;
+    s = w.first + r;
+    return r < w.count;
+}
+
+// ---------------------------------------------------------------------------
+// Force kernel.  For own rod i, vertex v and every candidate rod j of the 3^D
+// stencil of i's voxel:
+//   own part  (calc_i(i, j), row v):  sum over segments s of j of the point force
+//             between vertex v and its nearest point on segment s;
+//   ext part  (calc_j(j, i), row v):  for every vertex u of j, the force on the
+//             nearest point of i's segments (v-1, v) (weight t) and (v, v+1)
+//             (weight 1 - t), in the reference's (u, s) order.
+// Reference weights: same voxel 1/2 own + 1/2 ext; adjacent voxel 1/2 own
+// (physical: same-voxel weights everywhere).
+// ---------------------------------------------------------------------------
+template <int D, int POT>
+__global__ __launch_bounds__(kBlock) void k_force_rod(Arrays A, StateBuf S, RodArgs R, uint32_t physical) {
+    const BlockWork w = A.work[blockIdx.x];
+    uint64_t s;
+    uint32_t v;
+    if (!rod_lane(w, R.nv, s, v)) return;
+    const InstParams& P = A.ip[w.inst];
+    const PotParams<double>& PP = P.pd;
+    const uint32_t nv = R.nv, K = nv * D;
+    const double* own = R.rec + s * K;
+    double pv[D];
+#pragma unroll
+    for (int k = 0; k < D; k++) pv[k] = own[v * D + k];
+    const uint32_t lv = S.vox[s] - P.voxel_offset;
+    uint32_t idx[3];
+    idx[2] = lv % P.n[2];
+    idx[1] = (lv / P.n[2]) % P.n[1];
+    idx[0] = lv / (P.n[2] * P.n[1]);
+    uint32_t lo[3], hi[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        lo[a] = idx[a] == 0 ? 0 : idx[a] - 1;
+        hi[a] = min(idx[a] + 2, P.n[a]);
+    }
+    const int sp_i = S.species[s];
+    double acc[D];
+#pragma unroll
+    for (int k = 0; k < D; k++) acc[k] = 0.0;
+    bool calc_err = false;
+    for (uint32_t a = lo[0]; a < hi[0]; a++) {
+        for (uint32_t b = lo[1]; b < hi[1]; b++) {
+            const uint32_t row = P.voxel_offset + (a * P.n[1] + b) * P.n[2];
+            const uint32_t jb = A.start[row + lo[2]], je = A.start[row + hi[2]];
+            const uint32_t cb = A.start[row + idx[2]], ce = A.start[row + idx[2] + 1];
+            const bool own_row = (a == idx[0]) && (b == idx[1]);
+            for (uint32_t j = jb; j < je; j++) {
+                if (j == s) continue;
+                const double* ext = R.rec + (uint64_t)j * K;
+                const int sp_j = S.species[j];
+                const bool same = sp_i == sp_j;
+                double s_own = PP.strength, s_ext = PP.strength;
+                if constexpr (POT == CRM_POTENTIAL_MORSE_SPECIES) {
+                    s_own = PP.strength * PP.species_matrix[sp_i * CRM_MAX_SPECIES + sp_j];
+                    s_ext = PP.strength * PP.species_matrix[sp_j * CRM_MAX_SPECIES + sp_i];
+                }
+                // own part: vertex v against the segments of rod j
+                double fo_sum[D];
+#pragma unroll
+                for (int k = 0; k < D; k++) fo_sum[k] = 0.0;
+                for (uint32_t q = 0; q + 1 < nv; q++) {
+                    double x[D], t, z[D], f[D];
+                    nearest_point<double, D>(pv, ext + q * D, ext + (q + 1) * D, x, t);
+#pragma unroll
+                    for (int k = 0; k < D; k++) z[k] = pv[k] - x[k];
+                    calc_err |= point_force<POT, double, D>(z, same, s_own, PP, -1, f);
+#pragma unroll
+                    for (int k = 0; k < D; k++) fo_sum[k] = fo_sum[k] + f[k];
+                }
+#pragma unroll
+                for (int k = 0; k < D; k++) acc[k] = acc[k] + fo_sum[k] * 0.5;
+                const bool full = physical || (own_row && j >= cb && j < ce);
+                if (full) {
+                    // ext part: row v of the ext-force matrix of calc_j(j, i)
+                    double fe_sum[D];
+#pragma unroll
+                    for (int k = 0; k < D; k++) fe_sum[k] = 0.0;
+                    for (uint32_t u = 0; u < nv; u++) {
+                        const double* pu = ext + u * D;
+                        for (int side = 0; side < 2; side++) {
+                            // side 0: segment (v-1, v), v is its second end (weight t)
+                            // side 1: segment (v, v+1), v is its first end (weight 1 - t)
+                            const int q = (int)v - 1 + side;
+                            if (q < 0 || q + 1 >= (int)nv) continue;
+                            double x[D], t, z[D], f[D];
+                            nearest_point<double, D>(pu, own + q * D, own + (q + 1) * D, x, t);
+#pragma unroll
+                            for (int k = 0; k < D; k++) z[k] = pu[k] - x[k];
+                            calc_err |= point_force<POT, double, D>(z, same, s_ext, PP, -1, f);
+                            const double wgt = side == 0 ? t : 1.0 - t;
+#pragma unroll
+                            for (int k = 0; k < D; k++) fe_sum[k] = fe_sum[k] + (-f[k]) * wgt;
+                        }
+                    }
+#pragma unroll
+                    for (int k = 0; k < D; k++) acc[k] = acc[k] + fe_sum[k] * 0.5;
+                }
+            }
+        }
+    }
+    if (calc_err) raise_error(A.ctrl, CRM_CALC_ERROR, A.ctrl->step, S.id[s]);
+#pragma unroll
+    for (int k = 0; k < D; k++) A.force[(uint64_t)(v * D + k) * A.N + s] = acc[k];
+}
+
+// Normal k of a rod's K normals for (agent, step): Philox block k / 2, element k % 2.
+__device__ __forceinline__ double rod_normal(const StepArgs& sa, uint32_t id, uint64_t t, uint32_t k) {
+    if (sa.noise_mode == CRM_NOISE_BUFFER && sa.noise) return 0.0;  // caller supplies per-vertex values
+    double z0, z1;
+    gaussian_pair(sa.seed, id, t, k / 2, z0, z1);
+    return (k & 1) ? z1 : z0;
+}
+
+// RodMechanics::calculate_increment row v (bacterial_rods.rs:157-213): springs,
+// bending and damping, accumulated in the reference's order.  pos(i, k) and
+// vel(i, k) read vertex i of the rod.
+template <int D, typename Pos, typename Vel>
+__device__ __forceinline__ void rod_row_increment(const InstParams& P, uint32_t nv, uint32_t v, Pos pos, Vel vel,
+                                                  const double* force, double* dx, double* dv) {
+    double total[D];
+#pragma unroll
+    for (int k = 0; k < D; k++) total[k] = force[k];
+    // dist_internal[i] = pos[i] - pos[i + 1]
+    auto dist = [&](int i, double* d) {
+#pragma unroll
+        for (int k = 0; k < D; k++) d[k] = pos(i, k) - pos(i + 1, k);
+    };
+    auto spring = [&](int i, double* fi) {  // force_internal of spring i (zero if degenerate)
+        double d[D], n2 = 0.0;
+        dist(i, d);
+#pragma unroll
+        for (int k = 0; k < D; k++) n2 = n2 + d[k] * d[k];
+        const double nd = sqrt(n2);
+        if (nd == 0.0) return false;
+#pragma unroll
+        for (int k = 0; k < D; k++) fi[k] = ((-(d[k] / nd)) * P.spring_tension) * (nd - P.spring_length);
+        return true;
+    };
+    // springs i = v - 1 (v is its second vertex) then i = v (v is its first vertex)
+    if (v >= 1) {
+        double fi[D];
+        if (spring((int)v - 1, fi)) {
+#pragma unroll
+            for (int k = 0; k < D; k++) total[k] = total[k] + (-fi[k]) * 0.5;
+        }
+    }
+    if (v + 1 < nv) {
+        double fi[D];
+        if (spring((int)v, fi)) {
+#pragma unroll
+            for (int k = 0; k < D; k++) total[k] = total[k] + fi[k] * 0.5;
+        }
+    }
+    // bending windows i = v - 2, v - 1, v (v is vertex i + 2, i + 1, i)
+    auto bend = [&](int i, double* F) {
+        double c1[D], c2[D];
+        dist(i, c1);
+        dist(i + 1, c2);
+        double prod = 0.0, n1 = 0.0, n2 = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+            prod = prod + c1[k] * c2[k];
+            n1 = n1 + c1[k] * c1[k];
+            n2 = n2 + c2[k] * c2[k];
+        }
+        n1 = sqrt(n1);
+        n2 = sqrt(n2);
+        double ang = 0.0;
+        if (!(n1 == 0.0 || n2 == 0.0)) {  // nalgebra Matrix::angle
+            double cang = prod / (n1 * n2);
+            cang = cang < -1.0 ? -1.0 : (cang > 1.0 ? 1.0 : cang);
+            ang = acos(cang);
+        }
+        double fd[D], nf = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+            fd[k] = c1[k] - c2[k];
+            nf = nf + fd[k] * fd[k];
+        }
+        nf = sqrt(nf);
+        const double tn = tan(0.5 * ang);
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+            const double fdir = nf != 0.0 ? fd[k] / nf : fd[k];
+            F[k] = ((fdir * 2.0) * P.rigidity) * tn;
+        }
+    };
+    for (int i = (int)v - 2; i <= (int)v; i++) {
+        if (i < 0 || i + 2 >= (int)nv) continue;
+        double F[D];
+        bend(i, F);
+        const int role = (int)v - i;  // 0: v = i, 1: v = i + 1, 2: v = i + 2
+#pragma unroll
+        for (int k = 0; k < D; k++) total[k] = total[k] + (role == 1 ? F[k] : (-F[k]) * 0.5);
+    }
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+        dv[k] = total[k] - vel(v, k) * P.damping;
+        dx[k] = vel(v, k);
+    }
+}
+
+// Solver step of one vertex row (same formulas as point_step).
+template <int D, bool WITH_VEL, typename Hist>
+__device__ __forceinline__ void rod_row_solve(const StepArgs& sa, const Ring& rg, const double* x, const double* vv,
+                                              const double* dx, const double* dv, const double* dxr, Hist hist,
+                                              double* nx, double* nvel) {
+    if (rg.cap == 0) {
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+            nx[k] = (dx[k] * sa.dt + x[k]) * 1.0 + dxr[k] * sa.dt;
+            if (WITH_VEL) nvel[k] = (dv[k] * sa.dt + vv[k]) * 1.0 + 0.0 * sa.dt;
+        }
+    } else if (rg.n_read == 2) {
+        const double f0 = 23.0 / 12.0, f1 = (-16.0) / 12.0, f2 = 5.0 / 12.0;
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+            const double h0 = rg.it[0] == rg.wslot ? dx[k] : hist(0, k, false);
+            const double h1 = rg.it[1] == rg.wslot ? dx[k] : hist(1, k, false);
+            nx[k] = ((dx[k] * f0 + (h0 * f1 + h1 * f2)) * sa.dt + x[k]) * 1.0 + dxr[k] * sa.dt;
+            if (WITH_VEL) {
+                const double g0 = rg.it[0] == rg.wslot ? dv[k] : hist(0, k, true);
+                const double g1 = rg.it[1] == rg.wslot ? dv[k] : hist(1, k, true);
+                nvel[k] = ((dv[k] * f0 + (g0 * f1 + g1 * f2)) * sa.dt + vv[k]) * 1.0 + 0.0 * sa.dt;
+            }
+        }
+    } else {
+        const double g0c = 3.0 / 2.0, g1c = (-1.0) / 2.0;
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+            const double h0 = rg.it[0] == rg.wslot ? dx[k] : hist(0, k, false);
+            nx[k] = ((dx[k] * g0c + h0 * g1c) * sa.dt + x[k]) * 1.0 + dxr[k] * sa.dt;
+            if (WITH_VEL) {
+                const double q0 = rg.it[0] == rg.wslot ? dv[k] : hist(0, k, true);
+                nvel[k] = ((dv[k] * g0c + q0 * g1c) * sa.dt + vv[k]) * 1.0 + 0.0 * sa.dt;
+            }
+        }
+    }
+}
+
+// Position noise of vertex row v: N(0, sqrt dt) * 2^-2 * D / dt (bacterial_rods.rs:215-240).
+template <int D>
+__device__ __forceinline__ void rod_row_noise(const InstParams& P, const StepArgs& sa, uint32_t id, uint64_t t,
+                                              uint32_t nv, uint32_t v, double* dxr) {
+#pragma unroll
+    for (int k = 0; k < D; k++) dxr[k] = 0.0;
+    bool have = false;
+    double z[D];
+    if (sa.noise_mode == CRM_NOISE_BUFFER && sa.noise) {
+#pragma unroll
+        for (int k = 0; k < D; k++) z[k] = sa.noise[((uint64_t)id * nv + v) * D + k];
+        have = true;
+    } else if (sa.noise_mode == CRM_NOISE_PHILOX) {
+#pragma unroll
+        for (int k = 0; k < D; k++) z[k] = rod_normal(sa, id, t, v * D + k);
+        have = true;
+    }
+    if (have) {
+#pragma unroll
+        for (int k = 0; k < D; k++) dxr[k] = (((0.0 + sa.sqrt_dt * z[k]) * 0.25) * P.diffusion_constant) / sa.dt;
+    }
+}
+
+// Per-vertex reflection (bacterial_rods.rs:489-533); returns true if still outside.
+template <int D, bool WITH_VEL>
+__device__ __forceinline__ bool rod_row_reflect(const InstParams& P, double* nx, double* nvel) {
+    bool out = false;
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+        const int ax = k + 3 - D;
+        if (nx[k] < P.min[ax]) {
+            nx[k] = P.min[ax] * 2.0 - nx[k];
+            if (WITH_VEL) nvel[k] = fabs(nvel[k]);
+        }
+        if (nx[k] > P.max[ax]) {
+            nx[k] = P.max[ax] * 2.0 - nx[k];
+            if (WITH_VEL) nvel[k] = -fabs(nvel[k]);
+        }
+        out |= (nx[k] < P.min[ax]) || (nx[k] > P.max[ax]);
+    }
+    return out;
+}
+
+constexpr int kRodLdsK = kBlock * 3;  // one new position per lane
+
+// Next centroid voxel of every rod.  The rod's lanes compute their new vertex
+// positions (dx = velocity: the force does not enter the position update), the
+// first lane sums them in vertex order and divides by nv (row_sum / nrows).
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_key_rod(Arrays A, StateBuf S, RodArgs R, StepArgs sa) {
+    __shared__ double newpos[kRodLdsK];
+    __shared__ uint32_t outflag[kBlock];
+    const BlockWork w = A.work[blockIdx.x];
+    uint64_t s;
+    uint32_t v;
+    const bool active = rod_lane(w, R.nv, s, v);
+    const InstParams& P = A.ip[w.inst];
+    const uint64_t N = A.N, t = A.ctrl->step;
+    const uint32_t nv = R.nv;
+    if (active) {
+        const uint32_t id = S.id[s];
+        const Ring rg = ring_at(t, sa.cap, sa.newest_first);
+        double x[D], vv[D], dx[D], dv[D], dxr[D], nx[D], nvel[D];
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+            x[k] = S.pos[(uint64_t)(v * D + k) * N + s];
+            vv[k] = S.vel[(uint64_t)(v * D + k) * N + s];
+            dx[k] = vv[k];
+            dv[k] = 0.0;
+        }
+        rod_row_noise<D>(P, sa, id, t, nv, v, dxr);
+        auto hist = [&](int i, int k, bool) { return S.hdx[((uint64_t)rg.it[i] * nv * D + v * D + k) * N + s]; };
+        rod_row_solve<D, false>(sa, rg, x, vv, dx, dv, dxr, hist, nx, nvel);
+        outflag[threadIdx.x] = rod_row_reflect<D, false>(P, nx, nvel) ? 1u : 0u;
+#pragma unroll
+        for (int k = 0; k < D; k++) newpos[threadIdx.x * 3 + k] = nx[k];
+    }
+    __syncthreads();
+    if (!active || v != 0) return;
+    bool out = false;
+    double c[D];
+#pragma unroll
+    for (int k = 0; k < D; k++) c[k] = 0.0;
+    for (uint32_t q = 0; q < nv; q++) {
+        out |= outflag[threadIdx.x + q] != 0;
+#pragma unroll
+        for (int k = 0; k < D; k++) c[k] = c[k] + newpos[(threadIdx.x + q) * 3 + k];
+    }
+    if (out) raise_error(A.ctrl, CRM_BOUNDARY_ERROR, t, S.id[s]);
+    uint32_t idx[3] = {0, 0, 0};
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+        const int ax = k + 3 - D;
+        ok = ok && axis_index(c[k] / (double)nv, P.min[ax], P.dx[ax], P.n[ax], idx[ax]);
+    }
+    uint32_t key;
+    if (ok) {
+        key = P.voxel_offset + (idx[0] * P.n[1] + idx[1]) * P.n[2] + idx[2];
+    } else {
+        key = S.vox[s];
+        atomicAdd((unsigned long long*)&A.ctrl->out_of_grid, 1ull);
+    }
+    A.key[s] = key;
+    A.rank[s] = atomicAdd(&A.count[key], 1u);
+}
+
+// Gather + integrate for rods: lane (s, v) computes vertex row v of the rod that
+// lands in sorted slot s (coming from slot o = perm[s]).
+template <int D, bool INTEGRATE>
+__global__ __launch_bounds__(kBlock) void k_gather_rod(Arrays A, StateBuf src, StateBuf dst, RodArgs R,
+                                                         StepArgs sa) {
+    const BlockWork w = A.work[blockIdx.x];
+    uint64_t s;
+    uint32_t v;
+    if (!rod_lane(w, R.nv, s, v)) return;
+    const InstParams& P = A.ip[w.inst];
+    const uint64_t N = A.N;
+    const uint32_t nv = R.nv, K = nv * D;
+    const uint32_t o = A.perm[s];
+    const uint32_t id = src.id[o];
+    double nx[D], nvel[D];
+    if (INTEGRATE) {
+        const uint64_t t = A.ctrl->step - 1;
+        const Ring rg = ring_at(t, sa.cap, sa.newest_first);
+        auto pos = [&](int i, int k) { return src.pos[(uint64_t)(i * D + k) * N + o]; };
+        auto vel = [&](int i, int k) { return src.vel[(uint64_t)(i * D + k) * N + o]; };
+        double x[D], vv[D], f[D], dx[D], dv[D], dxr[D];
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+            x[k] = pos((int)v, k);
+            vv[k] = vel((int)v, k);
+            f[k] = A.force[(uint64_t)(v * D + k) * N + o];
+        }
+        rod_row_increment<D>(P, nv, v, pos, vel, f, dx, dv);
+        rod_row_noise<D>(P, sa, id, t, nv, v, dxr);
+        auto hist = [&](int i, int k, bool is_dv) {
+            const double* h = is_dv ? src.hdv : src.hdx;
+            return h[((uint64_t)rg.it[i] * K + v * D + k) * N + o];
+        };
+        rod_row_solve<D, true>(sa, rg, x, vv, dx, dv, dxr, hist, nx, nvel);
+        rod_row_reflect<D, true>(P, nx, nvel);
+        if (rg.cap > 0) {
+            const uint32_t n_hist = (uint32_t)min<uint64_t>(t + 1, rg.cap);
+            for (uint32_t h = 0; h < n_hist; h++) {
+#pragma unroll
+                for (int k = 0; k < D; k++) {
+                    const uint64_t e = ((uint64_t)h * K + v * D + k) * N;
+                    dst.hdx[e + s] = h == rg.wslot ? dx[k] : src.hdx[e + o];
+                    dst.hdv[e + s] = h == rg.wslot ? dv[k] : src.hdv[e + o];
+                }
+            }
+        }
+    } else {
+        const uint32_t n_hist = (uint32_t)min<uint64_t>(A.ctrl->step, sa.cap);
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+            nx[k] = src.pos[(uint64_t)(v * D + k) * N + o];
+            nvel[k] = src.vel[(uint64_t)(v * D + k) * N + o];
+        }
+        for (uint32_t h = 0; h < n_hist; h++) {
+#pragma unroll
+            for (int k = 0; k < D; k++) {
+                const uint64_t e = ((uint64_t)h * K + v * D + k) * N;
+                dst.hdx[e + s] = src.hdx[e + o];
+                dst.hdv[e + s] = src.hdv[e + o];
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+        dst.pos[(uint64_t)(v * D + k) * N + s] = nx[k];
+        dst.vel[(uint64_t)(v * D + k) * N + s] = nvel[k];
+        R.rec[s * K + v * D + k] = nx[k];
+    }
+    if (v == 0) {
+        dst.species[s] = src.species[o];
+        dst.id[s] = id;
+        dst.vox[s] = A.key[o];
+    }
+}
+
+}  // namespace crm

@@ -21,6 +21,8 @@ from .building_blocks import (
     MiePotential,
     MorsePotential,
     NewtonDamped,
+    RodInteraction,
+    RodMechanics,
     SpeciesMorsePotential,
 )
 from .simulation import InstanceSpec
@@ -80,6 +82,43 @@ def c2_sorting_3d(n: int = 1_000_000, seed: int = 2, steps: int = 1000) -> Workl
     species = (rng.random(n) < 0.5).astype(np.int32)
     return Workload("C2 3-D cell sorting Langevin", [InstanceSpec(dom, pot, mech)], 0.005, steps, 2, pos,
                     np.zeros_like(pos), species, [n], noise="philox")
+
+
+def straight_rods(rng, n: int, dim: int, L: float, n_vertices: int, spacing: float) -> np.ndarray:
+    """n straight rods of n_vertices vertices `spacing` apart: centres uniform in
+    [margin, L - margin)^dim (margin = half length + 0.25), directions uniform
+    (angle in 2-D, unit sphere in 3-D).  Returns (n, n_vertices, dim)."""
+    half = 0.5 * spacing * (n_vertices - 1)
+    margin = half + 0.25
+    centre = rng.uniform(margin, L - margin, size=(n, dim))
+    if dim == 2:
+        ang = rng.uniform(0.0, 2.0 * math.pi, n)
+        u = np.stack([np.cos(ang), np.sin(ang)], axis=1)
+    else:
+        u = rng.normal(size=(n, 3))
+        u /= np.linalg.norm(u, axis=1, keepdims=True)
+    off = (np.arange(n_vertices) - 0.5 * (n_vertices - 1)) * spacing
+    return centre[:, None, :] + off[None, :, None] * u[:, None, :]
+
+
+def c3_rods_2d(n: int = 200_000, seed: int = 3, steps: int = 500, n_vertices: int = 8,
+               diffusion_constant: float = 0.0) -> Workload:
+    """2-D bacterial rods: n rods of 8 vertices 0.5 apart (length 3.5), RodMechanics
+    (spring tension 10, rigidity 2, spring length 0.5, damping 1.5), RodInteraction of
+    Morse (R 0.5, lambda 2, strength 0.1, cutoff 1.5), dt 0.002, AB3.  Box side
+    2000 * sqrt(n / 200000), voxel edge 5.0 (>= rod length + cutoff, centroid binning).
+    Morse stiffness and strength are not specified for this workload; lambda 2 / s 0.1
+    give a soft contact."""
+    L = 2000.0 * math.sqrt(n / 200_000.0)
+    L = max(L, 20.0)
+    dom = CartesianCuboid.from_boundaries_and_interaction_range([0.0, 0.0], [L, L], 5.0)
+    pot = RodInteraction(MorsePotential(radius=0.5, potential_stiffness=2.0, cutoff=1.5, strength=0.1))
+    mech = RodMechanics(n_vertices=n_vertices, diffusion_constant=diffusion_constant, spring_tension=10.0,
+                        rigidity=2.0, spring_length=0.5, damping=1.5)
+    rng = np.random.default_rng(seed)
+    pos = straight_rods(rng, n, 2, L, n_vertices, 0.5)
+    return Workload("C3 2-D bacterial rods", [InstanceSpec(dom, pot, mech)], 0.002, steps, 2, pos,
+                    np.zeros_like(pos), None, [n], noise="philox" if diffusion_constant > 0 else "none")
 
 
 def c4_ensemble_2d(n_instances: int = 512, seed: int = 4, steps: int = 1000, n_min: int = 500,
