@@ -1,14 +1,21 @@
 """Benchmark: agent-steps/s of the per-step mechanics path on one MI355X.
 
-Workload: 3-D two-species cell sorting, Langevin, N = 1,000,000 agents (volume
-fraction 0.3, species-Morse R 1 / cutoff 2.5 / strength 0.5 x [[1, .2], [.2, 1]],
-gamma 2, kT 0.01, dt 0.005, AB3).  One step = forces -> integrator -> boundary
--> counting-sort re-binning of all agents, inputs resident in HBM.
+Default workload (the headline line): 3-D two-species cell sorting, Langevin,
+N = 1,000,000 agents (volume fraction 0.3, species-Morse R 1 / cutoff 2.5 /
+strength 0.5 x [[1, .2], [.2, 1]], gamma 2, kT 0.01, dt 0.005, AB3).  One step =
+forces -> integrator -> boundary -> counting-sort re-binning of all agents,
+inputs resident in HBM.
 
 Prints one JSON line.  `value` is the mixed-precision path (float pair forces
-with compensated sums, double state); the FP64 path, the force error of the
-mixed path against FP64, the force kernel's roofline figures and the CPU
-baseline (the oracle's chili restatement, threaded) are reported beside it.
+with compensated sums, double state) where the workload has one, else FP64; the
+FP64 path, the force error of the mixed path against FP64, the force kernel's
+roofline figures and the CPU baseline (the oracle's chili restatement, threaded)
+are reported beside it.  `--workload` selects the other configurations:
+  c1  2-D Morse NewtonDamped, 1000 agents, Euler
+  c3  2-D bacterial rods, 200,000 rods x 8 vertices, AB3 (FP64)
+  c4  512-instance 2-D Morse ensemble in one context, plus the same instances
+      run one after another (one context each)
+  c5  3-D Mie lattice, 4,000,000 agents, AB3
 """
 import argparse
 import json
@@ -22,24 +29,30 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+PROFILE = os.path.join(ROOT, "profiles", "c2_cell_sorting_1M.md")
 
 
 def candidates_per_agent(wl, start):
-    """Mean number of candidate agents over the clipped 3x3x3 stencil (own voxel included)."""
-    dom = wl.instances[0].domain
-    n = dom.get_n_voxels()
-    cnt = np.diff(start.astype(np.int64)).reshape(n)
-    pad = np.pad(cnt, 1)
-    stencil = np.zeros_like(cnt)
-    for a in (0, 1, 2):
-        for b in (0, 1, 2):
-            for c in (0, 1, 2):
-                stencil += pad[a:a + n[0], b:b + n[1], c:c + n[2]]
-    return float((stencil * cnt).sum() / cnt.sum())
-
-
-PROFILE = os.path.join(ROOT, "profiles", "c2_cell_sorting_1M.md")
-FORCE_KERNEL = "k_force_tiled<3, 3, true>"
+    """Mean number of candidate agents over the clipped 3^D stencil (own voxel included),
+    averaged over all agents of all instances."""
+    tot, num = 0.0, 0.0
+    off = 0
+    for spec in wl.instances:
+        n = spec.domain.get_n_voxels()
+        V = int(np.prod(n))
+        cnt = np.diff(start[off:off + V + 1].astype(np.int64)).reshape(n)
+        off += V
+        pad = np.pad(cnt, 1)
+        stencil = np.zeros_like(cnt)
+        D = len(n)
+        shifts = [(a, b, c) for a in range(3) for b in range(3) for c in range(3)] if D == 3 else \
+            [(a, b) for a in range(3) for b in range(3)]
+        for sh in shifts:
+            sl = tuple(slice(s, s + n[k]) for k, s in enumerate(sh))
+            stencil += pad[sl]
+        tot += float((stencil * cnt).sum())
+        num += float(cnt.sum())
+    return tot / num
 
 
 def measured_traffic(kernel):
@@ -53,17 +66,23 @@ def measured_traffic(kernel):
         return None
 
 
-def force_bytes_per_agent(cbar, record_bytes, dim, stencil=27):
-    # own record + (start, end) per stencil voxel + candidate records + force write (double)
-    return record_bytes + 8 * stencil + cbar * record_bytes + 8 * dim
+def force_bytes_per_agent(cbar, record_bytes, dim, stencil, force_bytes):
+    """Algorithmic bytes of the force kernel per agent: own record + (start, end) per
+    stencil voxel + candidate records + force write."""
+    return record_bytes + 8 * stencil + cbar * record_bytes + force_bytes
+
+
+def make_backend(wl, precision, graph, timing, seed):
+    from cellular_raza_amd import MechanicsBackend
+
+    g = MechanicsBackend(wl.instances, dt=wl.dt, mechanics_solver_order=wl.solver_order, noise=wl.noise,
+                         seed=seed, precision=precision, graph=graph, timing=timing)
+    g.set_agents(wl.pos, wl.vel, wl.species, wl.counts)
+    return g
 
 
 def run_gpu(wl, precision, steps, warmup, seed=1, timing_steps=20):
-    from cellular_raza_amd import MechanicsBackend
-
-    g = MechanicsBackend(wl.instances, dt=wl.dt, mechanics_solver_order=wl.solver_order, noise="philox",
-                         seed=seed, precision=precision, graph=True)
-    g.set_agents(wl.pos, wl.vel, wl.species)
+    g = make_backend(wl, precision, True, False, seed)
     f0 = g.compute_forces()
     g.step(warmup)
     t0 = time.perf_counter()
@@ -71,30 +90,91 @@ def run_gpu(wl, precision, steps, warmup, seed=1, timing_steps=20):
     t1 = time.perf_counter()
     sec = t1 - t0
     _, start = g.get_voxels()
+    pos_end = g.get_state()[0]
     g.close()
     # per-kernel HIP-event timing over a second timed region on the same stream
-    gt = MechanicsBackend(wl.instances, dt=wl.dt, mechanics_solver_order=wl.solver_order, noise="philox",
-                          seed=seed, precision=precision, graph=False, timing=True)
-    gt.set_agents(wl.pos, wl.vel, wl.species)
+    gt = make_backend(wl, precision, False, True, seed)
     gt.step(warmup)
     gt.reset_timing()
     gt.step(timing_steps)
     tim = gt.timing()
     gt.close()
-    return sec, f0, start, tim
+    return sec, f0, start, tim, pos_end
+
+
+def run_sequential(wl, precision, steps, warmup, seed=1):
+    """Every instance of an ensemble in a context of its own, run one after another."""
+    from cellular_raza_amd import MechanicsBackend
+
+    total = 0.0
+    off = 0
+    for spec, n in zip(wl.instances, wl.counts):
+        g = MechanicsBackend([spec], dt=wl.dt, mechanics_solver_order=wl.solver_order, noise=wl.noise, seed=seed,
+                             precision=precision, graph=True)
+        g.set_agents(wl.pos[off:off + n], wl.vel[off:off + n])
+        off += n
+        g.step(warmup)
+        t0 = time.perf_counter()
+        g.step(steps)
+        total += time.perf_counter() - t0
+        g.close()
+    return total
 
 
 def cpu_baseline(wl, threads, steps):
     from oracle.oracle import OracleSim
 
-    o = OracleSim(wl.instances, dt=wl.dt, mechanics_solver_order=wl.solver_order, noise="philox", seed=1,
+    o = OracleSim(wl.instances, dt=wl.dt, mechanics_solver_order=wl.solver_order, noise=wl.noise, seed=1,
                   n_subdomains=threads)
-    o.set_agents(wl.pos, wl.vel, wl.species)
+    o.set_agents(wl.pos, wl.vel, wl.species, wl.counts)
     o.step_timed(1)  # warm-up step
     sec = o.step_timed(steps)
-    f = o.compute_forces() if threads == 1 else None
     o.close()
-    return wl.n_agents * steps / sec, sec, f
+    return wl.n_agents * steps / sec, sec
+
+
+def oracle_forces(wl):
+    from oracle.oracle import OracleSim
+
+    # one subdomain: pairs across subdomain borders are weighted differently
+    # (full instead of half), so only the single-subdomain oracle matches
+    o = OracleSim(wl.instances, dt=wl.dt, mechanics_solver_order=wl.solver_order, noise=wl.noise, seed=1)
+    o.set_agents(wl.pos, wl.vel, wl.species, wl.counts)
+    f = o.compute_forces()
+    o.close()
+    return f
+
+
+def rel_err(a, b):
+    a = np.asarray(a).reshape(a.shape[0], -1)
+    b = np.asarray(b).reshape(b.shape[0], -1)
+    return float(np.linalg.norm(a - b, axis=1).max() / np.linalg.norm(b, axis=1).max())
+
+
+def rms_rel_err(a, b):
+    a = np.asarray(a).reshape(a.shape[0], -1)
+    b = np.asarray(b).reshape(b.shape[0], -1)
+    return float(np.sqrt(np.mean(np.sum((a - b) ** 2, axis=1))) / np.sqrt(np.mean(np.sum(b ** 2, axis=1))))
+
+
+# name -> (factory(args) -> Workload, mixed available, description, force kernel name for the profile lookup)
+def workload_table():
+    from cellular_raza_amd import workloads as W
+
+    return {
+        "c2": (lambda a: W.c2_sorting_3d(n=a.n or 1_000_000, seed=2, steps=a.steps), True,
+               "3-D two-species cell sorting, Langevin, N=1,000,000, volume fraction 0.3",
+               "k_force_tiled<3, 3, true>"),
+        "c1": (lambda a: W.c1_morse_2d(n=a.n or 1000, seed=1, steps=a.steps), True,
+               "2-D Morse NewtonDamped, N=1000 in 250x250, Euler", "k_force_tiled<2, 0, true>"),
+        "c3": (lambda a: W.c3_rods_2d(n=a.n or 200_000, seed=3, steps=a.steps), False,
+               "2-D bacterial rods, 200,000 rods x 8 vertices in 2000x2000, AB3", "k_force_rod<2, 0>"),
+        "c4": (lambda a: W.c4_ensemble_2d(n_instances=a.n or 512, seed=4, steps=a.steps), True,
+               "512-instance 2-D Morse ensemble (500..20000 agents each), Euler, one context",
+               "k_force_tiled<2, 0, true>"),
+        "c5": (lambda a: W.c5_mie_3d(n=a.n or 4_000_000, seed=5, steps=a.steps), True,
+               "3-D Mie lattice, N=4,000,000, AB3", "k_force_tiled<3, 1, true>"),
+    }
 
 
 def main():
@@ -102,9 +182,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--n", type=int, default=0, help="agents (c4: instances); 0 = the workload's own size")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--seq-steps", type=int, default=100, help="c4: steps of the one-after-another run")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy (reference for the FETCH_SIZE / WRITE_SIZE counters)")
@@ -126,35 +208,54 @@ def main():
     if int(os.environ.get("RANK", "0")) != 0:
         return
 
-    from cellular_raza_amd import workloads
-
-    wl = workloads.c2_sorting_3d(n=args.n, seed=2, steps=args.steps)
+    factory, has_mixed, desc, force_kernel = workload_table()[args.workload]
+    wl = factory(args)
     N = wl.n_agents
+    D = wl.dim
+    nv = getattr(wl.instances[0].mechanics, "n_vertices", 1)
     print(f"bench: workload {wl.name}, N={N}, steps={args.steps}, warmup={args.warmup}", file=sys.stderr,
           flush=True)
 
-    sec_m, f_mixed, start, tim_m = run_gpu(wl, "mixed", args.steps, args.warmup)
-    print(f"bench: mixed {N * args.steps / sec_m:.4e} agent-steps/s", file=sys.stderr, flush=True)
-    fp64_steps = max(args.steps // 4, 10)
-    sec_d, f_fp64, _, tim_d = run_gpu(wl, "fp64", fp64_steps, args.warmup)
-    print(f"bench: fp64 {N * fp64_steps / sec_d:.4e} agent-steps/s", file=sys.stderr, flush=True)
-
-    def err(a, b):
-        return float(np.linalg.norm(a - b, axis=1).max() / np.linalg.norm(b, axis=1).max())
-
-    force_err_mixed_vs_fp64 = err(f_mixed, f_fp64)
-
+    main_prec = "mixed" if has_mixed else "fp64"
+    sec_m, f_main, start, tim_m, pos_main = run_gpu(wl, main_prec, args.steps, args.warmup)
     value = N * args.steps / sec_m
-    value_fp64 = N * fp64_steps / sec_d
+    print(f"bench: {main_prec} {value:.4e} agent-steps/s", file=sys.stderr, flush=True)
+    extra = {}
+    if has_mixed:
+        fp64_steps = max(args.steps // 4, 10)
+        sec_d, f_fp64, _, tim_d, _ = run_gpu(wl, "fp64", fp64_steps, args.warmup)
+        value_fp64 = N * fp64_steps / sec_d
+        print(f"bench: fp64 {value_fp64:.4e} agent-steps/s", file=sys.stderr, flush=True)
+        extra["fp64_value"] = value_fp64
+        extra["mixed_over_fp64"] = value / value_fp64
+        extra["force_rel_err_mixed_vs_fp64"] = rel_err(f_main, f_fp64)
+        extra["force_rms_rel_err_mixed_vs_fp64"] = rms_rel_err(f_main, f_fp64)
+        extra["kernel_ms_fp64"] = {k: v["total_ms"] / max(v["launches"], 1) for k, v in tim_d.items()}
+    if args.workload == "c4":
+        sec_s = run_sequential(wl, main_prec, args.seq_steps, 5)
+        seq_value = N * args.seq_steps / sec_s
+        extra["sequential_value"] = seq_value
+        extra["batched_over_sequential"] = value / seq_value
+        extra["sequential_sample"] = f"{len(wl.instances)} contexts one after another, {args.seq_steps} steps each"
+        print(f"bench: sequential {seq_value:.4e} agent-steps/s", file=sys.stderr, flush=True)
+    if args.workload in ("c1", "c3"):
+        # force error of the benchmarked path against the oracle on the benchmarked input
+        extra["force_rel_err_vs_oracle"] = rel_err(f_main, oracle_forces(wl))
+
     cbar = candidates_per_agent(wl, start)
-    bf = force_bytes_per_agent(cbar, 16, 3)
+    if nv > 1:
+        rec_bytes = 8 * nv * D + 4  # vertex positions + species
+        bf = force_bytes_per_agent(cbar, rec_bytes, D, 3 ** D, 8 * nv * D)
+    else:
+        rec_bytes = (4 * D + 4) if has_mixed else 8 * D + 8
+        bf = force_bytes_per_agent(cbar, rec_bytes, D, 3 ** D, 8 * D)
     t_force_ms = tim_m["force"]["total_ms"] / max(tim_m["force"]["launches"], 1)
     achieved = bf * N / (t_force_ms * 1e-3) / 1e9
 
     cpu = None
     if not args.no_cpu:
         threads = min(args.cpu_threads, os.cpu_count() or 1)
-        cpu_val, cpu_sec, f_orc = cpu_baseline(wl, threads, args.cpu_steps)
+        cpu_val, cpu_sec = cpu_baseline(wl, threads, args.cpu_steps)
         cpu = {
             "value": cpu_val,
             "unit": "agent-steps/s",
@@ -176,34 +277,38 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 pair forces / f64 state",
-        "data": "synthetic (seeded uniform positions, Bernoulli(0.5) species)",
+        "dtype": "f32 pair forces / f64 state" if has_mixed else "f64",
+        "data": "synthetic (seeded positions" + (", Bernoulli(0.5) species)" if wl.species is not None else ")"),
         "config": {
-            "workload": "3-D two-species cell sorting, Langevin, N=1,000,000, volume fraction 0.3",
+            "workload": desc,
             "n_agents": N,
-            "precision": "mixed",
+            "n_instances": len(wl.instances),
+            "vertices_per_agent": nv,
+            "precision": main_prec,
             "mechanics_solver_order": wl.solver_order,
             "dt": wl.dt,
         },
-        "fp64_value": value_fp64,
-        "mixed_over_fp64": value / value_fp64,
-        "force_rel_err_mixed_vs_fp64": force_err_mixed_vs_fp64,
+        **{k: v for k, v in extra.items() if k in ("fp64_value", "mixed_over_fp64", "force_rel_err_mixed_vs_fp64",
+                                                  "force_rms_rel_err_mixed_vs_fp64")},
         "kernel_ms": {k: v["total_ms"] / max(v["launches"], 1) for k, v in tim_m.items()},
-        "kernel_ms_fp64": {k: v["total_ms"] / max(v["launches"], 1) for k, v in tim_d.items()},
         "roofline": {
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(FORCE_KERNEL),
-            "traffic_source": "profiles/c2_cell_sorting_1M.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
-            "kernel": FORCE_KERNEL,
+            "traffic": measured_traffic(force_kernel) if args.workload == "c2" else None,
+            "traffic_source": "profiles/c2_cell_sorting_1M.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)"
+            if args.workload == "c2" else None,
+            "kernel": force_kernel,
             "bytes_per_agent": bf,
             "candidates_per_agent": cbar,
         },
         "cpu_baseline": cpu,
     }
+    for k, v in extra.items():
+        if k not in line:
+            line[k] = v
     if cpu is not None:
         line["gpu_over_cpu"] = value / cpu["value"]
     print(json.dumps(line), flush=True)
