@@ -514,30 +514,40 @@ crm_status reset_control(crm_ctx* ctx, uint64_t step) {
 // neighbourhood of a few rows fits the LDS budget, otherwise single rows cut
 // along their length.  Sized from the mean density with a 20 % margin; a tile
 // that overflows at run time falls back to global loads.
+// Force tiles of one instance: plane a, R rows x W columns of voxels each, the
+// grid split evenly.  The shape keeps the expected own agents within one pass of
+// the workgroup (lanes busy), the expected staged region (x1.2 for density
+// fluctuations) within the LDS records, the voxel-start table within its LDS
+// array and the region extent within 31 voxels (>= 26 fixed-point bits per
+// voxel); among those it takes the most own agents, then the smallest region.
 void build_tiles(const InstParams& P, uint32_t inst, uint64_t n_agents, uint32_t cap, std::vector<ForceTile>& out) {
     const uint32_t n0 = P.n[0], n1 = P.n[1], n2 = P.n[2];
-    const double rows = (double)n0 * n1;
-    const double apr = (double)n_agents / rows;
-    const double rho = (double)n_agents / ((double)rows * n2);
+    const double rho = (double)n_agents / ((double)n0 * n1 * n2);
     const double planes = n0 > 1 ? 3.0 : 1.0;
-    const double margin = 1.2;
-    uint32_t R = 0;
-    for (uint32_t r = 1; r <= (uint32_t)kTileRowsMax && r <= n1; r++) {
-        const bool fits = planes * (r + 2) * apr * margin <= cap &&
-                          3u * (r + 2) * (n2 + 1) <= (uint32_t)kLdsVoxelStarts && r * apr <= 320.0;
-        if (fits) R = r;
+    const double margin = 1.2, own_max = 0.94 * kBlock;
+    uint32_t bR = 1, bW = 1;
+    double best_own = -1.0, best_staged = 0.0;
+    for (uint32_t r0 = 1; r0 <= std::min<uint32_t>(n1, kTileRowsMax); r0++) {
+        const uint32_t nb = (n1 + r0 - 1) / r0, R = (n1 + nb - 1) / nb;
+        for (uint32_t w0 = 1; w0 <= std::min<uint32_t>(n2, 29u); w0++) {
+            const uint32_t nc = (n2 + w0 - 1) / w0, W = (n2 + nc - 1) / nc;
+            const uint32_t cols = std::min(W + 2, n2);
+            const double own = (double)R * W * rho;
+            const double staged = planes * (R + 2) * (W + 2) * rho * margin;
+            if (own > own_max || staged > cap) continue;
+            if (3u * (R + 2) * (cols + 1) > (uint32_t)kLdsVoxelStarts || std::max(R + 2, cols) > 31u) continue;
+            if (own > best_own * (1.0 + 1e-9) || (own >= best_own * (1.0 - 1e-9) && staged < best_staged)) {
+                best_own = own;
+                best_staged = staged;
+                bR = R;
+                bW = W;
+            }
+        }
     }
-    if (R >= 1) {
-        for (uint32_t a = 0; a < n0; a++)
-            for (uint32_t b = 0; b < n1; b += R) out.push_back(ForceTile{inst, a, b, std::min(b + R, n1), 0u, n2});
-        return;
-    }
-    double lz = std::floor(cap / (planes * 3.0 * std::max(rho, 1e-12) * margin)) - 2.0;
-    lz = std::min(lz, (double)kLdsVoxelStarts / 9.0 - 3.0);
-    const uint32_t L = (uint32_t)std::max(1.0, std::min(lz, (double)n2));
     for (uint32_t a = 0; a < n0; a++)
-        for (uint32_t b = 0; b < n1; b++)
-            for (uint32_t c = 0; c < n2; c += L) out.push_back(ForceTile{inst, a, b, b + 1, c, std::min(c + L, n2)});
+        for (uint32_t b = 0; b < n1; b += bR)
+            for (uint32_t c = 0; c < n2; c += bW)
+                out.push_back(ForceTile{inst, a, b, std::min(b + bR, n1), c, std::min(c + bW, n2)});
 }
 
 template <typename T>

@@ -80,14 +80,22 @@ struct Kahan {
 // The physical mode gives adjacent pairs the same-voxel weight.  With a
 // symmetric interaction both evaluations are equal and one is computed.
 //
-// Mixed precision: positions are 30-bit fixed point relative to the voxel
-// origin (q = floor((p - origin) * 2^30 / dx)), so the difference of two agents
-// in neighbouring voxels is an exact int32 and converts to float with one
-// rounding: the direction is accurate to ~1e-7 however close the pair is.  A
-// pair whose float distance lies in a narrow band around the cutoff radius (the
-// forces jump there), a zero distance, or an agent flagged as outside its
-// voxel's fixed-point range, is re-evaluated from the FP64 positions so that
-// the cutoff decision is the reference's.
+// Work unit: a tile of voxels -- plane a0, rows [b0, b1), columns [c0, c1) of
+// one instance.  Its stencil region (planes a0-1..a0+1, rows b0-1..b1, columns
+// c0-1..c1, clipped to the grid) is staged in LDS, one contiguous segment of
+// sorted agents per (plane, row).
+//
+// Mixed precision: the force records hold 30-bit fixed-point positions relative
+// to the agent's voxel origin (q = floor((p - origin) * 2^30 / dx)).  For the
+// pair loop they are re-expressed relative to the tile region's origin with
+// `bits` bits per voxel, bits = 31 - ceil(log2(region extent in voxels)), so that
+// every coordinate and every difference is an exact int32: the displacement of
+// two agents is one integer subtraction and one rounding to float, accurate to
+// ~1e-7 of the distance however close the pair is.  A pair whose float distance
+// lies in a narrow band around the cutoff radius (the forces jump there), a zero
+// distance, or an agent flagged as outside its voxel's fixed-point range, is
+// re-evaluated from the FP64 positions so that the cutoff decision is the
+// reference's.
 // ---------------------------------------------------------------------------
 
 // blockIdx -> work block, so that each XCD (blocks are dealt round-robin over 8)
@@ -100,6 +108,12 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
 
 constexpr int kQBits = 30;
 constexpr int kQOne = 1 << kQBits;
+// Record word w (mixed): species in bits 0-3, the voxel's column along the last
+// axis in bits 4-30, the outside-its-voxel flag in bit 31.
+constexpr uint32_t kSpeciesMask = 0xfu;
+constexpr int kColShift = 4;
+constexpr uint32_t kColMask = 0x7ffffffu;
+constexpr uint32_t kFlagBit = 0x80000000u;
 
 template <bool MIXED>
 struct RecOf {
@@ -113,48 +127,73 @@ struct RecOf<true> {
 __device__ __forceinline__ const double4* rec_array(const Arrays& A, std::false_type) { return A.rec64; }
 __device__ __forceinline__ const int4* rec_array(const Arrays& A, std::true_type) { return A.recq; }
 
-// Candidate sources.  Within a stencil row (a, b) the voxels are consecutive in
-// the sorted order, so the candidates of voxels [c_lo, c_hi) of that row are one
-// contiguous index range; range() also returns the bounds of the agent's own
-// column c = i2 inside the row (b1, b2) so that each candidate's voxel offset
-// along the row is (j >= b1) + (j >= b2) - 1.
-struct RowRange {
-    uint32_t jb, je, b1, b2;
+// A tile's stencil region.  Staged row r = pa * RW + rb holds plane a0 - 1 + pa,
+// row b0 - 1 + rb (empty where that lies outside the grid), columns [cl, ch).
+struct Region {
+    uint32_t a0, b0;  // tile plane and first row
+    uint32_t cl, ch;  // staged columns
+    uint32_t RW, W1;  // staged rows per plane (tile rows + 2), voxel-start entries per row (ch - cl + 1)
+    int bits, shift;  // tile fixed point: bits per voxel, 30 - bits
 };
 
-template <bool MIXED>
+// Voxel-relative record -> region-relative fixed point.  oa, ob: the record's
+// plane and row offsets from the region origin (a0 - 1, b0 - 1).
+template <int D>
+__device__ __forceinline__ int4 to_region(const int4 r, int oa, int ob, const Region& G) {
+    const int oc = (int)(((uint32_t)r.w >> kColShift) & kColMask) - (int)G.cl;
+    const uint32_t w = (uint32_t)r.w & (kSpeciesMask | kFlagBit);
+    if constexpr (D == 3)
+        return make_int4((oa << G.bits) + (r.x >> G.shift), (ob << G.bits) + (r.y >> G.shift),
+                         (oc << G.bits) + (r.z >> G.shift), (int)w);
+    else
+        return make_int4((ob << G.bits) + (r.x >> G.shift), (oc << G.bits) + (r.y >> G.shift), 0, (int)w);
+}
+
+struct Span {
+    uint32_t b, e;
+};
+
+// Candidate sources.  Within a stencil row (a, b) the voxels are consecutive in
+// the sorted order, so the candidates of voxels [c_lo, c_hi) of that row are one
+// contiguous index range.
+template <int D, bool MIXED>
 struct GlobalSource {
     using Rec = typename RecOf<MIXED>::type;
     const Rec* rec;
     const uint32_t* start;
     uint32_t voff, n1, n2;
-    __device__ __forceinline__ RowRange range(uint32_t a, uint32_t b, uint32_t c_lo, uint32_t c_hi,
-                                              uint32_t i2) const {
-        const uint32_t row = voff + (a * n1 + b) * n2;
-        return RowRange{start[row + c_lo], start[row + c_hi], start[row + i2], start[row + i2 + 1]};
+    Region G;
+    // row handle of stencil row (a, b): index of its column 0 in the table;
+    // b + 1 adds sb, a + 1 adds sa
+    __device__ __forceinline__ const uint32_t* table() const { return start; }
+    __device__ __forceinline__ uint32_t row(uint32_t a, uint32_t b) const { return voff + (a * n1 + b) * n2; }
+    __device__ __forceinline__ uint32_t sb() const { return n2; }
+    __device__ __forceinline__ uint32_t sa() const { return n1 * n2; }
+    __device__ __forceinline__ Rec record(uint32_t j, uint32_t a, uint32_t b) const {
+        if constexpr (MIXED) return to_region<D>(rec[j], (int)(a + 1 - G.a0), (int)(b + 1 - G.b0), G);
+        else return rec[j];
     }
-    __device__ __forceinline__ Rec record(uint32_t j) const { return rec[j]; }
     __device__ __forceinline__ uint32_t slot(uint32_t j) const { return j; }
 };
 
-template <bool MIXED>
+template <int D, bool MIXED>
 struct LdsSource {
     using Rec = typename RecOf<MIXED>::type;
-    const Rec* lds;       // staged records
-    const uint32_t* vst;  // per candidate row: LDS index of the first agent of voxels cl .. ch
-    const uint32_t* gb;   // per candidate row: global slot of its first staged agent
-    const uint32_t* off;  // per candidate row: its first LDS index
-    uint32_t a0, b0, RW, W1, cl;
-    __device__ __forceinline__ uint32_t row_of(uint32_t a, uint32_t b) const { return (a + 1 - a0) * RW + (b + 1 - b0); }
-    __device__ __forceinline__ RowRange range(uint32_t a, uint32_t b, uint32_t c_lo, uint32_t c_hi,
-                                              uint32_t i2) const {
-        const uint32_t* r = vst + row_of(a, b) * W1 - cl;
-        return RowRange{r[c_lo], r[c_hi], r[i2], r[i2 + 1]};
+    const Rec* lds;       // staged records, converted
+    const uint32_t* vst;  // per staged row: LDS index of the first agent of columns cl .. ch
+    const uint32_t* gb;   // per staged row: global slot of its first agent
+    const uint32_t* off;  // per staged row: its first LDS index
+    Region G;
+    __device__ __forceinline__ const uint32_t* table() const { return vst; }
+    __device__ __forceinline__ uint32_t row(uint32_t a, uint32_t b) const {
+        return ((a + 1 - G.a0) * G.RW + (b + 1 - G.b0)) * G.W1 - G.cl;
     }
-    __device__ __forceinline__ Rec record(uint32_t j) const { return lds[j]; }
+    __device__ __forceinline__ uint32_t sb() const { return G.W1; }
+    __device__ __forceinline__ uint32_t sa() const { return G.RW * G.W1; }
+    __device__ __forceinline__ Rec record(uint32_t j, uint32_t, uint32_t) const { return lds[j]; }
     // global slot of LDS index j (only on the rare FP64 re-evaluation path)
     __device__ __forceinline__ uint32_t slot(uint32_t j) const {
-        uint32_t lo = 0, hi = 3 * RW;
+        uint32_t lo = 0, hi = 3 * G.RW;
         while (hi - lo > 1) {
             const uint32_t mid = (lo + hi) >> 1;
             if (off[mid] <= j) lo = mid;
@@ -164,12 +203,64 @@ struct LdsSource {
     }
 };
 
+// Strength row of the species matrix in four scalars (a register array indexed
+// by a run-time species would be placed in scratch memory).
+template <typename T>
+struct Row4 {
+    T v0, v1, v2, v3;
+    // bit tests rather than a compare chain on k, which the compiler turns back into
+    // an indexed (scratch) table
+    __device__ __forceinline__ T pick(int k) const {
+        const bool b0 = (k & 1) != 0, b1 = (k & 2) != 0;
+        const T lo = b0 ? v1 : v0, hi = b0 ? v3 : v2;
+        return b1 ? hi : lo;
+    }
+};
+
+// Position in an agent's flattened candidate stream: stencil row (ra, rb) with
+// row handle `row`, index j inside its range [., je).
+struct Walk {
+    uint32_t ra, rb, row, j, je;
+};
+
+// Per-agent constants of the walk: stencil bounds, columns [c_lo, c_hi) of each
+// row and the handle steps (next row in the plane, first row of the next plane).
+struct WalkPlan {
+    uint32_t lo0, hi0, lo1, hi1, c_lo, c_hi, step_b, step_a;
+    // Moves w past exhausted rows, carrying the overshoot j - je into the next
+    // row.  Returns false when the stencil ends.
+    __device__ __forceinline__ bool settle(const uint32_t* table, Walk& w) const {
+        while (w.j >= w.je) {
+            const uint32_t over = w.j - w.je;
+            if (++w.rb == hi1) {
+                w.rb = lo1;
+                if (++w.ra == hi0) return false;
+                w.row += step_a;
+            } else {
+                w.row += step_b;
+            }
+            w.j = table[w.row + c_lo] + over;
+            w.je = table[w.row + c_hi];
+        }
+        return true;
+    }
+};
+
 // Force on the agent in slot s (index `self` in the source's index space).
 // Candidate order: stencil rows (a, b) ascending, then index ascending, which is
 // voxel-ascending then slot-ascending -- the same for every source.
+//
+// Two passes per chunk of 64 stream positions: a cheap squared-distance test
+// marks the candidates that can be inside the cutoff in a 64-bit mask, then the
+// chunk is replayed and only the marked pairs are evaluated, in stream order.
+// Every unmarked pair is beyond the cutoff, where each potential's force is
+// exactly zero, so the sums are those of the full loop without its zero terms.
+// The wave pays max-over-lanes of the marked count for the expensive pass
+// instead of max-over-lanes of the candidate count.
 template <int D, int POT, bool MIXED, typename Src>
 __device__ __forceinline__ void agent_force(const Arrays& A, const StateBuf& S, const InstParams& P, uint64_t s,
-                                            uint32_t self, uint32_t physical, uint32_t symmetric, const Src& src) {
+                                            uint32_t self, uint32_t physical, uint32_t symmetric, const Src& src,
+                                            const float* unit) {
     using T = typename std::conditional<MIXED, float, double>::type;
     using Rec = typename RecOf<MIXED>::type;
     const uint64_t N = A.N;
@@ -189,25 +280,52 @@ __device__ __forceinline__ void agent_force(const Arrays& A, const StateBuf& S, 
     const float rc2 = (float)(rc * rc);
     const float band = 2e-6f * rc2;
 
-    const Rec me = src.record(self);
+    const Rec me = src.record(self, idx[0], idx[1]);
     int sp_i;
-    uint32_t flag_i = 0;
+    bool flag_i = false;
     if constexpr (MIXED) {
-        sp_i = me.w & 0x7fffffff;
-        flag_i = (uint32_t)me.w >> 31;
+        sp_i = (int)((uint32_t)me.w & kSpeciesMask);
+        flag_i = me.w < 0;
     } else {
         sp_i = (int)me.w;
     }
-    // species-matrix row (own species) and column (own species as ext) in registers
-    T srow[CRM_MAX_SPECIES], scol[CRM_MAX_SPECIES];
+    // candidate test thresholds: mixed marks d2 <= rc2 + band (the band pairs are
+    // re-decided in FP64) and every pair with a flagged agent; FP64 marks
+    // d2 <= rc^2 (1 + 1e-12), beyond which the rounded sqrt(d2) exceeds rc
+    float thr_f = rc2 + band;
+    if (!(thr_f >= 0.0f) || flag_i) thr_f = __builtin_inff();
+    const double rcd = (double)cut_radius<POT, double>(P.pd);
+    double thr_d = rcd * rcd * (1.0 + 1e-12);
+    if (!(thr_d >= 0.0)) thr_d = __builtin_inf();
+    const uint32_t* table = src.table();
+    // the own voxel's candidates (same-voxel pairs)
+    const uint32_t own_row = src.row(idx[0], idx[1]);
+    const Span ov{table[own_row + idx[2]], table[own_row + idx[2] + 1]};
+    WalkPlan wp;
+    wp.lo0 = lo[0];
+    wp.hi0 = hi[0];
+    wp.lo1 = lo[1];
+    wp.hi1 = hi[1];
+    wp.c_lo = lo[2];
+    wp.c_hi = hi[2];
+    wp.step_b = src.sb();
+    wp.step_a = src.sa() - (hi[1] - lo[1] - 1) * src.sb();
+
+    // species-matrix row (own species) and column (own species as ext)
+    Row4<T> srow, scol;
+    {
+        T r4[CRM_MAX_SPECIES], c4[CRM_MAX_SPECIES];
 #pragma unroll
-    for (int k = 0; k < CRM_MAX_SPECIES; k++) {
-        if constexpr (POT == CRM_POTENTIAL_MORSE_SPECIES) {
-            srow[k] = PP.strength * PP.species_matrix[sp_i * CRM_MAX_SPECIES + k];
-            scol[k] = PP.strength * PP.species_matrix[k * CRM_MAX_SPECIES + sp_i];
-        } else {
-            srow[k] = scol[k] = PP.strength;
+        for (int k = 0; k < CRM_MAX_SPECIES; k++) {
+            if constexpr (POT == CRM_POTENTIAL_MORSE_SPECIES) {
+                r4[k] = PP.strength * PP.species_matrix[sp_i * CRM_MAX_SPECIES + k];
+                c4[k] = PP.strength * PP.species_matrix[k * CRM_MAX_SPECIES + sp_i];
+            } else {
+                r4[k] = c4[k] = PP.strength;
+            }
         }
+        srow = Row4<T>{r4[0], r4[1], r4[2], r4[3]};
+        scol = Row4<T>{c4[0], c4[1], c4[2], c4[3]};
     }
     double accd[D];
     Kahan acck[D];
@@ -218,102 +336,113 @@ __device__ __forceinline__ void agent_force(const Arrays& A, const StateBuf& S, 
         acck[k].c = 0.0f;
     }
     bool calc_err = false;
-    // Flattened candidate stream: one iteration per candidate across all stencil
-    // rows, so a wave runs max over lanes of the total count rather than the sum
-    // over rows of the per-row maximum.
-    uint32_t ra = lo[0], rb = lo[1], j = 0;
-    bool done = (POT == CRM_POTENTIAL_NONE);
-    RowRange rr{0u, 0u, 0u, 0u};
-    bool own_row = false;
-    int sh[D];
-    auto load_row = [&](uint32_t a, uint32_t b) {
-        rr = src.range(a, b, lo[2], hi[2], idx[2]);
-        own_row = (a == idx[0]) && (b == idx[1]);
+
+    // Pair displacement own - candidate; returns the squared distance.
+    auto displacement = [&](const Rec& rj, T* z) -> T {
+        T d2 = T(0);
         if constexpr (MIXED) {
-            // fixed-point position of agent i relative to this row's voxel origins
-            const int off01[2] = {(int)a - (int)idx[0], (int)b - (int)idx[1]};
-            const int qi[3] = {me.x, me.y, me.z};
+            z[0] = (float)(me.x - rj.x) * unit[0];
+            z[1] = (float)(me.y - rj.y) * unit[1];
+            if constexpr (D == 3) z[2] = (float)(me.z - rj.z) * unit[2];
+        } else {
+            z[0] = me.x - rj.x;
+            z[1] = me.y - rj.y;
+            if constexpr (D == 3) z[2] = me.z - rj.z;
+        }
 #pragma unroll
-            for (int k = 0; k < D - 1; k++) sh[k] = qi[k] - off01[k + 3 - D] * kQOne;
-            sh[D - 1] = qi[D - 1];
-        }
-        j = rr.jb;
+        for (int k = 0; k < D; k++) d2 = d2 + z[k] * z[k];
+        return d2;
     };
-    if (!done) load_row(ra, rb);
-    while (true) {
-        while (j >= rr.je) {
-            if (++rb == hi[1]) {
-                rb = lo[1];
-                if (++ra == hi[0]) {
-                    done = true;
-                    break;
-                }
-            }
-            load_row(ra, rb);
-        }
-        if (done) break;
-        if (j != self) {
-            const Rec rj = src.record(j);
-            const bool in_col = (j >= rr.b1) && (j < rr.b2);
-            const bool full = physical || (own_row && in_col);
-            T z[D];
-            int sp_j;
-            int cut = -1;
-            if constexpr (MIXED) {
-                const int qj[3] = {rj.x, rj.y, rj.z};
-                sp_j = rj.w & 0x7fffffff;
-                const int offz = (int)(j >= rr.b1) + (int)(j >= rr.b2) - 1;
-                float d2 = 0.0f;
+
+    // The pair with the candidate at walk position w, weighted as in the reference.
+    auto evaluate = [&](const Walk& w) {
+        const uint32_t j = w.j;
+        const Rec rj = src.record(j, w.ra, w.rb);
+        const bool full = physical || (j >= ov.b && j < ov.e);
+        T z[D];
+        const T d2 = displacement(rj, z);
+        int sp_j;
+        int cut = -1;
+        if constexpr (MIXED) {
+            sp_j = (int)((uint32_t)rj.w & kSpeciesMask);
+            const bool refine = flag_i || rj.w < 0 || d2 == 0.0f || fabsf(d2 - rc2) <= band;
+            if (refine) {
+                const uint32_t gj = src.slot(j);
+                double zd[D], dd = 0.0;
 #pragma unroll
                 for (int k = 0; k < D; k++) {
-                    const int sk = (k == D - 1) ? sh[k] - offz * kQOne : sh[k];
-                    z[k] = (float)(sk - qj[k]) * P.qunit[k + 3 - D];
-                    d2 = d2 + z[k] * z[k];
+                    zd[k] = S.pos[k * N + s] - S.pos[k * N + gj];
+                    dd = dd + zd[k] * zd[k];
+                    z[k] = (float)zd[k];
                 }
-                const bool refine = (((uint32_t)rj.w >> 31) | flag_i) || d2 == 0.0f || fabsf(d2 - rc2) <= band;
-                if (refine) {
-                    const uint32_t gj = src.slot(j);
-                    double zd[D], dd = 0.0;
-#pragma unroll
-                    for (int k = 0; k < D; k++) {
-                        zd[k] = S.pos[k * N + s] - S.pos[k * N + gj];
-                        dd = dd + zd[k] * zd[k];
-                        z[k] = (float)zd[k];
-                    }
-                    cut = cut_decision<POT>(P.pd, sqrt(dd));
-                }
+                cut = cut_decision<POT>(P.pd, sqrt(dd));
+            }
+        } else {
+            sp_j = (int)rj.w;
+        }
+        T f[D];
+        const bool same_sp = sp_i == sp_j;
+        calc_err |= point_force<POT, T, D>(z, same_sp, srow.pick(sp_j), PP, cut, f);
+        T wgt = T(0.5);
+        if (full) {
+            if (symmetric) {
+                wgt = T(1);
             } else {
-                z[0] = me.x - rj.x;
-                z[1] = me.y - rj.y;
-                if constexpr (D == 3) z[2] = me.z - rj.z;
-                sp_j = (int)rj.w;
-            }
-            T f[D];
-            const bool same_sp = sp_i == sp_j;
-            calc_err |= point_force<POT, T, D>(z, same_sp, pick4<T>(srow, sp_j), PP, cut, f);
-            T wgt = T(0.5);
-            if (full) {
-                if (symmetric) {
-                    wgt = T(1);
-                } else {
-                    T zj[D], g[D];
+                T zj[D], g[D];
 #pragma unroll
-                    for (int k = 0; k < D; k++) zj[k] = -z[k];
-                    calc_err |= point_force<POT, T, D>(zj, same_sp, pick4<T>(scol, sp_j), PP, cut, g);
+                for (int k = 0; k < D; k++) zj[k] = -z[k];
+                calc_err |= point_force<POT, T, D>(zj, same_sp, scol.pick(sp_j), PP, cut, g);
 #pragma unroll
-                    for (int k = 0; k < D; k++) {
-                        if constexpr (MIXED) acck[k].add(-g[k] * 0.5f);
-                        else accd[k] = accd[k] + (-g[k]) * 0.5;
-                    }
+                for (int k = 0; k < D; k++) {
+                    if constexpr (MIXED) acck[k].add(-g[k] * 0.5f);
+                    else accd[k] = accd[k] + (-g[k]) * 0.5;
                 }
-            }
-#pragma unroll
-            for (int k = 0; k < D; k++) {
-                if constexpr (MIXED) acck[k].add(f[k] * wgt);
-                else accd[k] = accd[k] + f[k] * wgt;
             }
         }
-        ++j;
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+            if constexpr (MIXED) acck[k].add(f[k] * wgt);
+            else accd[k] = accd[k] + f[k] * wgt;
+        }
+    };
+
+    Walk cur;
+    cur.ra = lo[0];
+    cur.rb = lo[1];
+    cur.row = src.row(lo[0], lo[1]);
+    cur.j = table[cur.row + lo[2]];
+    cur.je = table[cur.row + hi[2]];
+    bool live = POT != CRM_POTENTIAL_NONE;
+    if (live) live = wp.settle(table, cur);
+    while (live) {
+        const Walk c0 = cur;
+        // pass 1: mark the candidates that can be inside the cutoff
+        uint64_t mask = 0;
+        for (uint32_t p = 0; p < 64u; p++) {
+            const uint32_t j = cur.j;
+            const Rec rj = src.record(j, cur.ra, cur.rb);
+            T z[D];
+            const T d2 = displacement(rj, z);
+            bool mark;
+            if constexpr (MIXED) mark = d2 <= thr_f || rj.w < 0;
+            else mark = d2 <= thr_d;
+            mark = mark && j != self;
+            mask |= (uint64_t)mark << p;
+            ++cur.j;
+            live = wp.settle(table, cur);
+            if (!live) break;
+        }
+        // pass 2: replay the chunk and evaluate the marked pairs in stream order
+        Walk w = c0;
+        uint32_t pos = 0;
+        while (mask) {
+            const uint32_t p = (uint32_t)__builtin_ctzll(mask);
+            mask &= mask - 1;
+            w.j += p - pos;
+            pos = p;
+            wp.settle(table, w);
+            evaluate(w);
+        }
     }
     if (calc_err) raise_error(A.ctrl, CRM_CALC_ERROR, A.ctrl->step, S.id[s]);
 #pragma unroll
@@ -325,11 +454,8 @@ __device__ __forceinline__ void agent_force(const Arrays& A, const StateBuf& S, 
     }
 }
 
-// A force tile: rows b in [b0, b1) of plane a0 of one instance, voxels c in
-// [c0, c1) of those rows (either whole rows, or one row cut along its length).
-// Its own agents are one contiguous slot range; the agents of its 3^D stencil
-// are at most 3 * (rows + 2) contiguous row segments.
-constexpr int kTileRowsMax = 8;
+// A force tile: plane a0, rows [b0, b1), columns [c0, c1) of one instance.
+constexpr int kTileRowsMax = 29;
 constexpr int kTileTable = 3 * (kTileRowsMax + 2);
 constexpr int kLdsRecordsMixed = 2048;  // 32 KiB of int4
 constexpr int kLdsRecordsFP64 = 1280;   // 40 KiB of double4
@@ -339,10 +465,10 @@ struct ForceTile {
     uint32_t inst, a0, b0, b1, c0, c1;
 };
 
-// Workgroup per tile: stage the stencil's row segments and their voxel starts in
-// LDS with coalesced loads, then one lane per own agent sums its forces from
-// LDS.  A tile whose neighbourhood does not fit (local crowding) reads from
-// global memory instead; the arithmetic and its order are the same.
+// Workgroup per tile: stage the region's row segments (converted records) and
+// their voxel starts in LDS with coalesced loads, then one lane per own agent
+// sums its forces from LDS.  A tile whose region does not fit (local crowding)
+// reads from global memory instead; the arithmetic and its order are the same.
 template <int D, int POT, bool MIXED>
 __global__ __launch_bounds__(kBlock) void k_force_tiled(Arrays A, StateBuf S, const ForceTile* __restrict__ tiles,
                                                           uint32_t physical, uint32_t symmetric, uint32_t no_lds) {
@@ -352,40 +478,74 @@ __global__ __launch_bounds__(kBlock) void k_force_tiled(Arrays A, StateBuf S, co
     __shared__ uint32_t vst[kLdsVoxelStarts];
     __shared__ uint32_t t_gb[kTileTable];
     __shared__ uint32_t t_off[kTileTable + 1];
+    __shared__ uint32_t own_n[kTileRowsMax + 1];  // prefix counts of the own row segments
     const Rec* __restrict__ rec = rec_array(A, std::integral_constant<bool, MIXED>());
     const ForceTile T = tiles[xcd_block(blockIdx.x, gridDim.x)];
     const InstParams& P = A.ip[T.inst];
     const uint32_t n0 = P.n[0], n1 = P.n[1], n2 = P.n[2], voff = P.voxel_offset;
-    const uint32_t RW = T.b1 - T.b0 + 2;
-    const uint32_t cl = T.c0 == 0 ? 0u : T.c0 - 1u, ch = min(T.c1 + 1u, n2);
-    const uint32_t W1 = ch - cl + 1;
-    const uint32_t n_rows = 3 * RW;
+    Region G;
+    G.a0 = T.a0;
+    G.b0 = T.b0;
+    G.RW = T.b1 - T.b0 + 2;
+    G.cl = T.c0 == 0 ? 0u : T.c0 - 1u;
+    G.ch = min(T.c1 + 1u, n2);
+    G.W1 = G.ch - G.cl + 1;
+    {
+        const uint32_t ext = max(max(D == 3 ? 3u : 1u, G.RW), G.ch - G.cl);
+        const int lg = ext <= 1 ? 0 : 32 - __clz(ext - 1);  // ceil(log2(ext))
+        G.bits = min(30, 31 - lg);
+        G.shift = 30 - G.bits;
+    }
+    float unit[3];
+#pragma unroll
+    for (int k = 0; k < D; k++) unit[k] = MIXED ? ldexpf(P.qunit[k + 3 - D], G.shift) : 0.0f;
+    const uint32_t R = T.b1 - T.b0;
+    const uint32_t n_rows = 3 * G.RW;
     if (threadIdx.x < n_rows) {
-        const uint32_t pa = threadIdx.x / RW, rb = threadIdx.x % RW;
+        const uint32_t pa = threadIdx.x / G.RW, rb = threadIdx.x % G.RW;
         const int a = (int)T.a0 - 1 + (int)pa, b = (int)T.b0 - 1 + (int)rb;
         uint32_t gb = 0, len = 0;
         if (a >= 0 && a < (int)n0 && b >= 0 && b < (int)n1) {
             const uint32_t base = voff + ((uint32_t)a * n1 + (uint32_t)b) * n2;
-            gb = A.start[base + cl];
-            len = A.start[base + ch] - gb;
+            gb = A.start[base + G.cl];
+            len = A.start[base + G.ch] - gb;
         }
         t_gb[threadIdx.x] = gb;
         t_off[threadIdx.x + 1] = len;  // lengths; prefix-summed below
     }
-    if (threadIdx.x == 0) t_off[0] = 0;
+    if (threadIdx.x < R) {
+        const uint32_t base = voff + (T.a0 * n1 + T.b0 + threadIdx.x) * n2;
+        own_n[threadIdx.x + 1] = A.start[base + T.c1] - A.start[base + T.c0];
+    }
+    if (threadIdx.x == 0) {
+        t_off[0] = 0;
+        own_n[0] = 0;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         for (uint32_t r = 1; r <= n_rows; r++) t_off[r] += t_off[r - 1];
+        for (uint32_t r = 1; r <= R; r++) own_n[r] += own_n[r - 1];
     }
     __syncthreads();
     const uint32_t total = t_off[n_rows];
-    const bool use_lds = !no_lds && total <= (uint32_t)CAP && n_rows * W1 <= (uint32_t)kLdsVoxelStarts;
-    const uint32_t own_b = A.start[voff + (T.a0 * n1 + T.b0) * n2 + T.c0];
-    const uint32_t own_e = A.start[voff + (T.a0 * n1 + T.b1 - 1) * n2 + T.c1];
+    const uint32_t n_own = own_n[R];
+    const bool use_lds = !no_lds && total <= (uint32_t)CAP && n_rows * G.W1 <= (uint32_t)kLdsVoxelStarts;
+    // own agent t of the tile: row r of the tile, slot in the sorted order
+    auto own_slot = [&](uint32_t t, uint32_t& r) -> uint32_t {
+        uint32_t l = 0, h = R;  // largest l with own_n[l] <= t
+        while (h - l > 1) {
+            const uint32_t mid = (l + h) >> 1;
+            if (own_n[mid] <= t) l = mid;
+            else h = mid;
+        }
+        r = l;
+        const uint32_t base = voff + (T.a0 * n1 + T.b0 + l) * n2;
+        return A.start[base + T.c0] + (t - own_n[l]);
+    };
     if (use_lds) {
-        for (uint32_t e = threadIdx.x; e < n_rows * W1; e += kBlock) {
-            const uint32_t r = e / W1, c = cl + e % W1;
-            const uint32_t pa = r / RW, rb = r % RW;
+        for (uint32_t e = threadIdx.x; e < n_rows * G.W1; e += kBlock) {
+            const uint32_t r = e / G.W1, c = G.cl + e % G.W1;
+            const uint32_t pa = r / G.RW, rb = r % G.RW;
             const int a = (int)T.a0 - 1 + (int)pa, b = (int)T.b0 - 1 + (int)rb;
             uint32_t x = t_off[r];
             if (a >= 0 && a < (int)n0 && b >= 0 && b < (int)n1)
@@ -393,28 +553,31 @@ __global__ __launch_bounds__(kBlock) void k_force_tiled(Arrays A, StateBuf S, co
             vst[e] = x;
         }
         for (uint32_t i = threadIdx.x; i < total; i += kBlock) {
-            uint32_t lo = 0, hi = n_rows;  // largest lo with t_off[lo] <= i
-            while (hi - lo > 1) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (t_off[mid] <= i) lo = mid;
-                else hi = mid;
+            uint32_t l = 0, h = n_rows;  // largest l with t_off[l] <= i
+            while (h - l > 1) {
+                const uint32_t mid = (l + h) >> 1;
+                if (t_off[mid] <= i) l = mid;
+                else h = mid;
             }
-            lds[i] = rec[t_gb[lo] + (i - t_off[lo])];
+            const Rec r = rec[t_gb[l] + (i - t_off[l])];
+            if constexpr (MIXED) lds[i] = to_region<D>(r, (int)(l / G.RW), (int)(l % G.RW), G);
+            else lds[i] = r;
         }
         __syncthreads();
-        LdsSource<MIXED> src{lds, vst, t_gb, t_off, T.a0, T.b0, RW, W1, cl};
-        const uint32_t own_row = RW;  // own rows are pa = 1
-        for (uint32_t s = own_b + threadIdx.x; s < own_e; s += kBlock) {
-            // own agent's row inside the tile: its slot lies in the staged row segment
-            const uint32_t lv = S.vox[s] - voff;
-            const uint32_t b = (lv / n2) % n1;
-            const uint32_t rt = own_row + (b + 1 - T.b0);
-            agent_force<D, POT, MIXED>(A, S, P, s, s - t_gb[rt] + t_off[rt], physical, symmetric, src);
+        LdsSource<D, MIXED> src{lds, vst, t_gb, t_off, G};
+        for (uint32_t t = threadIdx.x; t < n_own; t += kBlock) {
+            uint32_t r;
+            const uint32_t s = own_slot(t, r);
+            const uint32_t rt = G.RW + 1 + r;  // staged row of own row r (plane offset 1)
+            agent_force<D, POT, MIXED>(A, S, P, s, s - t_gb[rt] + t_off[rt], physical, symmetric, src, unit);
         }
     } else {
-        GlobalSource<MIXED> src{rec, A.start, voff, n1, n2};
-        for (uint32_t s = own_b + threadIdx.x; s < own_e; s += kBlock)
-            agent_force<D, POT, MIXED>(A, S, P, s, s, physical, symmetric, src);
+        GlobalSource<D, MIXED> src{rec, A.start, voff, n1, n2, G};
+        for (uint32_t t = threadIdx.x; t < n_own; t += kBlock) {
+            uint32_t r;
+            const uint32_t s = own_slot(t, r);
+            agent_force<D, POT, MIXED>(A, S, P, s, s, physical, symmetric, src, unit);
+        }
     }
 }
 
@@ -799,7 +962,8 @@ __device__ __forceinline__ void write_record(const Arrays& A, const InstParams& 
             q[k] = ok ? (int)tq : (tq < 0.0 ? 0 : kQOne - 1);
             outside |= ok ? 0u : 1u;
         }
-        A.recq[s] = make_int4(q[0], q[1], D == 3 ? q[2] : 0, (int)((uint32_t)sp | (outside << 31)));
+        const uint32_t w = ((uint32_t)sp & kSpeciesMask) | ((idx[2] & kColMask) << kColShift) | (outside << 31);
+        A.recq[s] = make_int4(q[0], q[1], D == 3 ? q[2] : 0, (int)w);
     } else {
         A.rec64[s] = make_double4(p[0], p[1], D == 3 ? p[D - 1] : 0.0, (double)sp);
     }
