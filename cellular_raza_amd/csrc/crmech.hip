@@ -214,8 +214,12 @@ Arrays arrays(crm_ctx* ctx) {
     A.ctrl = ctx->d_ctrl;
     A.start = ctx->d_start;
     A.count = ctx->d_count;
-    A.key = ctx->d_key;
+    A.stay = ctx->d_count + ctx->Vt;
+    // two key buffers: the current step's (read by the sort) and the next step's
+    // (binned ahead by the gather), by state-buffer parity
+    A.key = ctx->d_key + (size_t)ctx->cur * ctx->N;
     A.rank = ctx->d_rank;
+    A.key_next = ctx->d_key + (size_t)(1 - ctx->cur) * ctx->N;
     A.perm = ctx->d_perm;
     A.force = ctx->d_force;
     A.rec64 = ctx->d_rec64;
@@ -397,13 +401,36 @@ crm_status launch_gather_integrate_dm(crm_ctx* ctx, StateBuf src, StateBuf dst, 
     return CRM_INVALID_ARGUMENT;
 }
 
+// Initial placement gather; NewtonDamped / Langevin also bin the first step.
+template <int D, bool MIXED>
+crm_status launch_gather_setup_d(crm_ctx* ctx, StateBuf src, StateBuf dst, const StepArgs& sa) {
+    Arrays A = arrays(ctx);
+    const dim3 g(ctx->n_work), b(kBlock);
+    switch (ctx->mech_kind) {
+        case CRM_MECHANICS_NEWTON_DAMPED:
+            hipLaunchKernelGGL((k_gather_point<D, MIXED, true, CRM_MECHANICS_NEWTON_DAMPED>), g, b, 0, ctx->stream, A,
+                               src, dst, sa);
+            return CRM_OK;
+        case CRM_MECHANICS_LANGEVIN:
+            hipLaunchKernelGGL((k_gather_point<D, MIXED, true, CRM_MECHANICS_LANGEVIN>), g, b, 0, ctx->stream, A, src,
+                               dst, sa);
+            return CRM_OK;
+        case CRM_MECHANICS_BROWNIAN:
+            hipLaunchKernelGGL((k_gather_point<D, MIXED, false, CRM_MECHANICS_BROWNIAN>), g, b, 0, ctx->stream, A,
+                               src, dst, sa);
+            return CRM_OK;
+    }
+    return CRM_INVALID_ARGUMENT;
+}
+
 // scan + scatter + fixup + gather: the counting sort that rebuilds the voxel
 // lists from d_key / d_rank / d_count and moves the state to the other buffer.
 crm_status launch_sort(crm_ctx* ctx, bool advance, const StepArgs* sa) {
     Arrays A = arrays(ctx);
     const uint32_t Vt = (uint32_t)ctx->Vt;
     CRM_TRY(t_begin(ctx));
-    hipLaunchKernelGGL(k_scan_local, dim3(ctx->n_tiles), dim3(kBlock), 0, ctx->stream, ctx->d_count, ctx->d_start,
+    hipLaunchKernelGGL(k_scan_local, dim3(ctx->n_tiles), dim3(kBlock), 0, ctx->stream, ctx->d_count,
+                       ctx->d_count + Vt, ctx->d_start,
                        ctx->d_bsum, Vt);
     hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->d_bsum, ctx->n_tiles, ctx->d_start,
                        Vt, ctx->d_ctrl, (uint32_t)(advance ? 1 : 0));
@@ -439,20 +466,12 @@ crm_status launch_sort(crm_ctx* ctx, bool advance, const StepArgs* sa) {
         else
             gst = mixed ? launch_gather_integrate_dm<3, true>(ctx, src, dst, *sa)
                         : launch_gather_integrate_dm<3, false>(ctx, src, dst, *sa);
-    } else if (ctx->D == 2) {
-        if (mixed)
-            hipLaunchKernelGGL((k_gather_point<2, true>), dim3(ctx->n_work), dim3(kBlock), 0, ctx->stream, A, src, dst,
-                               cap);
-        else
-            hipLaunchKernelGGL((k_gather_point<2, false>), dim3(ctx->n_work), dim3(kBlock), 0, ctx->stream, A, src,
-                               dst, cap);
     } else {
-        if (mixed)
-            hipLaunchKernelGGL((k_gather_point<3, true>), dim3(ctx->n_work), dim3(kBlock), 0, ctx->stream, A, src, dst,
-                               cap);
-        else
-            hipLaunchKernelGGL((k_gather_point<3, false>), dim3(ctx->n_work), dim3(kBlock), 0, ctx->stream, A, src,
-                               dst, cap);
+        const StepArgs sa0 = sa ? *sa : step_args(ctx, nullptr);
+        gst = ctx->D == 2 ? (mixed ? launch_gather_setup_d<2, true>(ctx, src, dst, sa0)
+                                   : launch_gather_setup_d<2, false>(ctx, src, dst, sa0))
+                          : (mixed ? launch_gather_setup_d<3, true>(ctx, src, dst, sa0)
+                                   : launch_gather_setup_d<3, false>(ctx, src, dst, sa0));
     }
     if (gst) return gst;
     CRM_TRY(t_end(ctx, CRM_KERNEL_GATHER));
@@ -460,10 +479,22 @@ crm_status launch_sort(crm_ctx* ctx, bool advance, const StepArgs* sa) {
     return CRM_OK;
 }
 
+// NewtonDamped and Langevin point agents are binned one step ahead by the
+// gather (their position update needs neither force nor noise); the others get
+// a key kernel after the forces.
+bool keys_ahead(const crm_ctx* ctx) { return !is_rod(ctx) && ctx->mech_kind != CRM_MECHANICS_BROWNIAN; }
+
 crm_status launch_step(crm_ctx* ctx, const double* noise_step) {
     const StepArgs sa = step_args(ctx, noise_step);
     CRM_TRY(launch_force(ctx));
-    CRM_TRY(launch_key(ctx, sa));
+    if (keys_ahead(ctx)) {
+        CRM_TRY(t_begin(ctx));
+        hipLaunchKernelGGL(k_rank_ahead, dim3((uint32_t)((ctx->Vt + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                           ctx->stream, arrays(ctx));
+        CRM_TRY(t_end(ctx, CRM_KERNEL_INTEGRATE));
+    } else {
+        CRM_TRY(launch_key(ctx, sa));
+    }
     CRM_TRY(launch_sort(ctx, true, &sa));
     return CRM_OK;
 }
@@ -786,9 +817,9 @@ crm_status crm_set_agents(crm_ctx* ctx, const uint64_t* counts, const double* po
     ctx->n_tiles = (uint32_t)((Vt + kScanTile - 1) / kScanTile);
     CRM_TRY(dalloc(ctx, ctx->d_work, work.size()));
     CRM_TRY(dalloc(ctx, ctx->d_start, Vt + 1));
-    CRM_TRY(dalloc(ctx, ctx->d_count, Vt));
+    CRM_TRY(dalloc(ctx, ctx->d_count, 2 * Vt));  // arrivals, then stayers
     CRM_TRY(dalloc(ctx, ctx->d_bsum, ctx->n_tiles));
-    CRM_TRY(dalloc(ctx, ctx->d_key, N));
+    CRM_TRY(dalloc(ctx, ctx->d_key, 2 * N));
     CRM_TRY(dalloc(ctx, ctx->d_rank, N));
     CRM_TRY(dalloc(ctx, ctx->d_perm, N));
     CRM_TRY(dalloc(ctx, ctx->d_force, (size_t)K * N));
@@ -807,7 +838,7 @@ crm_status crm_set_agents(crm_ctx* ctx, const uint64_t* counts, const double* po
     CRM_HIP(hipMemcpy(ctx->d_work, work.data(), sizeof(BlockWork) * work.size(), hipMemcpyHostToDevice));
     CRM_TRY(dalloc(ctx, ctx->d_tiles, tiles.size()));
     CRM_HIP(hipMemcpy(ctx->d_tiles, tiles.data(), sizeof(ForceTile) * tiles.size(), hipMemcpyHostToDevice));
-    CRM_HIP(hipMemsetAsync(ctx->d_count, 0, sizeof(uint32_t) * std::max<uint64_t>(Vt, 1), ctx->stream));
+    CRM_HIP(hipMemsetAsync(ctx->d_count, 0, sizeof(uint32_t) * std::max<uint64_t>(2 * Vt, 1), ctx->stream));
 
     // upload in agent-index order (slot = agent index before the first sort)
     ctx->cur = 0;

@@ -20,6 +20,8 @@
 namespace crm {
 
 constexpr int kBlock = 256;
+// rank flag of an agent binned ahead that changes voxel (see k_rank_ahead)
+constexpr uint32_t kMoved = 0x80000000u;
 
 struct StateBuf {
     double* pos;  // [K][N]
@@ -36,9 +38,11 @@ struct Arrays {
     const BlockWork* work;
     Control* ctrl;
     uint32_t* start;  // [Vt + 1]
-    uint32_t* count;  // [Vt]
+    uint32_t* count;  // [Vt] arrivals per next voxel (atomic ranks)
+    uint32_t* stay;   // [Vt] agents binned ahead that stay in their voxel (zero otherwise)
     uint32_t* key;    // [N] next voxel
     uint32_t* rank;   // [N] arrival rank inside the next voxel
+    uint32_t* key_next;   // [N] next voxel of the following step (binned ahead by the gather)
     uint32_t* perm;   // [N] sorted slot -> previous slot
     double* force;    // [K][N]
     double4* rec64;   // [N] (x, y, z, species) absolute, FP64 path
@@ -864,6 +868,7 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t x, uint32_t& t
 }
 
 __global__ __launch_bounds__(kBlock) void k_scan_local(const uint32_t* __restrict__ count,
+                                                        const uint32_t* __restrict__ stay,
                                                         uint32_t* __restrict__ start, uint32_t* __restrict__ bsum,
                                                         uint32_t Vt) {
     const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
@@ -872,7 +877,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_local(const uint32_t* __restric
 #pragma unroll
     for (int i = 0; i < kScanItems; i++) {
         uint64_t q = base + i;
-        vals[i] = q < Vt ? count[q] : 0u;
+        vals[i] = q < Vt ? count[q] + stay[q] : 0u;
         sum += vals[i];
     }
     uint32_t total;
@@ -916,7 +921,9 @@ __global__ __launch_bounds__(kBlock) void k_scan_add(uint32_t* __restrict__ star
 __global__ __launch_bounds__(kBlock) void k_scatter(Arrays A) {
     const uint64_t s = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     if (s >= A.N) return;
-    A.perm[A.start[A.key[s]] + A.rank[s]] = (uint32_t)s;
+    const uint32_t k = A.key[s], r = A.rank[s];
+    const uint32_t pos = (r & kMoved) ? A.stay[k] + (r & ~kMoved) : r;
+    A.perm[A.start[k] + pos] = (uint32_t)s;
 }
 
 // Orders each voxel's slots by previous slot (atomic arrival order is not
@@ -969,37 +976,95 @@ __device__ __forceinline__ void write_record(const Arrays& A, const InstParams& 
     }
 }
 
+// Next voxel of the agent in slot s for the step that starts after t1 steps,
+// from its state at that point: position x, velocity v and the ring slots r0 /
+// r1 of position increments.  For NewtonDamped and Langevin the position update
+// depends on neither the force nor the noise, so the re-binning of step t1 is
+// known once step t1 - 1 has been integrated (k_key_point computes the same
+// position).  A position whose index has no voxel keeps `vox`.
+template <int D, int MECH>
+__device__ __forceinline__ void bin_ahead(const Arrays& A, const InstParams& P, const StepArgs& sa, uint64_t t1,
+                                          uint32_t id, uint64_t s, const double* x, const double* v,
+                                          const double* r0, const double* r1, uint32_t vox) {
+    const Ring rg = ring_at(t1, sa.cap, sa.newest_first);
+    double f[D], nx[D], nv[D], dx[D], dv[D];
+#pragma unroll
+    for (int k = 0; k < D; k++) f[k] = 0.0;
+    auto hist = [&](int i, int k, bool) { return rg.it[i] ? r1[k] : r0[k]; };
+    point_step<D, MECH, false>(P, sa, rg, t1, id, x, v, f, hist, nx, nv, dx, dv);
+    uint32_t idx[3] = {0, 0, 0};
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+        const int ax = k + 3 - D;
+        ok = ok && axis_index(nx[k], P.min[ax], P.dx[ax], P.n[ax], idx[ax]);
+    }
+    uint32_t key = vox;
+    if (ok) key = P.voxel_offset + (idx[0] * P.n[1] + idx[1]) * P.n[2] + idx[2];
+    else atomicAdd((unsigned long long*)&A.ctrl->out_of_grid, 1ull);
+    A.key_next[s] = key;
+}
+
+// Arrival ranks of the keys binned ahead, with an atomic only for agents that
+// change voxel.  Thread per voxel v of the current order: the agents of v that
+// stay get consecutive ranks in slot order and their number goes to stay[v];
+// the others take an atomic rank among the arrivals of their next voxel
+// (count), flagged with kMoved, which the scatter places after that voxel's
+// stayers.  k_fixup orders every voxel by previous slot afterwards, so only the
+// uniqueness of the ranks matters.
+__global__ __launch_bounds__(kBlock) void k_rank_ahead(Arrays A) {
+    const uint64_t v = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (v >= A.Vt) return;
+    const uint32_t b = A.start[v], e = A.start[v + 1];
+    uint32_t n = 0;
+    for (uint32_t s = b; s < e; s++) {
+        const uint32_t k = A.key[s];
+        if (k == (uint32_t)v) A.rank[s] = n++;
+        else A.rank[s] = kMoved | atomicAdd(&A.count[k], 1u);
+    }
+    A.stay[v] = n;
+}
+
 // Permutes the state into sorted order and writes the force records, without
 // integrating (initial placement).  The ring holds min(step, cap) increments.
-template <int D, bool MIXED>
-__global__ __launch_bounds__(kBlock) void k_gather_point(Arrays A, StateBuf src, StateBuf dst, uint32_t cap) {
+// With AHEAD the next step's voxels are binned as well.
+template <int D, bool MIXED, bool AHEAD, int MECH>
+__global__ __launch_bounds__(kBlock) void k_gather_point(Arrays A, StateBuf src, StateBuf dst, StepArgs sa) {
     const BlockWork w = A.work[blockIdx.x];
     if (threadIdx.x >= w.count) return;
     const uint64_t s = w.first + threadIdx.x;
     const InstParams& P = A.ip[w.inst];
     const uint64_t N = A.N;
     const uint32_t o = A.perm[s];
-    const uint32_t n_hist = (uint32_t)min<uint64_t>(A.ctrl->step, cap);
-    double p[D];
+    const uint64_t t = A.ctrl->step;
+    const uint32_t n_hist = (uint32_t)min<uint64_t>(t, sa.cap);
+    double p[D], v[D], r0[D], r1[D];
 #pragma unroll
     for (int k = 0; k < D; k++) {
         p[k] = src.pos[k * N + o];
+        v[k] = src.vel[k * N + o];
         dst.pos[k * N + s] = p[k];
-        dst.vel[k * N + s] = src.vel[k * N + o];
+        dst.vel[k * N + s] = v[k];
+        r0[k] = r1[k] = 0.0;
     }
     for (uint32_t h = 0; h < n_hist; h++) {
 #pragma unroll
         for (int k = 0; k < D; k++) {
-            dst.hdx[((uint64_t)h * D + k) * N + s] = src.hdx[((uint64_t)h * D + k) * N + o];
+            const double hx = src.hdx[((uint64_t)h * D + k) * N + o];
+            dst.hdx[((uint64_t)h * D + k) * N + s] = hx;
             dst.hdv[((uint64_t)h * D + k) * N + s] = src.hdv[((uint64_t)h * D + k) * N + o];
+            if (h == 0) r0[k] = hx;
+            else r1[k] = hx;
         }
     }
     const int32_t sp = src.species[o];
+    const uint32_t id = src.id[o];
     dst.species[s] = sp;
-    dst.id[s] = src.id[o];
+    dst.id[s] = id;
     const uint32_t key = A.key[o];
     dst.vox[s] = key;
     write_record<D, MIXED>(A, P, s, key, p, sp);
+    if constexpr (AHEAD) bin_ahead<D, MECH>(A, P, sa, t, id, s, p, v, r0, r1, key);
 }
 
 // Gather + integrate: the agent that lands in sorted slot s comes from slot
@@ -1026,11 +1091,24 @@ __global__ __launch_bounds__(kBlock) void k_gather_integrate_point(Arrays A, Sta
         v[k] = src.vel[k * N + o];
         f[k] = A.force[k * N + o];
     }
+    // the ring before this step (min(t, cap) entries), in registers
+    const uint32_t n_old = (uint32_t)min<uint64_t>(t, rg.cap);
+    double hx[2][D], hv[2][D];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+            const uint64_t e = ((uint64_t)h * D + k) * N + o;
+            hx[h][k] = (uint32_t)h < n_old ? src.hdx[e] : 0.0;
+            hv[h][k] = (uint32_t)h < n_old ? src.hdv[e] : 0.0;
+        }
+    }
     auto hist = [&](int i, int k, bool is_dv) {
-        const double* h = is_dv ? src.hdv : src.hdx;
-        return h[((uint64_t)rg.it[i] * D + k) * N + o];
+        const uint32_t sl = rg.it[i];
+        return is_dv ? (sl ? hv[1][k] : hv[0][k]) : (sl ? hx[1][k] : hx[0][k]);
     };
-    point_step<D, MECH, true>(P, sa, rg, t, id, x, v, f, hist, nx, nv, dx, dv);
+    const bool out = point_step<D, MECH, true>(P, sa, rg, t, id, x, v, f, hist, nx, nv, dx, dv);
+    if (out) raise_error(A.ctrl, CRM_BOUNDARY_ERROR, t, id);
 #pragma unroll
     for (int k = 0; k < D; k++) {
         dst.pos[k * N + s] = nx[k];
@@ -1039,12 +1117,19 @@ __global__ __launch_bounds__(kBlock) void k_gather_integrate_point(Arrays A, Sta
     // ring: the slot written this step gets (dx, dv), the other slot is carried over
     if (rg.cap > 0) {
         const uint32_t n_hist = (uint32_t)min<uint64_t>(t + 1, rg.cap);
-        for (uint32_t h = 0; h < n_hist; h++) {
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
 #pragma unroll
             for (int k = 0; k < D; k++) {
-                const uint64_t e = ((uint64_t)h * D + k) * N;
-                dst.hdx[e + s] = h == rg.wslot ? dx[k] : src.hdx[e + o];
-                dst.hdv[e + s] = h == rg.wslot ? dv[k] : src.hdv[e + o];
+                if (h == (int)rg.wslot) {
+                    hx[h][k] = dx[k];
+                    hv[h][k] = dv[k];
+                }
+                if ((uint32_t)h < n_hist) {
+                    const uint64_t e = ((uint64_t)h * D + k) * N + s;
+                    dst.hdx[e] = hx[h][k];
+                    dst.hdv[e] = hv[h][k];
+                }
             }
         }
     }
@@ -1054,6 +1139,8 @@ __global__ __launch_bounds__(kBlock) void k_gather_integrate_point(Arrays A, Sta
     const uint32_t key = A.key[o];
     dst.vox[s] = key;
     write_record<D, MIXED>(A, P, s, key, nx, sp);
+    if constexpr (MECH != CRM_MECHANICS_BROWNIAN)
+        bin_ahead<D, MECH>(A, P, sa, t + 1, id, s, nx, nv, hx[0], hx[1], key);
 }
 
 }  // namespace crm
