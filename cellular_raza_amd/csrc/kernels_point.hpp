@@ -461,9 +461,9 @@ __device__ __forceinline__ void agent_force(const Arrays& A, const StateBuf& S, 
 // A force tile: plane a0, rows [b0, b1), columns [c0, c1) of one instance.
 constexpr int kTileRowsMax = 29;
 constexpr int kTileTable = 3 * (kTileRowsMax + 2);
-constexpr int kLdsRecordsMixed = 2048;  // 32 KiB of int4
-constexpr int kLdsRecordsFP64 = 1280;   // 40 KiB of double4
-constexpr int kLdsVoxelStarts = 1536;   // 6 KiB of voxel-start indices
+constexpr int kLdsRecordsMixed = 1024;  // 16 KiB of int4
+constexpr int kLdsRecordsFP64 = 896;    // 28 KiB of double4
+constexpr int kLdsVoxelStarts = 768;    // 3 KiB of voxel-start indices
 
 struct ForceTile {
     uint32_t inst, a0, b0, b1, c0, c1;
@@ -474,7 +474,7 @@ struct ForceTile {
 // sums its forces from LDS.  A tile whose region does not fit (local crowding)
 // reads from global memory instead; the arithmetic and its order are the same.
 template <int D, int POT, bool MIXED>
-__global__ __launch_bounds__(kBlock) void k_force_tiled(Arrays A, StateBuf S, const ForceTile* __restrict__ tiles,
+__global__ __launch_bounds__(kBlock, MIXED ? 8 : 4) void k_force_tiled(Arrays A, StateBuf S, const ForceTile* __restrict__ tiles,
                                                           uint32_t physical, uint32_t symmetric, uint32_t no_lds) {
     using Rec = typename RecOf<MIXED>::type;
     constexpr int CAP = MIXED ? kLdsRecordsMixed : kLdsRecordsFP64;
