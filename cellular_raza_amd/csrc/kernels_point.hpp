@@ -64,17 +64,6 @@ __device__ __forceinline__ const PotParams<double>& pot_of<double>(const InstPar
 template <>
 __device__ __forceinline__ const PotParams<float>& pot_of<float>(const InstParams& P) { return P.pf; }
 
-// Compensated (Kahan) running sum for the float pair path.
-struct Kahan {
-    float s, c;
-    __device__ __forceinline__ void add(float x) {
-        float y = x - c;
-        float t = s + y;
-        c = (t - s) - y;
-        s = t;
-    }
-};
-
 // ---------------------------------------------------------------------------
 // Force kernel.  One lane per agent; candidates are the agents of the clipped
 // 3^D stencil (get_neighbor_voxel_indices, cartesian_cuboid_n.rs:641-662) plus
@@ -236,13 +225,12 @@ struct WalkPlan {
     __device__ __forceinline__ bool settle(const uint32_t* table, Walk& w) const {
         while (w.j >= w.je) {
             const uint32_t over = w.j - w.je;
-            if (++w.rb == hi1) {
-                w.rb = lo1;
-                if (++w.ra == hi0) return false;
-                w.row += step_a;
-            } else {
-                w.row += step_b;
-            }
+            const uint32_t rb1 = w.rb + 1;
+            const bool wrap = rb1 == hi1;
+            w.rb = wrap ? lo1 : rb1;
+            w.ra += wrap ? 1u : 0u;
+            if (w.ra == hi0) return false;
+            w.row += wrap ? step_a : step_b;
             w.j = table[w.row + c_lo] + over;
             w.je = table[w.row + c_hi];
         }
@@ -263,16 +251,11 @@ struct WalkPlan {
 // instead of max-over-lanes of the candidate count.
 template <int D, int POT, bool MIXED, typename Src>
 __device__ __forceinline__ void agent_force(const Arrays& A, const StateBuf& S, const InstParams& P, uint64_t s,
-                                            uint32_t self, uint32_t physical, uint32_t symmetric, const Src& src,
-                                            const float* unit) {
+                                            uint32_t self, const uint32_t* idx, uint32_t physical,
+                                            uint32_t symmetric, const Src& src, const float* unit) {
     using T = typename std::conditional<MIXED, float, double>::type;
     using Rec = typename RecOf<MIXED>::type;
     const uint64_t N = A.N;
-    const uint32_t lv = S.vox[s] - P.voxel_offset;
-    uint32_t idx[3];
-    idx[2] = lv % P.n[2];
-    idx[1] = (lv / P.n[2]) % P.n[1];
-    idx[0] = lv / (P.n[2] * P.n[1]);
     uint32_t lo[3], hi[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
@@ -296,8 +279,16 @@ __device__ __forceinline__ void agent_force(const Arrays& A, const StateBuf& S, 
     // candidate test thresholds: mixed marks d2 <= rc2 + band (the band pairs are
     // re-decided in FP64) and every pair with a flagged agent; FP64 marks
     // d2 <= rc^2 (1 + 1e-12), beyond which the rounded sqrt(d2) exceeds rc
-    float thr_f = rc2 + band;
-    if (!(thr_f >= 0.0f) || flag_i) thr_f = __builtin_inff();
+    // pass 1 measures in units of the first axis' fixed-point step (the other axes
+    // scaled by their step ratio) and keeps half a band more margin
+    float r1 = 1.0f, r2 = 1.0f, thr_f = 0.0f;
+    if constexpr (MIXED) {
+        const float u0 = unit[0];
+        r1 = unit[1] / u0;
+        r2 = unit[D - 1] / u0;
+        thr_f = (rc2 + 1.5f * band) / (u0 * u0);
+        if (!(thr_f >= 0.0f) || flag_i) thr_f = __builtin_inff();
+    }
     const double rcd = (double)cut_radius<POT, double>(P.pd);
     double thr_d = rcd * rcd * (1.0 + 1e-12);
     if (!(thr_d >= 0.0)) thr_d = __builtin_inf();
@@ -331,14 +322,22 @@ __device__ __forceinline__ void agent_force(const Arrays& A, const StateBuf& S, 
         srow = Row4<T>{r4[0], r4[1], r4[2], r4[3]};
         scol = Row4<T>{c4[0], c4[1], c4[2], c4[3]};
     }
+    // per-agent sums: double in FP64 mode; compensated (Kahan) float sums in
+    // mixed mode (ks + kc; measured faster on gfx950 than a double accumulator)
     double accd[D];
-    Kahan acck[D];
+    float ks[D], kc[D];
 #pragma unroll
     for (int k = 0; k < D; k++) {
         accd[k] = 0.0;
-        acck[k].s = 0.0f;
-        acck[k].c = 0.0f;
+        ks[k] = 0.0f;
+        kc[k] = 0.0f;
     }
+    auto kadd = [&](int k, float x) {
+        const float y = x - kc[k];
+        const float t = ks[k] + y;
+        kc[k] = (t - ks[k]) - y;
+        ks[k] = t;
+    };
     bool calc_err = false;
 
     // Pair displacement own - candidate; returns the squared distance.
@@ -398,14 +397,14 @@ __device__ __forceinline__ void agent_force(const Arrays& A, const StateBuf& S, 
                 calc_err |= point_force<POT, T, D>(zj, same_sp, scol.pick(sp_j), PP, cut, g);
 #pragma unroll
                 for (int k = 0; k < D; k++) {
-                    if constexpr (MIXED) acck[k].add(-g[k] * 0.5f);
+                    if constexpr (MIXED) kadd(k, -g[k] * 0.5f);
                     else accd[k] = accd[k] + (-g[k]) * 0.5;
                 }
             }
         }
 #pragma unroll
         for (int k = 0; k < D; k++) {
-            if constexpr (MIXED) acck[k].add(f[k] * wgt);
+            if constexpr (MIXED) kadd(k, f[k] * wgt);
             else accd[k] = accd[k] + f[k] * wgt;
         }
     };
@@ -420,18 +419,35 @@ __device__ __forceinline__ void agent_force(const Arrays& A, const StateBuf& S, 
     if (live) live = wp.settle(table, cur);
     while (live) {
         const Walk c0 = cur;
-        // pass 1: mark the candidates that can be inside the cutoff
-        uint64_t mask = 0;
+        // pass 1: mark the candidates that can be inside the cutoff (bit p of
+        // mlo / mhi for stream position p of the chunk)
+        uint32_t mlo = 0, mhi = 0;
         for (uint32_t p = 0; p < 64u; p++) {
             const uint32_t j = cur.j;
             const Rec rj = src.record(j, cur.ra, cur.rb);
-            T z[D];
-            const T d2 = displacement(rj, z);
             bool mark;
-            if constexpr (MIXED) mark = d2 <= thr_f || rj.w < 0;
-            else mark = d2 <= thr_d;
+            if constexpr (MIXED) {
+                const float c0 = (float)(me.x - rj.x);
+                const float c1 = (float)(me.y - rj.y) * r1;
+                float d2 = fmaf(c1, c1, c0 * c0);
+                if constexpr (D == 3) {
+                    const float c2 = (float)(me.z - rj.z) * r2;
+                    d2 = fmaf(c2, c2, d2);
+                }
+                mark = d2 <= thr_f || rj.w < 0;
+            } else {
+                const double z0 = me.x - rj.x, z1 = me.y - rj.y;
+                double d2 = fma(z1, z1, z0 * z0);
+                if constexpr (D == 3) {
+                    const double z2 = me.z - rj.z;
+                    d2 = fma(z2, z2, d2);
+                }
+                mark = d2 <= thr_d;
+            }
             mark = mark && j != self;
-            mask |= (uint64_t)mark << p;
+            const uint32_t bit = 1u << (p & 31u);
+            if (p < 32u) mlo = mark ? (mlo | bit) : mlo;
+            else mhi = mark ? (mhi | bit) : mhi;
             ++cur.j;
             live = wp.settle(table, cur);
             if (!live) break;
@@ -439,9 +455,15 @@ __device__ __forceinline__ void agent_force(const Arrays& A, const StateBuf& S, 
         // pass 2: replay the chunk and evaluate the marked pairs in stream order
         Walk w = c0;
         uint32_t pos = 0;
-        while (mask) {
-            const uint32_t p = (uint32_t)__builtin_ctzll(mask);
-            mask &= mask - 1;
+        while (mlo | mhi) {
+            uint32_t p;
+            if (mlo) {
+                p = (uint32_t)__builtin_ctz(mlo);
+                mlo &= mlo - 1;
+            } else {
+                p = 32u + (uint32_t)__builtin_ctz(mhi);
+                mhi &= mhi - 1;
+            }
             w.j += p - pos;
             pos = p;
             wp.settle(table, w);
@@ -450,12 +472,7 @@ __device__ __forceinline__ void agent_force(const Arrays& A, const StateBuf& S, 
     }
     if (calc_err) raise_error(A.ctrl, CRM_CALC_ERROR, A.ctrl->step, S.id[s]);
 #pragma unroll
-    for (int k = 0; k < D; k++) {
-        double fk;
-        if constexpr (MIXED) fk = (double)acck[k].s + (double)(-acck[k].c);
-        else fk = accd[k];
-        A.force[(uint64_t)k * N + s] = fk;
-    }
+    for (int k = 0; k < D; k++) A.force[(uint64_t)k * N + s] = MIXED ? (double)ks[k] + (double)(-kc[k]) : accd[k];
 }
 
 // A force tile: plane a0, rows [b0, b1), columns [c0, c1) of one instance.
@@ -573,14 +590,16 @@ __global__ __launch_bounds__(kBlock, MIXED ? 8 : 4) void k_force_tiled(Arrays A,
             uint32_t r;
             const uint32_t s = own_slot(t, r);
             const uint32_t rt = G.RW + 1 + r;  // staged row of own row r (plane offset 1)
-            agent_force<D, POT, MIXED>(A, S, P, s, s - t_gb[rt] + t_off[rt], physical, symmetric, src, unit);
+            const uint32_t idx[3] = {T.a0, T.b0 + r, S.vox[s] - (voff + (T.a0 * n1 + T.b0 + r) * n2)};
+            agent_force<D, POT, MIXED>(A, S, P, s, s - t_gb[rt] + t_off[rt], idx, physical, symmetric, src, unit);
         }
     } else {
         GlobalSource<D, MIXED> src{rec, A.start, voff, n1, n2, G};
         for (uint32_t t = threadIdx.x; t < n_own; t += kBlock) {
             uint32_t r;
             const uint32_t s = own_slot(t, r);
-            agent_force<D, POT, MIXED>(A, S, P, s, s, physical, symmetric, src, unit);
+            const uint32_t idx[3] = {T.a0, T.b0 + r, S.vox[s] - (voff + (T.a0 * n1 + T.b0 + r) * n2)};
+            agent_force<D, POT, MIXED>(A, S, P, s, s, idx, physical, symmetric, src, unit);
         }
     }
 }
