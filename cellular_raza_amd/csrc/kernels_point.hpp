@@ -173,11 +173,11 @@ template <int D, bool MIXED>
 struct LdsSource {
     using Rec = typename RecOf<MIXED>::type;
     const Rec* lds;       // staged records, converted
-    const uint32_t* vst;  // per staged row: LDS index of the first agent of columns cl .. ch
+    const uint16_t* vst;  // per staged row: LDS index of the first agent of columns cl .. ch
     const uint32_t* gb;   // per staged row: global slot of its first agent
     const uint32_t* off;  // per staged row: its first LDS index
     Region G;
-    __device__ __forceinline__ const uint32_t* table() const { return vst; }
+    __device__ __forceinline__ const uint16_t* table() const { return vst; }
     __device__ __forceinline__ uint32_t row(uint32_t a, uint32_t b) const {
         return ((a + 1 - G.a0) * G.RW + (b + 1 - G.b0)) * G.W1 - G.cl;
     }
@@ -222,7 +222,8 @@ struct WalkPlan {
     uint32_t lo0, hi0, lo1, hi1, c_lo, c_hi, step_b, step_a;
     // Moves w past exhausted rows, carrying the overshoot j - je into the next
     // row.  Returns false when the stencil ends.
-    __device__ __forceinline__ bool settle(const uint32_t* table, Walk& w) const {
+    template <typename TT>
+    __device__ __forceinline__ bool settle(const TT* table, Walk& w) const {
         while (w.j >= w.je) {
             const uint32_t over = w.j - w.je;
             const uint32_t rb1 = w.rb + 1;
@@ -292,7 +293,7 @@ __device__ __forceinline__ void agent_force(const Arrays& A, const StateBuf& S, 
     const double rcd = (double)cut_radius<POT, double>(P.pd);
     double thr_d = rcd * rcd * (1.0 + 1e-12);
     if (!(thr_d >= 0.0)) thr_d = __builtin_inf();
-    const uint32_t* table = src.table();
+    const auto* table = src.table();
     // the own voxel's candidates (same-voxel pairs)
     const uint32_t own_row = src.row(idx[0], idx[1]);
     const Span ov{table[own_row + idx[2]], table[own_row + idx[2] + 1]};
@@ -480,26 +481,49 @@ constexpr int kTileRowsMax = 29;
 constexpr int kTileTable = 3 * (kTileRowsMax + 2);
 constexpr int kLdsRecordsMixed = 1024;  // 16 KiB of int4
 constexpr int kLdsRecordsFP64 = 896;    // 28 KiB of double4
-constexpr int kLdsVoxelStarts = 768;    // 3 KiB of voxel-start indices
+constexpr int kLdsVoxelStarts = 768;    // 1.5 KiB of voxel-start indices
+constexpr int kSortBins = 64;           // candidate-count classes for the lane order
 
 struct ForceTile {
     uint32_t inst, a0, b0, b1, c0, c1;
 };
+
+// Inclusive prefix sum over the 64 lanes of a wave (all lanes active).
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    const uint32_t lane = threadIdx.x & 63u;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(v, o, 64);
+        if (lane >= (uint32_t)o) v += y;
+    }
+    return v;
+}
+
+// a / d for a < 2^14, 0 < d <= 64, with d's float reciprocal: (a + 1/2) / d keeps a
+// fractional part >= 1/128 that the two float roundings cannot carry across.
+__device__ __forceinline__ uint32_t small_div(uint32_t a, float inv_d) {
+    return (uint32_t)(((float)a + 0.5f) * inv_d);
+}
 
 // Workgroup per tile: stage the region's row segments (converted records) and
 // their voxel starts in LDS with coalesced loads, then one lane per own agent
 // sums its forces from LDS.  A tile whose region does not fit (local crowding)
 // reads from global memory instead; the arithmetic and its order are the same.
 template <int D, int POT, bool MIXED>
-__global__ __launch_bounds__(kBlock, MIXED ? 8 : 4) void k_force_tiled(Arrays A, StateBuf S, const ForceTile* __restrict__ tiles,
-                                                          uint32_t physical, uint32_t symmetric, uint32_t no_lds) {
+__global__ __launch_bounds__(kBlock, MIXED ? 8 : 4) void k_force_tiled(Arrays A, StateBuf S,
+                                                                     const ForceTile* __restrict__ tiles,
+                                                                     uint32_t physical, uint32_t symmetric,
+                                                                     uint32_t no_lds) {
     using Rec = typename RecOf<MIXED>::type;
     constexpr int CAP = MIXED ? kLdsRecordsMixed : kLdsRecordsFP64;
     __shared__ Rec lds[CAP];
-    __shared__ uint32_t vst[kLdsVoxelStarts];
+    __shared__ uint16_t vst[kLdsVoxelStarts];  // LDS indices (< 2^16)
+    __shared__ uint8_t ord[kBlock];            // own agents in lane order
+    __shared__ uint32_t bins[kSortBins];
     __shared__ uint32_t t_gb[kTileTable];
     __shared__ uint32_t t_off[kTileTable + 1];
     __shared__ uint32_t own_n[kTileRowsMax + 1];  // prefix counts of the own row segments
+    __shared__ uint32_t wave0_total;
     const Rec* __restrict__ rec = rec_array(A, std::integral_constant<bool, MIXED>());
     const ForceTile T = tiles[xcd_block(blockIdx.x, gridDim.x)];
     const InstParams& P = A.ip[T.inst];
@@ -520,58 +544,67 @@ __global__ __launch_bounds__(kBlock, MIXED ? 8 : 4) void k_force_tiled(Arrays A,
     float unit[3];
 #pragma unroll
     for (int k = 0; k < D; k++) unit[k] = MIXED ? ldexpf(P.qunit[k + 3 - D], G.shift) : 0.0f;
+    const float inv_w1 = 1.0f / (float)G.W1, inv_rw = 1.0f / (float)G.RW;
     const uint32_t R = T.b1 - T.b0;
-    const uint32_t n_rows = 3 * G.RW;
-    if (threadIdx.x < n_rows) {
-        const uint32_t pa = threadIdx.x / G.RW, rb = threadIdx.x % G.RW;
-        const int a = (int)T.a0 - 1 + (int)pa, b = (int)T.b0 - 1 + (int)rb;
-        uint32_t gb = 0, len = 0;
-        if (a >= 0 && a < (int)n0 && b >= 0 && b < (int)n1) {
-            const uint32_t base = voff + ((uint32_t)a * n1 + (uint32_t)b) * n2;
-            gb = A.start[base + G.cl];
-            len = A.start[base + G.ch] - gb;
+    const uint32_t n_rows = 3 * G.RW;  // <= kTileTable < 128
+    // Lengths of the staged rows (waves 0-1) and of the own row segments (wave
+    // 2), prefix-summed with wave scans.
+    uint32_t incl = 0;
+    if (threadIdx.x < 128u) {
+        const uint32_t r = threadIdx.x;
+        uint32_t len = 0;
+        if (r < n_rows) {
+            const uint32_t pa = small_div(r, inv_rw), rb = r - pa * G.RW;
+            const int a = (int)T.a0 - 1 + (int)pa, b = (int)T.b0 - 1 + (int)rb;
+            uint32_t gb = 0;
+            if (a >= 0 && a < (int)n0 && b >= 0 && b < (int)n1) {
+                const uint32_t base = voff + ((uint32_t)a * n1 + (uint32_t)b) * n2;
+                gb = A.start[base + G.cl];
+                len = A.start[base + G.ch] - gb;
+            }
+            t_gb[r] = gb;
         }
-        t_gb[threadIdx.x] = gb;
-        t_off[threadIdx.x + 1] = len;  // lengths; prefix-summed below
-    }
-    if (threadIdx.x < R) {
-        const uint32_t base = voff + (T.a0 * n1 + T.b0 + threadIdx.x) * n2;
-        own_n[threadIdx.x + 1] = A.start[base + T.c1] - A.start[base + T.c0];
-    }
-    if (threadIdx.x == 0) {
-        t_off[0] = 0;
-        own_n[0] = 0;
+        incl = wave_incl_scan(len);
+        if (threadIdx.x == 63u) wave0_total = incl;
+    } else if (threadIdx.x < 192u) {
+        const uint32_t r = threadIdx.x - 128u;
+        uint32_t len = 0;
+        if (r < R) {
+            const uint32_t base = voff + (T.a0 * n1 + T.b0 + r) * n2;
+            len = A.start[base + T.c1] - A.start[base + T.c0];
+        }
+        const uint32_t x = wave_incl_scan(len);
+        if (r < R) own_n[r + 1] = x;
+        if (r == 0) own_n[0] = 0;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        for (uint32_t r = 1; r <= n_rows; r++) t_off[r] += t_off[r - 1];
-        for (uint32_t r = 1; r <= R; r++) own_n[r] += own_n[r - 1];
+    if (threadIdx.x < 128u) {
+        if (threadIdx.x < n_rows) t_off[threadIdx.x + 1] = incl + (threadIdx.x >= 64u ? wave0_total : 0u);
+        if (threadIdx.x == 0) t_off[0] = 0;
     }
     __syncthreads();
     const uint32_t total = t_off[n_rows];
     const uint32_t n_own = own_n[R];
     const bool use_lds = !no_lds && total <= (uint32_t)CAP && n_rows * G.W1 <= (uint32_t)kLdsVoxelStarts;
-    // own agent t of the tile: row r of the tile, slot in the sorted order
-    auto own_slot = [&](uint32_t t, uint32_t& r) -> uint32_t {
-        uint32_t l = 0, h = R;  // largest l with own_n[l] <= t
+    // own row of own agent t (largest r with own_n[r] <= t)
+    auto own_row = [&](uint32_t t) -> uint32_t {
+        uint32_t l = 0, h = R;
         while (h - l > 1) {
             const uint32_t mid = (l + h) >> 1;
             if (own_n[mid] <= t) l = mid;
             else h = mid;
         }
-        r = l;
-        const uint32_t base = voff + (T.a0 * n1 + T.b0 + l) * n2;
-        return A.start[base + T.c0] + (t - own_n[l]);
+        return l;
     };
     if (use_lds) {
         for (uint32_t e = threadIdx.x; e < n_rows * G.W1; e += kBlock) {
-            const uint32_t r = e / G.W1, c = G.cl + e % G.W1;
-            const uint32_t pa = r / G.RW, rb = r % G.RW;
+            const uint32_t r = small_div(e, inv_w1), c = G.cl + (e - r * G.W1);
+            const uint32_t pa = small_div(r, inv_rw), rb = r - pa * G.RW;
             const int a = (int)T.a0 - 1 + (int)pa, b = (int)T.b0 - 1 + (int)rb;
             uint32_t x = t_off[r];
             if (a >= 0 && a < (int)n0 && b >= 0 && b < (int)n1)
                 x = A.start[voff + ((uint32_t)a * n1 + (uint32_t)b) * n2 + c] - t_gb[r] + t_off[r];
-            vst[e] = x;
+            vst[e] = (uint16_t)x;
         }
         for (uint32_t i = threadIdx.x; i < total; i += kBlock) {
             uint32_t l = 0, h = n_rows;  // largest l with t_off[l] <= i
@@ -581,24 +614,78 @@ __global__ __launch_bounds__(kBlock, MIXED ? 8 : 4) void k_force_tiled(Arrays A,
                 else h = mid;
             }
             const Rec r = rec[t_gb[l] + (i - t_off[l])];
-            if constexpr (MIXED) lds[i] = to_region<D>(r, (int)(l / G.RW), (int)(l % G.RW), G);
-            else lds[i] = r;
+            if constexpr (MIXED) {
+                const uint32_t pa = small_div(l, inv_rw);
+                lds[i] = to_region<D>(r, (int)pa, (int)(l - pa * G.RW), G);
+            } else {
+                lds[i] = r;
+            }
         }
         __syncthreads();
         LdsSource<D, MIXED> src{lds, vst, t_gb, t_off, G};
-        for (uint32_t t = threadIdx.x; t < n_own; t += kBlock) {
-            uint32_t r;
-            const uint32_t s = own_slot(t, r);
+        // own agent t: its LDS index, global slot and voxel column, from the tables
+        auto own_agent = [&](uint32_t t, uint32_t& r, uint32_t& self, uint32_t& c) -> uint32_t {
+            r = own_row(t);
             const uint32_t rt = G.RW + 1 + r;  // staged row of own row r (plane offset 1)
-            const uint32_t idx[3] = {T.a0, T.b0 + r, S.vox[s] - (voff + (T.a0 * n1 + T.b0 + r) * n2)};
-            agent_force<D, POT, MIXED>(A, S, P, s, s - t_gb[rt] + t_off[rt], idx, physical, symmetric, src, unit);
+            const uint32_t row = rt * G.W1 - G.cl;  // vst[row + c]: column c (modular index)
+            self = (uint32_t)vst[row + T.c0] + (t - own_n[r]);
+            uint32_t lo = T.c0, hi = T.c1;  // largest c with vst[row + c] <= self
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if ((uint32_t)vst[row + mid] <= self) lo = mid;
+                else hi = mid;
+            }
+            c = lo;
+            return t_gb[rt] + (self - t_off[rt]);
+        };
+        // Lanes take the own agents in descending order of candidate count, so
+        // that a wave's lanes walk streams of similar length (a wave runs as long
+        // as its longest stream).  Which lane computes an agent does not change
+        // the agent's sums.
+        const bool sorted = n_own <= (uint32_t)kBlock;
+        if (sorted) {
+            if (threadIdx.x < kSortBins) bins[threadIdx.x] = 0;
+            __syncthreads();
+            uint32_t key = 0, kr = 0;
+            if (threadIdx.x < n_own) {
+                uint32_t r, self, c;
+                own_agent(threadIdx.x, r, self, c);
+                const uint32_t b = T.b0 + r;
+                const uint32_t a_lo = T.a0 == 0 ? 0 : T.a0 - 1, a_hi = min(T.a0 + 2, n0);
+                const uint32_t b_lo = b == 0 ? 0 : b - 1, b_hi = min(b + 2, n1);
+                const uint32_t c_lo = c == 0 ? 0 : c - 1, c_hi = min(c + 2, n2);
+                uint32_t L = 0;
+                for (uint32_t a = a_lo; a < a_hi; a++)
+                    for (uint32_t bb = b_lo; bb < b_hi; bb++) {
+                        const uint32_t row = src.row(a, bb);
+                        L += (uint32_t)vst[row + c_hi] - (uint32_t)vst[row + c_lo];
+                    }
+                key = (uint32_t)(kSortBins - 1) - min(L >> 1, (uint32_t)(kSortBins - 1));
+                kr = atomicAdd(&bins[key], 1u);
+            }
+            __syncthreads();
+            if (threadIdx.x < kSortBins) {  // exclusive scan of the class sizes (wave 0)
+                const uint32_t v = bins[threadIdx.x];
+                bins[threadIdx.x] = wave_incl_scan(v) - v;
+            }
+            __syncthreads();
+            if (threadIdx.x < n_own) ord[bins[key] + kr] = (uint8_t)threadIdx.x;
+            __syncthreads();
+        }
+        for (uint32_t t0 = threadIdx.x; t0 < n_own; t0 += kBlock) {
+            const uint32_t t = sorted ? (uint32_t)ord[t0] : t0;
+            uint32_t r, self, c;
+            const uint32_t s = own_agent(t, r, self, c);
+            const uint32_t idx[3] = {T.a0, T.b0 + r, c};
+            agent_force<D, POT, MIXED>(A, S, P, s, self, idx, physical, symmetric, src, unit);
         }
     } else {
         GlobalSource<D, MIXED> src{rec, A.start, voff, n1, n2, G};
         for (uint32_t t = threadIdx.x; t < n_own; t += kBlock) {
-            uint32_t r;
-            const uint32_t s = own_slot(t, r);
-            const uint32_t idx[3] = {T.a0, T.b0 + r, S.vox[s] - (voff + (T.a0 * n1 + T.b0 + r) * n2)};
+            const uint32_t r = own_row(t);
+            const uint32_t base = voff + (T.a0 * n1 + T.b0 + r) * n2;
+            const uint32_t s = A.start[base + T.c0] + (t - own_n[r]);
+            const uint32_t idx[3] = {T.a0, T.b0 + r, S.vox[s] - base};
             agent_force<D, POT, MIXED>(A, S, P, s, s, idx, physical, symmetric, src, unit);
         }
     }
