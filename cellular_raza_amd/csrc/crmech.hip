@@ -548,13 +548,16 @@ crm_status reset_control(crm_ctx* ctx, uint64_t step) {
 // Force tiles of one instance: plane a, R rows x W columns of voxels each, the
 // grid split evenly.  The shape keeps the expected own agents within one pass of
 // the workgroup (lanes busy), the expected staged region (x1.2 for density
-// fluctuations) within the LDS records, the voxel-start table within its LDS
-// array and the region extent within 31 voxels (>= 26 fixed-point bits per
-// voxel); among those it takes the most own agents, then the smallest region.
-void build_tiles(const InstParams& P, uint32_t inst, uint64_t n_agents, uint32_t cap, std::vector<ForceTile>& out) {
+// fluctuations) within the LDS records, the staged voxels and the own voxels
+// within their LDS tables and the region extent within 31 voxels (>= 26
+// fixed-point bits per voxel); among those it takes the most own agents, then
+// the smallest region.
+void build_tiles(const InstParams& P, uint32_t dim, uint32_t inst, uint64_t n_agents, uint32_t cap,
+                 std::vector<ForceTile>& out) {
     const uint32_t n0 = P.n[0], n1 = P.n[1], n2 = P.n[2];
     const double rho = (double)n_agents / ((double)n0 * n1 * n2);
-    const double planes = n0 > 1 ? 3.0 : 1.0;
+    const uint32_t np = dim == 3 ? 3u : 1u;  // staged planes
+    const double planes = (double)std::min(np, n0 > 1 ? 3u : 1u);
     const double margin = 1.2, own_max = 0.94 * kBlock;
     uint32_t bR = 1, bW = 1;
     double best_own = -1.0, best_staged = 0.0;
@@ -566,7 +569,9 @@ void build_tiles(const InstParams& P, uint32_t inst, uint64_t n_agents, uint32_t
             const double own = (double)R * W * rho;
             const double staged = planes * (R + 2) * (W + 2) * rho * margin;
             if (own > own_max || staged > cap) continue;
-            if (3u * (R + 2) * (cols + 1) > (uint32_t)kLdsVoxelStarts || std::max(R + 2, cols) > 31u) continue;
+            if (np * (R + 2) * cols + 1 > (uint32_t)kLdsVoxelStarts || R * W > (uint32_t)kOwnVoxMax ||
+                std::max(R + 2, cols) > 31u)
+                continue;
             if (own > best_own * (1.0 + 1e-9) || (own >= best_own * (1.0 - 1e-9) && staged < best_staged)) {
                 best_own = own;
                 best_staged = staged;
@@ -809,7 +814,7 @@ crm_status crm_set_agents(crm_ctx* ctx, const uint64_t* counts, const double* po
     ctx->n_work = (uint32_t)work.size();
     std::vector<ForceTile> tiles;
     const uint32_t cap = ctx->cfg.precision == CRM_PRECISION_MIXED ? kLdsRecordsMixed : kLdsRecordsFP64;
-    for (uint32_t i = 0; i < ctx->n_inst; i++) build_tiles(ctx->hparams[i], i, counts[i], cap, tiles);
+    for (uint32_t i = 0; i < ctx->n_inst; i++) build_tiles(ctx->hparams[i], ctx->D, i, counts[i], cap, tiles);
     ctx->n_force_tiles = (uint32_t)tiles.size();
     CRM_HIP(hipMemcpy(ctx->d_ip, ctx->hparams.data(), sizeof(InstParams) * ctx->n_inst, hipMemcpyHostToDevice));
 

@@ -120,12 +120,13 @@ struct RecOf<true> {
 __device__ __forceinline__ const double4* rec_array(const Arrays& A, std::false_type) { return A.rec64; }
 __device__ __forceinline__ const int4* rec_array(const Arrays& A, std::true_type) { return A.recq; }
 
-// A tile's stencil region.  Staged row r = pa * RW + rb holds plane a0 - 1 + pa,
-// row b0 - 1 + rb (empty where that lies outside the grid), columns [cl, ch).
+// A tile's stencil region: rows b0 - 1 .. b1 (RW of them, empty where outside the
+// grid), columns [cl, ch) (NC of them) and, in 3-D, planes a0 - 1 .. a0 + 1 (NP = 3;
+// NP = 1 in 2-D).
 struct Region {
     uint32_t a0, b0;  // tile plane and first row
     uint32_t cl, ch;  // staged columns
-    uint32_t RW, W1;  // staged rows per plane (tile rows + 2), voxel-start entries per row (ch - cl + 1)
+    uint32_t RW, NC;  // staged rows (tile rows + 2), staged columns (ch - cl)
     int bits, shift;  // tile fixed point: bits per voxel, 30 - bits
 };
 
@@ -146,54 +147,104 @@ struct Span {
     uint32_t b, e;
 };
 
-// Candidate sources.  Within a stencil row (a, b) the voxels are consecutive in
-// the sorted order, so the candidates of voxels [c_lo, c_hi) of that row are one
-// contiguous index range.
-template <int D, bool MIXED>
-struct GlobalSource {
-    using Rec = typename RecOf<MIXED>::type;
-    const Rec* rec;
-    const uint32_t* start;
-    uint32_t voff, n1, n2;
-    Region G;
-    // row handle of stencil row (a, b): index of its column 0 in the table;
-    // b + 1 adds sb, a + 1 adds sa
-    __device__ __forceinline__ const uint32_t* table() const { return start; }
-    __device__ __forceinline__ uint32_t row(uint32_t a, uint32_t b) const { return voff + (a * n1 + b) * n2; }
-    __device__ __forceinline__ uint32_t sb() const { return n2; }
-    __device__ __forceinline__ uint32_t sa() const { return n1 * n2; }
-    __device__ __forceinline__ Rec record(uint32_t j, uint32_t a, uint32_t b) const {
-        if constexpr (MIXED) return to_region<D>(rec[j], (int)(a + 1 - G.a0), (int)(b + 1 - G.b0), G);
-        else return rec[j];
+// Candidate order of an agent at (a, b, c): stencil rows b' ascending, then
+// columns c' ascending, then planes a' ascending, then slot -- i.e. the 3^D voxels
+// with the row outermost.  Both sources below produce this order.
+//
+// LDS source: the region is staged voxel by voxel in that same (row, column,
+// plane) order, so the candidates of one stencil row (3 columns x all planes)
+// are one contiguous LDS range and an agent's stream is three ranges.
+struct Walk3 {
+    uint32_t j, je;          // current range [., je)
+    uint32_t j1, e1, j2, e2; // the ranges after it
+    uint32_t left;           // how many of those remain
+    // Moves past exhausted ranges, carrying the overshoot; false at the end.
+    __device__ __forceinline__ bool settle() {
+        while (j >= je) {
+            if (left == 0) return false;
+            const uint32_t over = j - je;
+            j = j1 + over;
+            je = e1;
+            j1 = j2;
+            e1 = e2;
+            left--;
+        }
+        return true;
     }
-    __device__ __forceinline__ uint32_t slot(uint32_t j) const { return j; }
 };
 
 template <int D, bool MIXED>
 struct LdsSource {
     using Rec = typename RecOf<MIXED>::type;
+    using W = Walk3;
     const Rec* lds;       // staged records, converted
-    const uint16_t* vst;  // per staged row: LDS index of the first agent of columns cl .. ch
-    const uint32_t* gb;   // per staged row: global slot of its first agent
-    const uint32_t* off;  // per staged row: its first LDS index
+    const uint16_t* vst;  // LDS index of the first agent of staged voxel v; vst[n_vox] = total
     Region G;
-    __device__ __forceinline__ const uint16_t* table() const { return vst; }
-    __device__ __forceinline__ uint32_t row(uint32_t a, uint32_t b) const {
-        return ((a + 1 - G.a0) * G.RW + (b + 1 - G.b0)) * G.W1 - G.cl;
+    uint32_t np, n_vox;
+    const uint32_t* start;
+    uint32_t voff, n1, n2;
+    // staged voxel index of (row rb, column c, plane pa)
+    __device__ __forceinline__ uint32_t vox(uint32_t rb, uint32_t c, uint32_t pa) const {
+        return (rb * G.NC + (c - G.cl)) * np + pa;
     }
-    __device__ __forceinline__ uint32_t sb() const { return G.W1; }
-    __device__ __forceinline__ uint32_t sa() const { return G.RW * G.W1; }
-    __device__ __forceinline__ Rec record(uint32_t j, uint32_t, uint32_t) const { return lds[j]; }
+    __device__ __forceinline__ Rec record(uint32_t j, const W&) const { return lds[j]; }
     // global slot of LDS index j (only on the rare FP64 re-evaluation path)
-    __device__ __forceinline__ uint32_t slot(uint32_t j) const {
-        uint32_t lo = 0, hi = 3 * G.RW;
+    __device__ __forceinline__ uint32_t slot(uint32_t j, const W&) const {
+        uint32_t lo = 0, hi = n_vox;  // largest v with vst[v] <= j
         while (hi - lo > 1) {
             const uint32_t mid = (lo + hi) >> 1;
-            if (off[mid] <= j) lo = mid;
+            if ((uint32_t)vst[mid] <= j) lo = mid;
             else hi = mid;
         }
-        return gb[lo] + (j - off[lo]);
+        const uint32_t pa = lo % np, t = lo / np, c = G.cl + t % G.NC, rb = t / G.NC;
+        const uint32_t a = G.a0 + pa - (np == 3 ? 1u : 0u), b = G.b0 - 1 + rb;
+        return start[voff + (a * n1 + b) * n2 + c] + (j - (uint32_t)vst[lo]);
     }
+};
+
+// Global source (tiles whose region does not fit in LDS): the stream walks the
+// stencil voxel by voxel in the same order.
+struct WalkVox {
+    uint32_t j, je;
+    uint32_t rb, cc, pa;     // current voxel: row b_lo + rb, column c_lo + cc, plane a_lo + pa
+    uint32_t nb, ncol, na;   // stencil extent (clipped)
+    uint32_t a_lo, b_lo, c_lo;
+    const uint32_t* start;
+    uint32_t voff, n1, n2;
+    __device__ __forceinline__ uint32_t gvox() const {
+        return voff + ((a_lo + pa) * n1 + (b_lo + rb)) * n2 + c_lo + cc;
+    }
+    __device__ __forceinline__ bool settle() {
+        while (j >= je) {
+            const uint32_t over = j - je;
+            if (++pa == na) {
+                pa = 0;
+                if (++cc == ncol) {
+                    cc = 0;
+                    if (++rb == nb) return false;
+                }
+            }
+            const uint32_t g = gvox();
+            j = start[g] + over;
+            je = start[g + 1];
+        }
+        return true;
+    }
+};
+
+template <int D, bool MIXED>
+struct GlobalSource {
+    using Rec = typename RecOf<MIXED>::type;
+    using W = WalkVox;
+    const Rec* rec;
+    Region G;
+    __device__ __forceinline__ Rec record(uint32_t j, const W& w) const {
+        if constexpr (MIXED)
+            return to_region<D>(rec[j], (int)(w.a_lo + w.pa + 1 - G.a0), (int)(w.b_lo + w.rb + 1 - G.b0), G);
+        else
+            return rec[j];
+    }
+    __device__ __forceinline__ uint32_t slot(uint32_t j, const W&) const { return j; }
 };
 
 // Strength row of the species matrix in four scalars (a register array indexed
@@ -210,38 +261,9 @@ struct Row4 {
     }
 };
 
-// Position in an agent's flattened candidate stream: stencil row (ra, rb) with
-// row handle `row`, index j inside its range [., je).
-struct Walk {
-    uint32_t ra, rb, row, j, je;
-};
-
-// Per-agent constants of the walk: stencil bounds, columns [c_lo, c_hi) of each
-// row and the handle steps (next row in the plane, first row of the next plane).
-struct WalkPlan {
-    uint32_t lo0, hi0, lo1, hi1, c_lo, c_hi, step_b, step_a;
-    // Moves w past exhausted rows, carrying the overshoot j - je into the next
-    // row.  Returns false when the stencil ends.
-    template <typename TT>
-    __device__ __forceinline__ bool settle(const TT* table, Walk& w) const {
-        while (w.j >= w.je) {
-            const uint32_t over = w.j - w.je;
-            const uint32_t rb1 = w.rb + 1;
-            const bool wrap = rb1 == hi1;
-            w.rb = wrap ? lo1 : rb1;
-            w.ra += wrap ? 1u : 0u;
-            if (w.ra == hi0) return false;
-            w.row += wrap ? step_a : step_b;
-            w.j = table[w.row + c_lo] + over;
-            w.je = table[w.row + c_hi];
-        }
-        return true;
-    }
-};
-
-// Force on the agent in slot s (index `self` in the source's index space).
-// Candidate order: stencil rows (a, b) ascending, then index ascending, which is
-// voxel-ascending then slot-ascending -- the same for every source.
+// Force on the agent in slot s: `me` is its record in the source's coordinates,
+// `self` its index in the source's index space, `ov` its own voxel's range there
+// (same-voxel pairs) and `w0` the start of its candidate stream.
 //
 // Two passes per chunk of 64 stream positions: a cheap squared-distance test
 // marks the candidates that can be inside the cutoff in a 64-bit mask, then the
@@ -252,23 +274,18 @@ struct WalkPlan {
 // instead of max-over-lanes of the candidate count.
 template <int D, int POT, bool MIXED, typename Src>
 __device__ __forceinline__ void agent_force(const Arrays& A, const StateBuf& S, const InstParams& P, uint64_t s,
-                                            uint32_t self, const uint32_t* idx, uint32_t physical,
-                                            uint32_t symmetric, const Src& src, const float* unit) {
+                                            uint32_t self, const typename RecOf<MIXED>::type me, Span ov,
+                                            typename Src::W cur, uint32_t physical, uint32_t symmetric,
+                                            const Src& src, const float* unit) {
     using T = typename std::conditional<MIXED, float, double>::type;
     using Rec = typename RecOf<MIXED>::type;
+    using W = typename Src::W;
     const uint64_t N = A.N;
-    uint32_t lo[3], hi[3];
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-        lo[a] = idx[a] == 0 ? 0 : idx[a] - 1;
-        hi[a] = min(idx[a] + 2, P.n[a]);
-    }
     const PotParams<T>& PP = pot_of<T>(P);
     const T rc = cut_radius<POT, T>(PP);
     const float rc2 = (float)(rc * rc);
     const float band = 2e-6f * rc2;
 
-    const Rec me = src.record(self, idx[0], idx[1]);
     int sp_i;
     bool flag_i = false;
     if constexpr (MIXED) {
@@ -279,9 +296,9 @@ __device__ __forceinline__ void agent_force(const Arrays& A, const StateBuf& S, 
     }
     // candidate test thresholds: mixed marks d2 <= rc2 + band (the band pairs are
     // re-decided in FP64) and every pair with a flagged agent; FP64 marks
-    // d2 <= rc^2 (1 + 1e-12), beyond which the rounded sqrt(d2) exceeds rc
-    // pass 1 measures in units of the first axis' fixed-point step (the other axes
-    // scaled by their step ratio) and keeps half a band more margin
+    // d2 <= rc^2 (1 + 1e-12), beyond which the rounded sqrt(d2) exceeds rc.
+    // Mixed pass 1 measures in units of the first axis' fixed-point step (the
+    // other axes scaled by their step ratio) and keeps half a band more margin.
     float r1 = 1.0f, r2 = 1.0f, thr_f = 0.0f;
     if constexpr (MIXED) {
         const float u0 = unit[0];
@@ -293,19 +310,6 @@ __device__ __forceinline__ void agent_force(const Arrays& A, const StateBuf& S, 
     const double rcd = (double)cut_radius<POT, double>(P.pd);
     double thr_d = rcd * rcd * (1.0 + 1e-12);
     if (!(thr_d >= 0.0)) thr_d = __builtin_inf();
-    const auto* table = src.table();
-    // the own voxel's candidates (same-voxel pairs)
-    const uint32_t own_row = src.row(idx[0], idx[1]);
-    const Span ov{table[own_row + idx[2]], table[own_row + idx[2] + 1]};
-    WalkPlan wp;
-    wp.lo0 = lo[0];
-    wp.hi0 = hi[0];
-    wp.lo1 = lo[1];
-    wp.hi1 = hi[1];
-    wp.c_lo = lo[2];
-    wp.c_hi = hi[2];
-    wp.step_b = src.sb();
-    wp.step_a = src.sa() - (hi[1] - lo[1] - 1) * src.sb();
 
     // species-matrix row (own species) and column (own species as ext)
     Row4<T> srow, scol;
@@ -359,9 +363,9 @@ __device__ __forceinline__ void agent_force(const Arrays& A, const StateBuf& S, 
     };
 
     // The pair with the candidate at walk position w, weighted as in the reference.
-    auto evaluate = [&](const Walk& w) {
+    auto evaluate = [&](const W& w) {
         const uint32_t j = w.j;
-        const Rec rj = src.record(j, w.ra, w.rb);
+        const Rec rj = src.record(j, w);
         const bool full = physical || (j >= ov.b && j < ov.e);
         T z[D];
         const T d2 = displacement(rj, z);
@@ -371,7 +375,7 @@ __device__ __forceinline__ void agent_force(const Arrays& A, const StateBuf& S, 
             sp_j = (int)((uint32_t)rj.w & kSpeciesMask);
             const bool refine = flag_i || rj.w < 0 || d2 == 0.0f || fabsf(d2 - rc2) <= band;
             if (refine) {
-                const uint32_t gj = src.slot(j);
+                const uint32_t gj = src.slot(j, w);
                 double zd[D], dd = 0.0;
 #pragma unroll
                 for (int k = 0; k < D; k++) {
@@ -410,22 +414,16 @@ __device__ __forceinline__ void agent_force(const Arrays& A, const StateBuf& S, 
         }
     };
 
-    Walk cur;
-    cur.ra = lo[0];
-    cur.rb = lo[1];
-    cur.row = src.row(lo[0], lo[1]);
-    cur.j = table[cur.row + lo[2]];
-    cur.je = table[cur.row + hi[2]];
     bool live = POT != CRM_POTENTIAL_NONE;
-    if (live) live = wp.settle(table, cur);
+    if (live) live = cur.settle();
     while (live) {
-        const Walk c0 = cur;
+        const W c0 = cur;
         // pass 1: mark the candidates that can be inside the cutoff (bit p of
         // mlo / mhi for stream position p of the chunk)
         uint32_t mlo = 0, mhi = 0;
         for (uint32_t p = 0; p < 64u; p++) {
             const uint32_t j = cur.j;
-            const Rec rj = src.record(j, cur.ra, cur.rb);
+            const Rec rj = src.record(j, cur);
             bool mark;
             if constexpr (MIXED) {
                 const float c0 = (float)(me.x - rj.x);
@@ -450,11 +448,11 @@ __device__ __forceinline__ void agent_force(const Arrays& A, const StateBuf& S, 
             if (p < 32u) mlo = mark ? (mlo | bit) : mlo;
             else mhi = mark ? (mhi | bit) : mhi;
             ++cur.j;
-            live = wp.settle(table, cur);
+            live = cur.settle();
             if (!live) break;
         }
         // pass 2: replay the chunk and evaluate the marked pairs in stream order
-        Walk w = c0;
+        W w = c0;
         uint32_t pos = 0;
         while (mlo | mhi) {
             uint32_t p;
@@ -467,7 +465,7 @@ __device__ __forceinline__ void agent_force(const Arrays& A, const StateBuf& S, 
             }
             w.j += p - pos;
             pos = p;
-            wp.settle(table, w);
+            w.settle();
             evaluate(w);
         }
     }
@@ -478,10 +476,10 @@ __device__ __forceinline__ void agent_force(const Arrays& A, const StateBuf& S, 
 
 // A force tile: plane a0, rows [b0, b1), columns [c0, c1) of one instance.
 constexpr int kTileRowsMax = 29;
-constexpr int kTileTable = 3 * (kTileRowsMax + 2);
 constexpr int kLdsRecordsMixed = 1024;  // 16 KiB of int4
 constexpr int kLdsRecordsFP64 = 896;    // 28 KiB of double4
-constexpr int kLdsVoxelStarts = 768;    // 1.5 KiB of voxel-start indices
+constexpr int kLdsVoxelStarts = 768;    // 1.5 KiB of voxel-start indices (staged voxels + 1)
+constexpr int kOwnVoxMax = 256;         // own voxels per tile
 constexpr int kSortBins = 64;           // candidate-count classes for the lane order
 
 struct ForceTile {
@@ -499,16 +497,35 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
     return v;
 }
 
+// Exclusive prefix sum over the workgroup; total = the sum of all x.
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t x, uint32_t& total) {
+    __shared__ uint32_t wave_tot[kBlock / 64];
+    const int wave = threadIdx.x >> 6;
+    const uint32_t inc = wave_incl_scan(x);
+    if ((threadIdx.x & 63) == 63) wave_tot[wave] = inc;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+#pragma unroll
+    for (int q = 0; q < kBlock / 64; q++) {
+        const uint32_t wt = wave_tot[q];
+        if (q < wave) before += wt;
+        all += wt;
+    }
+    __syncthreads();
+    total = all;
+    return before + inc - x;
+}
+
 // a / d for a < 2^14, 0 < d <= 64, with d's float reciprocal: (a + 1/2) / d keeps a
 // fractional part >= 1/128 that the two float roundings cannot carry across.
 __device__ __forceinline__ uint32_t small_div(uint32_t a, float inv_d) {
     return (uint32_t)(((float)a + 0.5f) * inv_d);
 }
 
-// Workgroup per tile: stage the region's row segments (converted records) and
-// their voxel starts in LDS with coalesced loads, then one lane per own agent
-// sums its forces from LDS.  A tile whose region does not fit (local crowding)
-// reads from global memory instead; the arithmetic and its order are the same.
+// Workgroup per tile: stage the region voxel by voxel (converted records, voxel
+// starts) in LDS, then one lane per own agent sums its forces from LDS.  A tile
+// whose region does not fit (local crowding) reads global memory instead; the
+// candidate order and the arithmetic are the same.
 template <int D, int POT, bool MIXED>
 __global__ __launch_bounds__(kBlock, MIXED ? 8 : 4) void k_force_tiled(Arrays A, StateBuf S,
                                                                      const ForceTile* __restrict__ tiles,
@@ -516,14 +533,14 @@ __global__ __launch_bounds__(kBlock, MIXED ? 8 : 4) void k_force_tiled(Arrays A,
                                                                      uint32_t no_lds) {
     using Rec = typename RecOf<MIXED>::type;
     constexpr int CAP = MIXED ? kLdsRecordsMixed : kLdsRecordsFP64;
+    constexpr uint32_t NP = D == 3 ? 3u : 1u;         // staged planes
+    constexpr uint32_t PA_OWN = D == 3 ? 1u : 0u;     // the tile's own plane among them
     __shared__ Rec lds[CAP];
-    __shared__ uint16_t vst[kLdsVoxelStarts];  // LDS indices (< 2^16)
-    __shared__ uint8_t ord[kBlock];            // own agents in lane order
+    __shared__ uint16_t vst[kLdsVoxelStarts];        // LDS index of each staged voxel's first agent
+    __shared__ uint16_t own_pre[kOwnVoxMax + 1];     // own agents before own voxel u
+    __shared__ uint32_t own_gs[kOwnVoxMax];          // global slot of own voxel u's first agent
+    __shared__ uint8_t ord[kBlock];                  // own agents in lane order
     __shared__ uint32_t bins[kSortBins];
-    __shared__ uint32_t t_gb[kTileTable];
-    __shared__ uint32_t t_off[kTileTable + 1];
-    __shared__ uint32_t own_n[kTileRowsMax + 1];  // prefix counts of the own row segments
-    __shared__ uint32_t wave0_total;
     const Rec* __restrict__ rec = rec_array(A, std::integral_constant<bool, MIXED>());
     const ForceTile T = tiles[xcd_block(blockIdx.x, gridDim.x)];
     const InstParams& P = A.ip[T.inst];
@@ -534,9 +551,9 @@ __global__ __launch_bounds__(kBlock, MIXED ? 8 : 4) void k_force_tiled(Arrays A,
     G.RW = T.b1 - T.b0 + 2;
     G.cl = T.c0 == 0 ? 0u : T.c0 - 1u;
     G.ch = min(T.c1 + 1u, n2);
-    G.W1 = G.ch - G.cl + 1;
+    G.NC = G.ch - G.cl;
     {
-        const uint32_t ext = max(max(D == 3 ? 3u : 1u, G.RW), G.ch - G.cl);
+        const uint32_t ext = max(max(NP, G.RW), G.NC);
         const int lg = ext <= 1 ? 0 : 32 - __clz(ext - 1);  // ceil(log2(ext))
         G.bits = min(30, 31 - lg);
         G.shift = 30 - G.bits;
@@ -544,149 +561,164 @@ __global__ __launch_bounds__(kBlock, MIXED ? 8 : 4) void k_force_tiled(Arrays A,
     float unit[3];
 #pragma unroll
     for (int k = 0; k < D; k++) unit[k] = MIXED ? ldexpf(P.qunit[k + 3 - D], G.shift) : 0.0f;
-    const float inv_w1 = 1.0f / (float)G.W1, inv_rw = 1.0f / (float)G.RW;
-    const uint32_t R = T.b1 - T.b0;
-    const uint32_t n_rows = 3 * G.RW;  // <= kTileTable < 128
-    // Lengths of the staged rows (waves 0-1) and of the own row segments (wave
-    // 2), prefix-summed with wave scans.
-    uint32_t incl = 0;
-    if (threadIdx.x < 128u) {
-        const uint32_t r = threadIdx.x;
-        uint32_t len = 0;
-        if (r < n_rows) {
-            const uint32_t pa = small_div(r, inv_rw), rb = r - pa * G.RW;
-            const int a = (int)T.a0 - 1 + (int)pa, b = (int)T.b0 - 1 + (int)rb;
-            uint32_t gb = 0;
-            if (a >= 0 && a < (int)n0 && b >= 0 && b < (int)n1) {
-                const uint32_t base = voff + ((uint32_t)a * n1 + (uint32_t)b) * n2;
-                gb = A.start[base + G.cl];
-                len = A.start[base + G.ch] - gb;
+    const uint32_t R = T.b1 - T.b0, W = T.c1 - T.c0;
+    const uint32_t n_vox = G.RW * G.NC * NP;
+    const uint32_t n_ownv = R * W;
+    const bool fits = !no_lds && n_vox < (uint32_t)kLdsVoxelStarts && n_ownv <= (uint32_t)kOwnVoxMax;
+
+    if (fits) {
+        // staged voxel v = (rb * NC + c - cl) * NP + pa; thread t holds v = 3t .. 3t + 2
+        const float inv_np = 1.0f / (float)NP, inv_nc = 1.0f / (float)G.NC;
+        uint32_t cnt[3], gs[3], sum = 0;
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const uint32_t v = 3 * threadIdx.x + q;
+            cnt[q] = 0;
+            gs[q] = 0;
+            if (v < n_vox) {
+                const uint32_t t = small_div(v, inv_np), pa = v - t * NP;
+                const uint32_t rb = small_div(t, inv_nc), c = G.cl + (t - rb * G.NC);
+                const int a = (int)T.a0 + (int)pa - (int)PA_OWN, b = (int)T.b0 - 1 + (int)rb;
+                if (a >= 0 && a < (int)n0 && b >= 0 && b < (int)n1) {
+                    const uint32_t g = voff + ((uint32_t)a * n1 + (uint32_t)b) * n2 + c;
+                    gs[q] = A.start[g];
+                    cnt[q] = A.start[g + 1] - gs[q];
+                }
             }
-            t_gb[r] = gb;
+            sum += cnt[q];
         }
-        incl = wave_incl_scan(len);
-        if (threadIdx.x == 63u) wave0_total = incl;
-    } else if (threadIdx.x < 192u) {
-        const uint32_t r = threadIdx.x - 128u;
-        uint32_t len = 0;
-        if (r < R) {
-            const uint32_t base = voff + (T.a0 * n1 + T.b0 + r) * n2;
-            len = A.start[base + T.c1] - A.start[base + T.c0];
-        }
-        const uint32_t x = wave_incl_scan(len);
-        if (r < R) own_n[r + 1] = x;
-        if (r == 0) own_n[0] = 0;
-    }
-    __syncthreads();
-    if (threadIdx.x < 128u) {
-        if (threadIdx.x < n_rows) t_off[threadIdx.x + 1] = incl + (threadIdx.x >= 64u ? wave0_total : 0u);
-        if (threadIdx.x == 0) t_off[0] = 0;
-    }
-    __syncthreads();
-    const uint32_t total = t_off[n_rows];
-    const uint32_t n_own = own_n[R];
-    const bool use_lds = !no_lds && total <= (uint32_t)CAP && n_rows * G.W1 <= (uint32_t)kLdsVoxelStarts;
-    // own row of own agent t (largest r with own_n[r] <= t)
-    auto own_row = [&](uint32_t t) -> uint32_t {
-        uint32_t l = 0, h = R;
-        while (h - l > 1) {
-            const uint32_t mid = (l + h) >> 1;
-            if (own_n[mid] <= t) l = mid;
-            else h = mid;
-        }
-        return l;
-    };
-    if (use_lds) {
-        for (uint32_t e = threadIdx.x; e < n_rows * G.W1; e += kBlock) {
-            const uint32_t r = small_div(e, inv_w1), c = G.cl + (e - r * G.W1);
-            const uint32_t pa = small_div(r, inv_rw), rb = r - pa * G.RW;
-            const int a = (int)T.a0 - 1 + (int)pa, b = (int)T.b0 - 1 + (int)rb;
-            uint32_t x = t_off[r];
-            if (a >= 0 && a < (int)n0 && b >= 0 && b < (int)n1)
-                x = A.start[voff + ((uint32_t)a * n1 + (uint32_t)b) * n2 + c] - t_gb[r] + t_off[r];
-            vst[e] = (uint16_t)x;
-        }
-        for (uint32_t i = threadIdx.x; i < total; i += kBlock) {
-            uint32_t l = 0, h = n_rows;  // largest l with t_off[l] <= i
-            while (h - l > 1) {
-                const uint32_t mid = (l + h) >> 1;
-                if (t_off[mid] <= i) l = mid;
-                else h = mid;
-            }
-            const Rec r = rec[t_gb[l] + (i - t_off[l])];
-            if constexpr (MIXED) {
-                const uint32_t pa = small_div(l, inv_rw);
-                lds[i] = to_region<D>(r, (int)pa, (int)(l - pa * G.RW), G);
-            } else {
-                lds[i] = r;
-            }
-        }
-        __syncthreads();
-        LdsSource<D, MIXED> src{lds, vst, t_gb, t_off, G};
-        // own agent t: its LDS index, global slot and voxel column, from the tables
-        auto own_agent = [&](uint32_t t, uint32_t& r, uint32_t& self, uint32_t& c) -> uint32_t {
-            r = own_row(t);
-            const uint32_t rt = G.RW + 1 + r;  // staged row of own row r (plane offset 1)
-            const uint32_t row = rt * G.W1 - G.cl;  // vst[row + c]: column c (modular index)
-            self = (uint32_t)vst[row + T.c0] + (t - own_n[r]);
-            uint32_t lo = T.c0, hi = T.c1;  // largest c with vst[row + c] <= self
-            while (hi - lo > 1) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if ((uint32_t)vst[row + mid] <= self) lo = mid;
-                else hi = mid;
-            }
-            c = lo;
-            return t_gb[rt] + (self - t_off[rt]);
-        };
-        // Lanes take the own agents in descending order of candidate count, so
-        // that a wave's lanes walk streams of similar length (a wave runs as long
-        // as its longest stream).  Which lane computes an agent does not change
-        // the agent's sums.
-        const bool sorted = n_own <= (uint32_t)kBlock;
-        if (sorted) {
-            if (threadIdx.x < kSortBins) bins[threadIdx.x] = 0;
-            __syncthreads();
-            uint32_t key = 0, kr = 0;
-            if (threadIdx.x < n_own) {
-                uint32_t r, self, c;
-                own_agent(threadIdx.x, r, self, c);
-                const uint32_t b = T.b0 + r;
-                const uint32_t a_lo = T.a0 == 0 ? 0 : T.a0 - 1, a_hi = min(T.a0 + 2, n0);
-                const uint32_t b_lo = b == 0 ? 0 : b - 1, b_hi = min(b + 2, n1);
-                const uint32_t c_lo = c == 0 ? 0 : c - 1, c_hi = min(c + 2, n2);
-                uint32_t L = 0;
-                for (uint32_t a = a_lo; a < a_hi; a++)
-                    for (uint32_t bb = b_lo; bb < b_hi; bb++) {
-                        const uint32_t row = src.row(a, bb);
-                        L += (uint32_t)vst[row + c_hi] - (uint32_t)vst[row + c_lo];
+        uint32_t total;
+        uint32_t run = block_exclusive_scan(sum, total);
+        const bool use_lds = total <= (uint32_t)CAP;  // uniform
+        if (use_lds) {
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                const uint32_t v = 3 * threadIdx.x + q;
+                if (v < n_vox) {
+                    vst[v] = (uint16_t)run;
+                    const uint32_t t = small_div(v, inv_np), pa = v - t * NP;
+                    const uint32_t rb = small_div(t, inv_nc), c = G.cl + (t - rb * G.NC);
+                    for (uint32_t k = 0; k < cnt[q]; k++) {
+                        const Rec r = rec[gs[q] + k];
+                        if constexpr (MIXED) lds[run + k] = to_region<D>(r, (int)pa, (int)rb, G);
+                        else lds[run + k] = r;
                     }
-                key = (uint32_t)(kSortBins - 1) - min(L >> 1, (uint32_t)(kSortBins - 1));
-                kr = atomicAdd(&bins[key], 1u);
+                    // own voxel u = r * W + (c - c0): its count and first global slot
+                    if (pa == PA_OWN && rb >= 1 && rb <= R && c >= T.c0 && c < T.c1) {
+                        const uint32_t u = (rb - 1) * W + (c - T.c0);
+                        own_pre[u + 1] = (uint16_t)cnt[q];
+                        own_gs[u] = gs[q];
+                    }
+                    run += cnt[q];
+                }
+            }
+            if (threadIdx.x == 0) {
+                vst[n_vox] = (uint16_t)total;
+                own_pre[0] = 0;
             }
             __syncthreads();
-            if (threadIdx.x < kSortBins) {  // exclusive scan of the class sizes (wave 0)
-                const uint32_t v = bins[threadIdx.x];
-                bins[threadIdx.x] = wave_incl_scan(v) - v;
+            {  // own_pre: exclusive prefix of the own voxel counts (n_ownv <= 256)
+                const uint32_t u = threadIdx.x;
+                const uint32_t x = u < n_ownv ? (uint32_t)own_pre[u + 1] : 0u;
+                uint32_t tot;
+                const uint32_t ex = block_exclusive_scan(x, tot);
+                if (u < n_ownv) own_pre[u + 1] = (uint16_t)(ex + x);
             }
             __syncthreads();
-            if (threadIdx.x < n_own) ord[bins[key] + kr] = (uint8_t)threadIdx.x;
-            __syncthreads();
+            const uint32_t n_own = own_pre[n_ownv];
+            LdsSource<D, MIXED> src{lds, vst, G, NP, n_vox, A.start, voff, n1, n2};
+            const float inv_w = 1.0f / (float)W;
+            // own agent t: own voxel u (largest u with own_pre[u] <= t), LDS index,
+            // global slot, and the three ranges of its stream
+            auto own_agent = [&](uint32_t t, uint32_t& self, Span& ov, Walk3& w) -> uint32_t {
+                uint32_t lo = 0, hi = n_ownv;
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if ((uint32_t)own_pre[mid] <= t) lo = mid;
+                    else hi = mid;
+                }
+                const uint32_t k = t - (uint32_t)own_pre[lo];
+                const uint32_t r = small_div(lo, inv_w), c = T.c0 + (lo - r * W);
+                const uint32_t vo = src.vox(r + 1, c, PA_OWN);
+                self = (uint32_t)vst[vo] + k;
+                ov = Span{vst[vo], vst[vo + 1]};
+                const uint32_t c_lo = c == 0 ? 0 : c - 1, c_hi = min(c + 2, n2);
+                // stencil rows r, r + 1, r + 2 of the region (b - 1, b, b + 1; rows
+                // outside the grid are staged empty)
+                w.j = vst[src.vox(r, c_lo, 0)];
+                w.je = vst[src.vox(r, c_hi, 0)];
+                w.j1 = vst[src.vox(r + 1, c_lo, 0)];
+                w.e1 = vst[src.vox(r + 1, c_hi, 0)];
+                w.j2 = vst[src.vox(r + 2, c_lo, 0)];
+                w.e2 = vst[src.vox(r + 2, c_hi, 0)];
+                w.left = 2;
+                return own_gs[lo] + k;
+            };
+            // Lanes take the own agents in descending order of candidate count, so
+            // that a wave's lanes walk streams of similar length (a wave runs as
+            // long as its longest stream).  Which lane computes an agent does not
+            // change the agent's sums.
+            const bool sorted = n_own <= (uint32_t)kBlock;
+            if (sorted) {
+                if (threadIdx.x < kSortBins) bins[threadIdx.x] = 0;
+                __syncthreads();
+                uint32_t key = 0, kr = 0;
+                if (threadIdx.x < n_own) {
+                    uint32_t self;
+                    Span ov;
+                    Walk3 w;
+                    own_agent(threadIdx.x, self, ov, w);
+                    const uint32_t L = (w.je - w.j) + (w.e1 - w.j1) + (w.e2 - w.j2);
+                    key = (uint32_t)(kSortBins - 1) - min(L >> 1, (uint32_t)(kSortBins - 1));
+                    kr = atomicAdd(&bins[key], 1u);
+                }
+                __syncthreads();
+                if (threadIdx.x < kSortBins) {  // exclusive scan of the class sizes (wave 0)
+                    const uint32_t v = bins[threadIdx.x];
+                    bins[threadIdx.x] = wave_incl_scan(v) - v;
+                }
+                __syncthreads();
+                if (threadIdx.x < n_own) ord[bins[key] + kr] = (uint8_t)threadIdx.x;
+                __syncthreads();
+            }
+            for (uint32_t t0 = threadIdx.x; t0 < n_own; t0 += kBlock) {
+                const uint32_t t = sorted ? (uint32_t)ord[t0] : t0;
+                uint32_t self;
+                Span ov;
+                Walk3 w;
+                const uint32_t s = own_agent(t, self, ov, w);
+                agent_force<D, POT, MIXED>(A, S, P, s, self, lds[self], ov, w, physical, symmetric, src, unit);
+            }
+            return;
         }
-        for (uint32_t t0 = threadIdx.x; t0 < n_own; t0 += kBlock) {
-            const uint32_t t = sorted ? (uint32_t)ord[t0] : t0;
-            uint32_t r, self, c;
-            const uint32_t s = own_agent(t, r, self, c);
-            const uint32_t idx[3] = {T.a0, T.b0 + r, c};
-            agent_force<D, POT, MIXED>(A, S, P, s, self, idx, physical, symmetric, src, unit);
-        }
-    } else {
-        GlobalSource<D, MIXED> src{rec, A.start, voff, n1, n2, G};
-        for (uint32_t t = threadIdx.x; t < n_own; t += kBlock) {
-            const uint32_t r = own_row(t);
-            const uint32_t base = voff + (T.a0 * n1 + T.b0 + r) * n2;
-            const uint32_t s = A.start[base + T.c0] + (t - own_n[r]);
-            const uint32_t idx[3] = {T.a0, T.b0 + r, S.vox[s] - base};
-            agent_force<D, POT, MIXED>(A, S, P, s, s, idx, physical, symmetric, src, unit);
+    }
+    // global path: own agents row by row, candidates voxel by voxel
+    GlobalSource<D, MIXED> src{rec, G};
+    for (uint32_t r = 0; r < R; r++) {
+        const uint32_t b = T.b0 + r, base = voff + (T.a0 * n1 + b) * n2;
+        const uint32_t sb = A.start[base + T.c0], se = A.start[base + T.c1];
+        for (uint32_t s = sb + threadIdx.x; s < se; s += kBlock) {
+            const uint32_t c = S.vox[s] - base;
+            WalkVox w;
+            w.start = A.start;
+            w.voff = voff;
+            w.n1 = n1;
+            w.n2 = n2;
+            w.a_lo = T.a0 == 0 || NP == 1 ? T.a0 : T.a0 - 1;
+            w.b_lo = b == 0 ? 0 : b - 1;
+            w.c_lo = c == 0 ? 0 : c - 1;
+            w.na = (NP == 1 ? T.a0 + 1 : min(T.a0 + 2, n0)) - w.a_lo;
+            w.nb = min(b + 2, n1) - w.b_lo;
+            w.ncol = min(c + 2, n2) - w.c_lo;
+            w.rb = w.cc = w.pa = 0;
+            const uint32_t g0 = w.gvox();
+            w.j = A.start[g0];
+            w.je = A.start[g0 + 1];
+            const Span ov{A.start[base + c], A.start[base + c + 1]};
+            WalkVox wm = w;  // the own record: its voxel (a0, b, c) in region coordinates
+            wm.pa = T.a0 - w.a_lo;
+            wm.rb = b - w.b_lo;
+            agent_force<D, POT, MIXED>(A, S, P, s, s, src.record(s, wm), ov, w, physical, symmetric, src, unit);
         }
     }
 }
@@ -950,28 +982,6 @@ __global__ __launch_bounds__(kBlock) void k_initial_key(Arrays A, StateBuf S, ui
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kBlock * kScanItems;
 
-__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t x, uint32_t& total) {
-    __shared__ uint32_t wave_tot[kBlock / 64];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint32_t inc = x;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        uint32_t y = __shfl_up(inc, d, 64);
-        if (lane >= d) inc += y;
-    }
-    if (lane == 63) wave_tot[wave] = inc;
-    __syncthreads();
-    uint32_t before = 0, all = 0;
-#pragma unroll
-    for (int q = 0; q < kBlock / 64; q++) {
-        uint32_t wt = wave_tot[q];
-        if (q < wave) before += wt;
-        all += wt;
-    }
-    __syncthreads();
-    total = all;
-    return before + inc - x;
-}
 
 __global__ __launch_bounds__(kBlock) void k_scan_local(const uint32_t* __restrict__ count,
                                                         const uint32_t* __restrict__ stay,
