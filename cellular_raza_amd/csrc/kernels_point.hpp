@@ -83,12 +83,11 @@ __device__ __forceinline__ const PotParams<float>& pot_of<float>(const InstParam
 // pair loop they are re-expressed relative to the tile region's origin with
 // `bits` bits per voxel, bits = 31 - ceil(log2(region extent in voxels)), so that
 // every coordinate and every difference is an exact int32: the displacement of
-// two agents is one integer subtraction and one rounding to float, accurate to
-// ~1e-7 of the distance however close the pair is.  A pair whose float distance
-// lies in a narrow band around the cutoff radius (the forces jump there), a zero
-// distance, or an agent flagged as outside its voxel's fixed-point range, is
-// re-evaluated from the FP64 positions so that the cutoff decision is the
-// reference's.
+// two agents is one integer subtraction and one rounding to float.  A pair whose
+// float distance lies in a narrow band around the cutoff radius (the forces jump
+// there), a pair closer than 2e6 fixed-point steps (where the step would show in
+// the direction), or an agent flagged as outside its voxel's fixed-point range,
+// is re-evaluated from the FP64 positions, with the reference's cutoff decision.
 // ---------------------------------------------------------------------------
 
 // blockIdx -> work block, so that each XCD (blocks are dealt round-robin over 8)
@@ -299,13 +298,17 @@ __device__ __forceinline__ void agent_force(const Arrays& A, const StateBuf& S, 
     // d2 <= rc^2 (1 + 1e-12), beyond which the rounded sqrt(d2) exceeds rc.
     // Mixed pass 1 measures in units of the first axis' fixed-point step (the
     // other axes scaled by their step ratio) and keeps half a band more margin.
-    float r1 = 1.0f, r2 = 1.0f, thr_f = 0.0f;
+    float r1 = 1.0f, r2 = 1.0f, thr_f = 0.0f, close2 = 0.0f;
     if constexpr (MIXED) {
         const float u0 = unit[0];
         r1 = unit[1] / u0;
         r2 = unit[D - 1] / u0;
         thr_f = (rc2 + 1.5f * band) / (u0 * u0);
         if (!(thr_f >= 0.0f) || flag_i) thr_f = __builtin_inff();
+        // pairs closer than 2e6 fixed-point steps (where the quantised direction
+        // would be off by more than 1e-6) take their displacement from FP64
+        const float umax = fmaxf(fmaxf(unit[0], unit[1]), unit[D - 1]);
+        close2 = (2e6f * umax) * (2e6f * umax);
     }
     const double rcd = (double)cut_radius<POT, double>(P.pd);
     double thr_d = rcd * rcd * (1.0 + 1e-12);
@@ -373,7 +376,7 @@ __device__ __forceinline__ void agent_force(const Arrays& A, const StateBuf& S, 
         int cut = -1;
         if constexpr (MIXED) {
             sp_j = (int)((uint32_t)rj.w & kSpeciesMask);
-            const bool refine = flag_i || rj.w < 0 || d2 == 0.0f || fabsf(d2 - rc2) <= band;
+            const bool refine = flag_i || rj.w < 0 || d2 < close2 || fabsf(d2 - rc2) <= band;
             if (refine) {
                 const uint32_t gj = src.slot(j, w);
                 double zd[D], dd = 0.0;
