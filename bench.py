@@ -12,7 +12,7 @@ FP64 path, the force error of the mixed path against FP64, the force kernel's
 roofline figures and the CPU baseline (the oracle's chili restatement, threaded)
 are reported beside it.  `--workload` selects the other configurations:
   c1  2-D Morse NewtonDamped, 1000 agents, Euler
-  c3  2-D bacterial rods, 200,000 rods x 8 vertices, AB3 (FP64)
+  c3  2-D bacterial rods, 200,000 rods x 8 vertices, AB3
   c4  512-instance 2-D Morse ensemble in one context, plus the same instances
       run one after another (one context each)
   c5  3-D Mie lattice, 4,000,000 agents, AB3
@@ -172,8 +172,8 @@ def workload_table():
                "k_force_tiled<3, 3, true>"),
         "c1": (lambda a: W.c1_morse_2d(n=a.n or 1000, seed=1, steps=a.steps), True,
                "2-D Morse NewtonDamped, N=1000 in 250x250, Euler", "k_force_tiled<2, 0, true>"),
-        "c3": (lambda a: W.c3_rods_2d(n=a.n or 200_000, seed=3, steps=a.steps), False,
-               "2-D bacterial rods, 200,000 rods x 8 vertices in 2000x2000, AB3", "k_force_rod<2, 0>"),
+        "c3": (lambda a: W.c3_rods_2d(n=a.n or 200_000, seed=3, steps=a.steps), True,
+               "2-D bacterial rods, 200,000 rods x 8 vertices in 2000x2000, AB3", "k_force_rod<2, 0, true>"),
         "c4": (lambda a: W.c4_ensemble_2d(n_instances=a.n or 512, seed=4, steps=a.steps), True,
                "512-instance 2-D Morse ensemble (500..20000 agents each), Euler, one context",
                "k_force_tiled<2, 0, true>"),

@@ -158,6 +158,7 @@ struct crm_ctx {
     double* d_noise = nullptr;
     size_t noise_cap = 0;
     double* d_recrod = nullptr;  // rods: AoS vertex positions [N][K]
+    double4* d_bnd = nullptr;    // rods: bounding balls [N]
     uint32_t rods_per_block = 0;
 
     uint64_t N = 0;
@@ -182,7 +183,7 @@ thread_local std::string g_create_error;
 
 void free_device(crm_ctx* ctx) {
     void* ptrs[] = {ctx->d_work, ctx->d_tiles, ctx->d_start, ctx->d_count, ctx->d_bsum, ctx->d_key, ctx->d_rank,
-                    ctx->d_perm, ctx->d_force, ctx->d_rec64, ctx->d_recq, ctx->d_noise, ctx->d_recrod};
+                    ctx->d_perm, ctx->d_force, ctx->d_rec64, ctx->d_recq, ctx->d_noise, ctx->d_recrod, ctx->d_bnd};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& b : ctx->buf) {
@@ -199,6 +200,7 @@ void free_device(crm_ctx* ctx) {
     ctx->d_recq = nullptr;
     ctx->d_noise = nullptr;
     ctx->d_recrod = nullptr;
+    ctx->d_bnd = nullptr;
     ctx->noise_cap = 0;
     for (auto& g : ctx->graph)
         if (g) {
@@ -234,6 +236,7 @@ bool is_rod(const crm_ctx* ctx) { return ctx->mech_kind == CRM_MECHANICS_ROD; }
 RodArgs rod_args(crm_ctx* ctx) {
     RodArgs R;
     R.rec = ctx->d_recrod;
+    R.bnd = ctx->d_bnd;
     R.nv = ctx->nv;
     R.rpb = ctx->rods_per_block;
     return R;
@@ -293,31 +296,38 @@ crm_status launch_force_d(crm_ctx* ctx, StateBuf S) {
     return CRM_INVALID_ARGUMENT;
 }
 
-template <int D>
-crm_status launch_force_rod_d(crm_ctx* ctx, StateBuf S) {
+template <int D, bool MIXED>
+crm_status launch_force_rod_dm(crm_ctx* ctx, StateBuf S) {
     Arrays A = arrays(ctx);
     const RodArgs R = rod_args(ctx);
     const uint32_t phys = ctx->cfg.weighting == CRM_WEIGHT_PHYSICAL;
     const dim3 g(ctx->n_work), b(kBlock);
     switch (ctx->pot_kind) {
         case CRM_POTENTIAL_MORSE:
-            hipLaunchKernelGGL((k_force_rod<D, CRM_POTENTIAL_MORSE>), g, b, 0, ctx->stream, A, S, R, phys);
+            hipLaunchKernelGGL((k_force_rod<D, CRM_POTENTIAL_MORSE, MIXED>), g, b, 0, ctx->stream, A, S, R, phys);
             return CRM_OK;
         case CRM_POTENTIAL_MORSE_SPECIES:
-            hipLaunchKernelGGL((k_force_rod<D, CRM_POTENTIAL_MORSE_SPECIES>), g, b, 0, ctx->stream, A, S, R, phys);
-            return CRM_OK;
-        case CRM_POTENTIAL_MIE:
-            hipLaunchKernelGGL((k_force_rod<D, CRM_POTENTIAL_MIE>), g, b, 0, ctx->stream, A, S, R, phys);
-            return CRM_OK;
-        case CRM_POTENTIAL_BOUND_LENNARD_JONES:
-            hipLaunchKernelGGL((k_force_rod<D, CRM_POTENTIAL_BOUND_LENNARD_JONES>), g, b, 0, ctx->stream, A, S, R,
+            hipLaunchKernelGGL((k_force_rod<D, CRM_POTENTIAL_MORSE_SPECIES, MIXED>), g, b, 0, ctx->stream, A, S, R,
                                phys);
             return CRM_OK;
+        case CRM_POTENTIAL_MIE:
+            hipLaunchKernelGGL((k_force_rod<D, CRM_POTENTIAL_MIE, MIXED>), g, b, 0, ctx->stream, A, S, R, phys);
+            return CRM_OK;
+        case CRM_POTENTIAL_BOUND_LENNARD_JONES:
+            hipLaunchKernelGGL((k_force_rod<D, CRM_POTENTIAL_BOUND_LENNARD_JONES, MIXED>), g, b, 0, ctx->stream, A, S,
+                               R, phys);
+            return CRM_OK;
         case CRM_POTENTIAL_NONE:
-            hipLaunchKernelGGL((k_force_rod<D, CRM_POTENTIAL_NONE>), g, b, 0, ctx->stream, A, S, R, phys);
+            hipLaunchKernelGGL((k_force_rod<D, CRM_POTENTIAL_NONE, MIXED>), g, b, 0, ctx->stream, A, S, R, phys);
             return CRM_OK;
     }
     return CRM_INVALID_ARGUMENT;
+}
+
+template <int D>
+crm_status launch_force_rod_d(crm_ctx* ctx, StateBuf S) {
+    return ctx->cfg.precision == CRM_PRECISION_MIXED ? launch_force_rod_dm<D, true>(ctx, S)
+                                                     : launch_force_rod_dm<D, false>(ctx, S);
 }
 
 crm_status launch_force(crm_ctx* ctx) {
@@ -689,8 +699,6 @@ crm_status crm_create(const crm_config* cfg, const crm_instance* instances, uint
         ctx->nv = instances[0].mechanics.n_vertices;
         if (ctx->nv < 2 || ctx->nv > CRM_MAX_ROD_VERTICES)
             return fail(CRM_INVALID_ARGUMENT, "rods need 2..16 vertices");
-        if (cfg->precision != CRM_PRECISION_FP64)
-            return fail(CRM_INVALID_ARGUMENT, "rods run in FP64 precision only");
         if (ctx->pot_kind == CRM_POTENTIAL_CELL_SORTING)
             return fail(CRM_INVALID_ARGUMENT, "rods take a point potential (Morse, Mie, BoundLennardJones)");
         ctx->rods_per_block = kBlock / ctx->nv;
@@ -828,7 +836,10 @@ crm_status crm_set_agents(crm_ctx* ctx, const uint64_t* counts, const double* po
     CRM_TRY(dalloc(ctx, ctx->d_rank, N));
     CRM_TRY(dalloc(ctx, ctx->d_perm, N));
     CRM_TRY(dalloc(ctx, ctx->d_force, (size_t)K * N));
-    if (is_rod(ctx)) CRM_TRY(dalloc(ctx, ctx->d_recrod, (size_t)K * N));
+    if (is_rod(ctx)) {
+        CRM_TRY(dalloc(ctx, ctx->d_recrod, (size_t)K * N));
+        CRM_TRY(dalloc(ctx, ctx->d_bnd, N));
+    }
     else if (ctx->cfg.precision == CRM_PRECISION_MIXED) CRM_TRY(dalloc(ctx, ctx->d_recq, N));
     else CRM_TRY(dalloc(ctx, ctx->d_rec64, N));
     for (auto& b : ctx->buf) {

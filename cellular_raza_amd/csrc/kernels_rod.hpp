@@ -21,6 +21,7 @@ namespace crm {
 
 struct RodArgs {
     double* rec;   // [N][K] AoS vertex positions (force records)
+    double4* bnd;  // [N] bounding ball of each rod: vertex mean (x, y, z), radius
     uint32_t nv;   // vertices per rod
     uint32_t rpb;  // rods per workgroup
 };
@@ -43,8 +44,16 @@ __device__ __forceinline__ bool rod_lane(const BlockWork& w, uint32_t nv, uint64
 //             (weight 1 - t), in the reference's (u, s) order.
 // Reference weights: same voxel 1/2 own + 1/2 ext; adjacent voxel 1/2 own
 // (physical: same-voxel weights everywhere).
+//
+// A rod's segments lie inside the ball around its vertex mean that holds its
+// vertices, and a segment inside the ball around its midpoint of half its
+// length: a point farther than (radius + cutoff) from such a ball has every
+// nearest point beyond the cutoff, where every term is exactly zero.  Pass 1
+// marks the (candidate, segment) pairs that can be inside the cutoff, pass 2
+// evaluates only those.  Own parts are summed per candidate in the reference's
+// order; the ext parts follow the own parts of all candidates.
 // ---------------------------------------------------------------------------
-template <int D, int POT>
+template <int D, int POT, bool MIXED>
 __global__ __launch_bounds__(kBlock) void k_force_rod(Arrays A, StateBuf S, RodArgs R, uint32_t physical) {
     const BlockWork w = A.work[blockIdx.x];
     uint64_t s;
@@ -52,7 +61,7 @@ __global__ __launch_bounds__(kBlock) void k_force_rod(Arrays A, StateBuf S, RodA
     if (!rod_lane(w, R.nv, s, v)) return;
     const InstParams& P = A.ip[w.inst];
     const PotParams<double>& PP = P.pd;
-    const uint32_t nv = R.nv, K = nv * D;
+    const uint32_t nv = R.nv, K = nv * D, nseg = nv - 1;
     const double* own = R.rec + s * K;
     double pv[D];
 #pragma unroll
@@ -73,64 +82,275 @@ __global__ __launch_bounds__(kBlock) void k_force_rod(Arrays A, StateBuf S, RodA
 #pragma unroll
     for (int k = 0; k < D; k++) acc[k] = 0.0;
     bool calc_err = false;
-    for (uint32_t a = lo[0]; a < hi[0]; a++) {
-        for (uint32_t b = lo[1]; b < hi[1]; b++) {
-            const uint32_t row = P.voxel_offset + (a * P.n[1] + b) * P.n[2];
-            const uint32_t jb = A.start[row + lo[2]], je = A.start[row + hi[2]];
-            const uint32_t cb = A.start[row + idx[2]], ce = A.start[row + idx[2] + 1];
-            const bool own_row = (a == idx[0]) && (b == idx[1]);
-            for (uint32_t j = jb; j < je; j++) {
-                if (j == s) continue;
-                const double* ext = R.rec + (uint64_t)j * K;
-                const int sp_j = S.species[j];
-                const bool same = sp_i == sp_j;
-                double s_own = PP.strength, s_ext = PP.strength;
-                if constexpr (POT == CRM_POTENTIAL_MORSE_SPECIES) {
-                    s_own = PP.strength * PP.species_matrix[sp_i * CRM_MAX_SPECIES + sp_j];
-                    s_ext = PP.strength * PP.species_matrix[sp_j * CRM_MAX_SPECIES + sp_i];
-                }
-                // own part: vertex v against the segments of rod j
-                double fo_sum[D];
+    // Reach tests in float from double differences: a ball of radius r around
+    // centre c is out of reach of point p when |p - c| > r + cutoff.  The margin
+    // (1e-5 absolute + 1e-6 relative) is far above the float rounding of
+    // distances within a stencil.
+    const double rcd = cut_radius<POT, double>(PP);
+    float reach = (float)rcd * (1.0f + 1e-6f) + 1e-5f;
+    if (!(reach >= 0.0f)) reach = __builtin_inff();
+    auto out_of_reach = [&](const double* p, const double* c, float radius) {
+        float d2 = 0.0f;
 #pragma unroll
-                for (int k = 0; k < D; k++) fo_sum[k] = 0.0;
-                for (uint32_t q = 0; q + 1 < nv; q++) {
-                    double x[D], t, z[D], f[D];
-                    nearest_point<double, D>(pv, ext + q * D, ext + (q + 1) * D, x, t);
+        for (int k = 0; k < D; k++) {
+            const float d = (float)(p[k] - c[k]);
+            d2 = fmaf(d, d, d2);
+        }
+        const float lim = radius + reach;
+        return d2 > lim * lim;
+    };
+    // segment (a, b): midpoint m and half length h (+ margin); zero-length
+    // segments get an infinite radius, so they are always evaluated (their
+    // nearest point is undefined, as in the reference)
+    auto seg_ball = [&](const double* a, const double* b, double* m) {
+        float h2 = 0.0f;
 #pragma unroll
-                    for (int k = 0; k < D; k++) z[k] = pv[k] - x[k];
-                    calc_err |= point_force<POT, double, D>(z, same, s_own, PP, -1, f);
+        for (int k = 0; k < D; k++) {
+            m[k] = (a[k] + b[k]) * 0.5;
+            const float d = (float)(b[k] - a[k]);
+            h2 = fmaf(d, d, h2);
+        }
+        bool zero = true;
 #pragma unroll
-                    for (int k = 0; k < D; k++) fo_sum[k] = fo_sum[k] + f[k];
-                }
+        for (int k = 0; k < D; k++) zero = zero && a[k] == b[k];
+        return zero ? __builtin_inff() : 0.5f * sqrtf(h2) * (1.0f + 1e-6f) + 1e-6f;
+    };
+    const double4 bi = R.bnd[s];
+    const double ci[3] = {bi.x, bi.y, bi.z};
+    const float ri = (float)bi.w * (1.0f + 1e-6f);
+    // the own voxel's rods (same-voxel pairs)
+    const uint32_t own_row = P.voxel_offset + (idx[0] * P.n[1] + idx[1]) * P.n[2];
+    const uint32_t cb = A.start[own_row + idx[2]], ce = A.start[own_row + idx[2] + 1];
+    // One vertex-segment evaluation: the force on point p from its nearest point
+    // on segment (a, b), and the segment parameter t of that point.  Mixed
+    // precision: the segment relative to p in float, the potential in float; a
+    // pair whose squared distance lies within 2e-6 (relative) of the cutoff
+    // radius's, a distance below a tenth of p's distance to the segment ends, or
+    // a zero-length segment is evaluated in FP64.
+    const float rc2f = (float)(rcd * rcd), bandf = 2e-6f * rc2f;
+    auto pair = [&](const double* p, const double* a, const double* b, bool same, double strength, double* f,
+                    double& t) {
+        if constexpr (MIXED) {
+            float av[D], ab[D], num = 0.0f, den = 0.0f, span = 0.0f;
 #pragma unroll
-                for (int k = 0; k < D; k++) acc[k] = acc[k] + fo_sum[k] * 0.5;
-                const bool full = physical || (own_row && j >= cb && j < ce);
-                if (full) {
-                    // ext part: row v of the ext-force matrix of calc_j(j, i)
-                    double fe_sum[D];
+            for (int k = 0; k < D; k++) {
+                av[k] = (float)(a[k] - p[k]);
+                const float bk = (float)(b[k] - p[k]);
+                ab[k] = bk - av[k];
+                num = num - av[k] * ab[k];
+                den = den + ab[k] * ab[k];
+                span = span + av[k] * av[k] + bk * bk;
+            }
+            float tt = num / den;
+            tt = tt < 0.0f ? 0.0f : (tt > 1.0f ? 1.0f : tt);
+            float z[D], d2 = 0.0f;
 #pragma unroll
-                    for (int k = 0; k < D; k++) fe_sum[k] = 0.0;
-                    for (uint32_t u = 0; u < nv; u++) {
-                        const double* pu = ext + u * D;
-                        for (int side = 0; side < 2; side++) {
-                            // side 0: segment (v-1, v), v is its second end (weight t)
-                            // side 1: segment (v, v+1), v is its first end (weight 1 - t)
-                            const int q = (int)v - 1 + side;
-                            if (q < 0 || q + 1 >= (int)nv) continue;
-                            double x[D], t, z[D], f[D];
-                            nearest_point<double, D>(pu, own + q * D, own + (q + 1) * D, x, t);
+            for (int k = 0; k < D; k++) {
+                z[k] = -(av[k] + ab[k] * tt);
+                d2 = d2 + z[k] * z[k];
+            }
+            // (a distance below a tenth of the endpoints' would carry their float
+            // rounding into the direction at more than ~1e-6)
+            if (den > 0.0f && d2 > 1e-2f * span && fabsf(d2 - rc2f) > bandf) {
+                float ff[D];
+                const bool e = point_force<POT, float, D>(z, same, (float)strength, P.pf, -1, ff);
 #pragma unroll
-                            for (int k = 0; k < D; k++) z[k] = pu[k] - x[k];
-                            calc_err |= point_force<POT, double, D>(z, same, s_ext, PP, -1, f);
-                            const double wgt = side == 0 ? t : 1.0 - t;
+                for (int k = 0; k < D; k++) f[k] = (double)ff[k];
+                t = (double)tt;
+                return e;
+            }
+        }
+        double x[D], z[D];
+        nearest_point<double, D>(p, a, b, x, t);
 #pragma unroll
-                            for (int k = 0; k < D; k++) fe_sum[k] = fe_sum[k] + (-f[k]) * wgt;
-                        }
+        for (int k = 0; k < D; k++) z[k] = p[k] - x[k];
+        return point_force<POT, double, D>(z, same, strength, PP, -1, f);
+    };
+    // own segments (v - 1, v) and (v, v + 1): balls for the ext part
+    double m0[D], m1[D];
+    float h0 = -1.0f, h1 = -1.0f;  // negative: no such segment
+    if (v >= 1) h0 = seg_ball(own + (v - 1) * D, own + v * D, m0);
+    if (v + 1 < nv) h1 = seg_ball(own + v * D, own + (v + 1) * D, m1);
+
+    struct RodWalk {
+        uint32_t ra, rb, j, je;
+    };
+    // candidates: stencil rows (ra, rb) ascending, each the rods of columns [lo2, hi2)
+    auto settle = [&](RodWalk& wk) {
+        while (wk.j >= wk.je) {
+            const uint32_t over = wk.j - wk.je;
+            if (++wk.rb == hi[1]) {
+                wk.rb = lo[1];
+                if (++wk.ra == hi[0]) return false;
+            }
+            const uint32_t row = P.voxel_offset + (wk.ra * P.n[1] + wk.rb) * P.n[2];
+            wk.j = A.start[row + lo[2]] + over;
+            wk.je = A.start[row + hi[2]];
+        }
+        return true;
+    };
+    auto first = [&](RodWalk& wk) {
+        wk.ra = lo[0];
+        wk.rb = lo[1];
+        const uint32_t row = P.voxel_offset + (wk.ra * P.n[1] + wk.rb) * P.n[2];
+        wk.j = A.start[row + lo[2]];
+        wk.je = A.start[row + hi[2]];
+        return settle(wk);
+    };
+    auto strengths = [&](uint32_t j, bool& same, double& s_own, double& s_ext) {
+        const int sp_j = S.species[j];
+        same = sp_i == sp_j;
+        s_own = s_ext = PP.strength;
+        if constexpr (POT == CRM_POTENTIAL_MORSE_SPECIES) {
+            s_own = PP.strength * PP.species_matrix[sp_i * CRM_MAX_SPECIES + sp_j];
+            s_ext = PP.strength * PP.species_matrix[sp_j * CRM_MAX_SPECIES + sp_i];
+        }
+    };
+
+    // Own part, calc_i(i, j) row v: per candidate rod, the point force between
+    // vertex v and its nearest point on each segment, summed over segments.
+    // Chunks of `per` candidates; pass 1 sets bit (c * nseg + q) for each segment
+    // q of candidate c that can reach v, pass 2 evaluates the set bits in order.
+    const uint32_t per = 64u / nseg < 32u ? 64u / nseg : 32u;
+    RodWalk cur;
+    bool live = POT != CRM_POTENTIAL_NONE && first(cur);
+    while (live) {
+        const RodWalk c0 = cur;
+        uint64_t mo = 0;
+        for (uint32_t c = 0; c < per; c++) {
+            const uint32_t j = cur.j;
+            if (j != s) {
+                const double4 bj = R.bnd[j];
+                const double cj[3] = {bj.x, bj.y, bj.z};
+                if (!out_of_reach(pv, cj, (float)bj.w * (1.0f + 1e-6f))) {
+                    const double* ext = R.rec + (uint64_t)j * K;
+                    for (uint32_t q = 0; q < nseg; q++) {
+                        double m[D];
+                        const float h = seg_ball(ext + q * D, ext + (q + 1) * D, m);
+                        if (!out_of_reach(pv, m, h)) mo |= 1ull << (c * nseg + q);
                     }
-#pragma unroll
-                    for (int k = 0; k < D; k++) acc[k] = acc[k] + fe_sum[k] * 0.5;
                 }
             }
+            ++cur.j;
+            live = settle(cur);
+            if (!live) break;
+        }
+        // pass 2: bits in order; a candidate's segment sum goes to acc (x 1/2)
+        // when the next candidate starts
+        RodWalk wk = c0;
+        uint32_t pos = 0, cand = 0xffffffffu;
+        const double* ext = nullptr;
+        bool same = false;
+        double s_own = 0.0, s_ext = 0.0;
+        double fo_sum[D];
+#pragma unroll
+        for (int k = 0; k < D; k++) fo_sum[k] = 0.0;
+        while (mo) {
+            const uint32_t bit = (uint32_t)__builtin_ctzll(mo);
+            mo &= mo - 1;
+            const uint32_t c = bit / nseg, q = bit - c * nseg;
+            if (c != cand) {
+                if (cand != 0xffffffffu) {
+#pragma unroll
+                    for (int k = 0; k < D; k++) {
+                        acc[k] = acc[k] + fo_sum[k] * 0.5;
+                        fo_sum[k] = 0.0;
+                    }
+                }
+                cand = c;
+                wk.j += c - pos;
+                pos = c;
+                settle(wk);
+                ext = R.rec + (uint64_t)wk.j * K;
+                strengths(wk.j, same, s_own, s_ext);
+            }
+            double t, f[D];
+            calc_err |= pair(pv, ext + q * D, ext + (q + 1) * D, same, s_own, f, t);
+#pragma unroll
+            for (int k = 0; k < D; k++) fo_sum[k] = fo_sum[k] + f[k];
+        }
+        if (cand != 0xffffffffu) {
+#pragma unroll
+            for (int k = 0; k < D; k++) acc[k] = acc[k] + fo_sum[k] * 0.5;
+        }
+    }
+
+    // Ext part, calc_j(j, i) row v, for same-voxel candidates (every candidate in
+    // the physical weighting): for each vertex u of j, the force on the nearest
+    // point of i's segments (v - 1, v) (weight t) and (v, v + 1) (weight 1 - t).
+    // Bit (c * 2 nv + 2 u + side) per chunk of candidates.
+    const uint32_t per_e = 64u / (2u * nv) < 32u ? 64u / (2u * nv) : 32u;
+    RodWalk ew;
+    if (physical) {
+        live = POT != CRM_POTENTIAL_NONE && first(ew);
+    } else {
+        ew.ra = ew.rb = 0;
+        ew.j = cb;
+        ew.je = ce;
+        live = POT != CRM_POTENTIAL_NONE && ew.j < ew.je;
+    }
+    auto esettle = [&](RodWalk& wk) { return physical ? settle(wk) : wk.j < wk.je; };
+    while (live) {
+        const RodWalk c0 = ew;
+        uint64_t me = 0;
+        for (uint32_t c = 0; c < per_e; c++) {
+            const uint32_t j = ew.j;
+            if (j != s) {
+                const double4 bj = R.bnd[j];
+                const double cj[3] = {bj.x, bj.y, bj.z};
+                if (!out_of_reach(ci, cj, ri + (float)bj.w * (1.0f + 1e-6f))) {
+                    const double* ext = R.rec + (uint64_t)j * K;
+                    for (uint32_t u = 0; u < nv; u++) {
+                        const double* pu = ext + u * D;
+                        if (h0 >= 0.0f && !out_of_reach(pu, m0, h0)) me |= 1ull << (c * 2 * nv + 2 * u);
+                        if (h1 >= 0.0f && !out_of_reach(pu, m1, h1)) me |= 1ull << (c * 2 * nv + 2 * u + 1);
+                    }
+                }
+            }
+            ++ew.j;
+            live = esettle(ew);
+            if (!live) break;
+        }
+        RodWalk wk = c0;
+        uint32_t pos = 0, cand = 0xffffffffu;
+        const double* ext = nullptr;
+        bool same = false;
+        double s_own = 0.0, s_ext = 0.0;
+        double fe_sum[D];
+#pragma unroll
+        for (int k = 0; k < D; k++) fe_sum[k] = 0.0;
+        while (me) {
+            const uint32_t bit = (uint32_t)__builtin_ctzll(me);
+            me &= me - 1;
+            const uint32_t c = bit / (2 * nv), rem = bit - c * 2 * nv, u = rem >> 1, side = rem & 1u;
+            if (c != cand) {
+                if (cand != 0xffffffffu) {
+#pragma unroll
+                    for (int k = 0; k < D; k++) {
+                        acc[k] = acc[k] + fe_sum[k] * 0.5;
+                        fe_sum[k] = 0.0;
+                    }
+                }
+                cand = c;
+                wk.j += c - pos;
+                pos = c;
+                if (physical) settle(wk);
+                ext = R.rec + (uint64_t)wk.j * K;
+                strengths(wk.j, same, s_own, s_ext);
+            }
+            // side 0: segment (v - 1, v), v is its second end (weight t)
+            // side 1: segment (v, v + 1), v is its first end (weight 1 - t)
+            const int q = (int)v - 1 + (int)side;
+            const double* pu = ext + u * D;
+            double t, f[D];
+            calc_err |= pair(pu, own + q * D, own + (q + 1) * D, same, s_ext, f, t);
+            const double wgt = side == 0 ? t : 1.0 - t;
+#pragma unroll
+            for (int k = 0; k < D; k++) fe_sum[k] = fe_sum[k] + (-f[k]) * wgt;
+        }
+        if (cand != 0xffffffffu) {
+#pragma unroll
+            for (int k = 0; k < D; k++) acc[k] = acc[k] + fe_sum[k] * 0.5;
         }
     }
     if (calc_err) raise_error(A.ctrl, CRM_CALC_ERROR, A.ctrl->step, S.id[s]);
@@ -385,73 +605,112 @@ __global__ __launch_bounds__(kBlock) void k_key_rod(Arrays A, StateBuf S, RodArg
 template <int D, bool INTEGRATE>
 __global__ __launch_bounds__(kBlock) void k_gather_rod(Arrays A, StateBuf src, StateBuf dst, RodArgs R,
                                                          StepArgs sa) {
+    __shared__ double newpos[kRodLdsK];
     const BlockWork w = A.work[blockIdx.x];
     uint64_t s;
     uint32_t v;
-    if (!rod_lane(w, R.nv, s, v)) return;
+    const bool active = rod_lane(w, R.nv, s, v);
     const InstParams& P = A.ip[w.inst];
     const uint64_t N = A.N;
     const uint32_t nv = R.nv, K = nv * D;
-    const uint32_t o = A.perm[s];
-    const uint32_t id = src.id[o];
     double nx[D], nvel[D];
-    if (INTEGRATE) {
-        const uint64_t t = A.ctrl->step - 1;
-        const Ring rg = ring_at(t, sa.cap, sa.newest_first);
-        auto pos = [&](int i, int k) { return src.pos[(uint64_t)(i * D + k) * N + o]; };
-        auto vel = [&](int i, int k) { return src.vel[(uint64_t)(i * D + k) * N + o]; };
-        double x[D], vv[D], f[D], dx[D], dv[D], dxr[D];
+    if (active) {
+        const uint32_t o = A.perm[s];
+        const uint32_t id = src.id[o];
+        if (INTEGRATE) {
+            const uint64_t t = A.ctrl->step - 1;
+            const Ring rg = ring_at(t, sa.cap, sa.newest_first);
+            auto pos = [&](int i, int k) { return src.pos[(uint64_t)(i * D + k) * N + o]; };
+            auto vel = [&](int i, int k) { return src.vel[(uint64_t)(i * D + k) * N + o]; };
+            double x[D], vv[D], f[D], dx[D], dv[D], dxr[D];
 #pragma unroll
-        for (int k = 0; k < D; k++) {
-            x[k] = pos((int)v, k);
-            vv[k] = vel((int)v, k);
-            f[k] = A.force[(uint64_t)(v * D + k) * N + o];
-        }
-        rod_row_increment<D>(P, nv, v, pos, vel, f, dx, dv);
-        rod_row_noise<D>(P, sa, id, t, nv, v, dxr);
-        auto hist = [&](int i, int k, bool is_dv) {
-            const double* h = is_dv ? src.hdv : src.hdx;
-            return h[((uint64_t)rg.it[i] * K + v * D + k) * N + o];
-        };
-        rod_row_solve<D, true>(sa, rg, x, vv, dx, dv, dxr, hist, nx, nvel);
-        rod_row_reflect<D, true>(P, nx, nvel);
-        if (rg.cap > 0) {
-            const uint32_t n_hist = (uint32_t)min<uint64_t>(t + 1, rg.cap);
+            for (int k = 0; k < D; k++) {
+                x[k] = pos((int)v, k);
+                vv[k] = vel((int)v, k);
+                f[k] = A.force[(uint64_t)(v * D + k) * N + o];
+            }
+            rod_row_increment<D>(P, nv, v, pos, vel, f, dx, dv);
+            rod_row_noise<D>(P, sa, id, t, nv, v, dxr);
+            auto hist = [&](int i, int k, bool is_dv) {
+                const double* h = is_dv ? src.hdv : src.hdx;
+                return h[((uint64_t)rg.it[i] * K + v * D + k) * N + o];
+            };
+            rod_row_solve<D, true>(sa, rg, x, vv, dx, dv, dxr, hist, nx, nvel);
+            rod_row_reflect<D, true>(P, nx, nvel);
+            if (rg.cap > 0) {
+                const uint32_t n_hist = (uint32_t)min<uint64_t>(t + 1, rg.cap);
+                for (uint32_t h = 0; h < n_hist; h++) {
+#pragma unroll
+                    for (int k = 0; k < D; k++) {
+                        const uint64_t e = ((uint64_t)h * K + v * D + k) * N;
+                        dst.hdx[e + s] = h == rg.wslot ? dx[k] : src.hdx[e + o];
+                        dst.hdv[e + s] = h == rg.wslot ? dv[k] : src.hdv[e + o];
+                    }
+                }
+            }
+        } else {
+            const uint32_t n_hist = (uint32_t)min<uint64_t>(A.ctrl->step, sa.cap);
+#pragma unroll
+            for (int k = 0; k < D; k++) {
+                nx[k] = src.pos[(uint64_t)(v * D + k) * N + o];
+                nvel[k] = src.vel[(uint64_t)(v * D + k) * N + o];
+            }
             for (uint32_t h = 0; h < n_hist; h++) {
 #pragma unroll
                 for (int k = 0; k < D; k++) {
                     const uint64_t e = ((uint64_t)h * K + v * D + k) * N;
-                    dst.hdx[e + s] = h == rg.wslot ? dx[k] : src.hdx[e + o];
-                    dst.hdv[e + s] = h == rg.wslot ? dv[k] : src.hdv[e + o];
+                    dst.hdx[e + s] = src.hdx[e + o];
+                    dst.hdv[e + s] = src.hdv[e + o];
                 }
             }
         }
-    } else {
-        const uint32_t n_hist = (uint32_t)min<uint64_t>(A.ctrl->step, sa.cap);
 #pragma unroll
         for (int k = 0; k < D; k++) {
-            nx[k] = src.pos[(uint64_t)(v * D + k) * N + o];
-            nvel[k] = src.vel[(uint64_t)(v * D + k) * N + o];
+            dst.pos[(uint64_t)(v * D + k) * N + s] = nx[k];
+            dst.vel[(uint64_t)(v * D + k) * N + s] = nvel[k];
+            R.rec[s * K + v * D + k] = nx[k];
+            newpos[threadIdx.x * 3 + k] = nx[k];
         }
-        for (uint32_t h = 0; h < n_hist; h++) {
+        if (v == 0) {
+            dst.species[s] = src.species[o];
+            dst.id[s] = id;
+            dst.vox[s] = A.key[o];
+        }
+    }
+    __syncthreads();
+    // bounding ball of the rod (first lane): vertex mean and the largest distance to it
+    if (active && v == 0) {
+        double c[3] = {0.0, 0.0, 0.0};
+        for (uint32_t q = 0; q < nv; q++) {
+#pragma unroll
+            for (int k = 0; k < D; k++) c[k] = c[k] + newpos[(threadIdx.x + q) * 3 + k];
+        }
+#pragma unroll
+        for (int k = 0; k < D; k++) c[k] = c[k] / (double)nv;
+        double r2 = 0.0;
+        for (uint32_t q = 0; q < nv; q++) {
+            double d2 = 0.0;
 #pragma unroll
             for (int k = 0; k < D; k++) {
-                const uint64_t e = ((uint64_t)h * K + v * D + k) * N;
-                dst.hdx[e + s] = src.hdx[e + o];
-                dst.hdv[e + s] = src.hdv[e + o];
+                const double d = newpos[(threadIdx.x + q) * 3 + k] - c[k];
+                d2 = d2 + d * d;
             }
+            r2 = fmax(r2, d2);
         }
-    }
+        // a zero-length segment gets an infinite radius: such a rod is never
+        // skipped, so its undefined nearest points reach the sums as in the
+        // reference
+        bool degenerate = false;
+        for (uint32_t q = 0; q + 1 < nv; q++) {
+            bool same = true;
 #pragma unroll
-    for (int k = 0; k < D; k++) {
-        dst.pos[(uint64_t)(v * D + k) * N + s] = nx[k];
-        dst.vel[(uint64_t)(v * D + k) * N + s] = nvel[k];
-        R.rec[s * K + v * D + k] = nx[k];
-    }
-    if (v == 0) {
-        dst.species[s] = src.species[o];
-        dst.id[s] = id;
-        dst.vox[s] = A.key[o];
+            for (int k = 0; k < D; k++)
+                same = same && newpos[(threadIdx.x + q) * 3 + k] == newpos[(threadIdx.x + q + 1) * 3 + k];
+            degenerate = degenerate || same;
+        }
+        // radius rounded up by a relative 1e-12 so that it covers every vertex
+        const double radius = degenerate ? __builtin_inf() : sqrt(r2) * (1.0 + 1e-12);
+        R.bnd[s] = make_double4(c[0], c[1], D == 3 ? c[2] : 0.0, radius);
     }
 }
 

@@ -206,10 +206,26 @@ def test_rod_boundary_error_same_step_as_oracle():
     assert inf.error_step_valid and inf.error_step == 0 and inf.error_agent == 1
 
 
+@pytest.mark.parametrize("pot", ["morse", "mie", "blj", "species"])
+def test_rod_mixed_precision(pot):
+    """Mixed mode (float vertex-segment forces, FP64 state): forces within 1e-5 of the
+    FP64 oracle, and for the soft Morse cases a 10-step trajectory within 1e-6 (the
+    Mie / BoundLJ overlaps of this crowded case amplify any difference within a few
+    steps)."""
+    spec, pos, vel, sp = rods_case(pot=pot, seed=21)
+    g = MechanicsBackend(spec, dt=0.002, precision="mixed")
+    g.set_agents(pos, vel, sp)
+    o = OracleSim(spec, dt=0.002)
+    o.set_agents(pos, vel, sp)
+    assert rel_err(g.compute_forces(), o.compute_forces()) <= 1e-5
+    if pot in ("morse", "species"):
+        g.step(10)
+        o.step(10)
+        assert rel_err(g.get_state()[0], o.get_state()[0]) <= 1e-6
+
+
 def test_rod_create_rejects_bad_configs():
     spec, pos, vel, sp = rods_case(n=10, L=30.0)
-    with pytest.raises(SimulationError):
-        MechanicsBackend(spec, dt=0.002, precision="mixed")
     dom = spec[0].domain
     bad = RodMechanics(n_vertices=17, diffusion_constant=0.0, spring_tension=1.0, rigidity=1.0, spring_length=0.5,
                        damping=1.0)
