@@ -74,10 +74,11 @@ def force_bytes_per_agent(cbar, record_bytes, dim, stencil, force_bytes):
 
 
 def make_backend(wl, precision, graph, timing, seed):
+    """Context with the tiling chosen by the library's autotuner (timed on these agents)."""
     from cellular_raza_amd import MechanicsBackend
 
     g = MechanicsBackend(wl.instances, dt=wl.dt, mechanics_solver_order=wl.solver_order, noise=wl.noise,
-                         seed=seed, precision=precision, graph=graph, timing=timing)
+                         seed=seed, precision=precision, graph=graph, timing=timing, autotune=True)
     g.set_agents(wl.pos, wl.vel, wl.species, wl.counts)
     return g
 
@@ -93,6 +94,9 @@ def run_gpu(wl, precision, steps, warmup, seed=1, timing_steps=20):
     t1 = time.perf_counter()
     sec = t1 - t0
     _, start = g.get_voxels()
+    tiling = None
+    if getattr(wl.instances[0].mechanics, "n_vertices", 1) == 1:
+        tiling = dict(g.tiling(0), policy=g.tiling_policy)
     t_get0 = time.perf_counter()
     pos_end = g.get_state()[0]
     t_get = time.perf_counter() - t_get0  # download of positions and velocities to host buffers
@@ -104,7 +108,7 @@ def run_gpu(wl, precision, steps, warmup, seed=1, timing_steps=20):
     gt.step(timing_steps)
     tim = gt.timing()
     gt.close()
-    return sec, f0, start, tim, pos_end, t_set + t_get
+    return sec, f0, start, tim, pos_end, t_set + t_get, tiling
 
 
 def run_sequential(wl, precision, steps, warmup, seed=1):
@@ -222,7 +226,7 @@ def main():
           flush=True)
 
     main_prec = "mixed" if has_mixed else "fp64"
-    sec_m, f_main, start, tim_m, pos_main, host_io = run_gpu(wl, main_prec, args.steps, args.warmup)
+    sec_m, f_main, start, tim_m, pos_main, host_io, tiling = run_gpu(wl, main_prec, args.steps, args.warmup)
     value = N * args.steps / sec_m
     print(f"bench: {main_prec} {value:.4e} agent-steps/s", file=sys.stderr, flush=True)
     extra = {
@@ -233,7 +237,7 @@ def main():
     }
     if has_mixed:
         fp64_steps = max(args.steps // 4, 10)
-        sec_d, f_fp64, _, tim_d, _, _ = run_gpu(wl, "fp64", fp64_steps, args.warmup)
+        sec_d, f_fp64, _, tim_d, _, _, _ = run_gpu(wl, "fp64", fp64_steps, args.warmup)
         value_fp64 = N * fp64_steps / sec_d
         print(f"bench: fp64 {value_fp64:.4e} agent-steps/s", file=sys.stderr, flush=True)
         extra["fp64_value"] = value_fp64
@@ -297,6 +301,7 @@ def main():
             "precision": main_prec,
             "mechanics_solver_order": wl.solver_order,
             "dt": wl.dt,
+            "force_tiling": tiling,
         },
         **{k: v for k, v in extra.items() if k in ("fp64_value", "mixed_over_fp64", "force_rel_err_mixed_vs_fp64",
                                                   "force_rms_rel_err_mixed_vs_fp64")},

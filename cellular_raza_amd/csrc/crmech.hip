@@ -11,6 +11,9 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <cstdlib>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "crmech.h"
@@ -142,6 +145,9 @@ struct crm_ctx {
     uint32_t n_work = 0;
     ForceTile* d_tiles = nullptr;
     uint32_t n_force_tiles = 0;
+    size_t tiles_cap = 0;
+    uint32_t policy = 0;                  // tiling policy (index into kTilePolicies)
+    std::vector<uint32_t> tile_rows, tile_cols, tile_count;  // per instance
     Control* d_ctrl = nullptr;
     uint32_t* d_start = nullptr;
     uint32_t* d_count = nullptr;
@@ -194,6 +200,7 @@ void free_device(crm_ctx* ctx) {
     }
     ctx->d_work = nullptr;
     ctx->d_tiles = nullptr;
+    ctx->tiles_cap = 0;
     ctx->d_start = ctx->d_count = ctx->d_bsum = ctx->d_key = ctx->d_rank = ctx->d_perm = nullptr;
     ctx->d_force = nullptr;
     ctx->d_rec64 = nullptr;
@@ -562,13 +569,23 @@ crm_status reset_control(crm_ctx* ctx, uint64_t step) {
 // within their LDS tables and the region extent within 31 voxels (>= 26
 // fixed-point bits per voxel); among those it takes the most own agents, then
 // the smallest region.
+// Tiling policies the autotuner chooses from: the fraction of the workgroup's
+// lanes the expected own agents may fill, and the density margin on the staged
+// region.  Policy 0 is the default.
+struct TilePolicy {
+    double own_frac, margin;
+};
+constexpr TilePolicy kTilePolicies[] = {{0.94, 1.2}, {0.94, 1.1}, {0.94, 1.4}, {0.80, 1.2},
+                                        {0.70, 1.2}, {0.55, 1.2}, {0.40, 1.2}, {0.70, 1.4}};
+constexpr uint32_t kNumTilePolicies = sizeof(kTilePolicies) / sizeof(kTilePolicies[0]);
+
 void build_tiles(const InstParams& P, uint32_t dim, uint32_t inst, uint64_t n_agents, uint32_t cap,
-                 std::vector<ForceTile>& out) {
+                 const TilePolicy& pol, std::vector<ForceTile>& out, uint32_t& rows, uint32_t& cols_out) {
     const uint32_t n0 = P.n[0], n1 = P.n[1], n2 = P.n[2];
     const double rho = (double)n_agents / ((double)n0 * n1 * n2);
     const uint32_t np = dim == 3 ? 3u : 1u;  // staged planes
     const double planes = (double)std::min(np, n0 > 1 ? 3u : 1u);
-    const double margin = 1.2, own_max = 0.94 * kBlock;
+    const double margin = pol.margin, own_max = pol.own_frac * kBlock;
     uint32_t bR = 1, bW = 1;
     double best_own = -1.0, best_staged = 0.0;
     for (uint32_t r0 = 1; r0 <= std::min<uint32_t>(n1, kTileRowsMax); r0++) {
@@ -594,11 +611,40 @@ void build_tiles(const InstParams& P, uint32_t dim, uint32_t inst, uint64_t n_ag
         for (uint32_t b = 0; b < n1; b += bR)
             for (uint32_t c = 0; c < n2; c += bW)
                 out.push_back(ForceTile{inst, a, b, std::min(b + bR, n1), c, std::min(c + bW, n2)});
+    rows = bR;
+    cols_out = bW;
 }
 
 template <typename T>
 crm_status dalloc(crm_ctx* ctx, T*& p, size_t n) {
     CRM_HIP(hipMalloc((void**)&p, std::max<size_t>(n, 1) * sizeof(T)));
+    return CRM_OK;
+}
+
+// Force tiles of every instance under tiling policy `pol`.
+void make_tiles(crm_ctx* ctx, uint32_t pol, std::vector<ForceTile>& tiles) {
+    const uint32_t cap = ctx->cfg.precision == CRM_PRECISION_MIXED ? kLdsRecordsMixed : kLdsRecordsFP64;
+    tiles.clear();
+    ctx->tile_rows.assign(ctx->n_inst, 0);
+    ctx->tile_cols.assign(ctx->n_inst, 0);
+    ctx->tile_count.assign(ctx->n_inst, 0);
+    for (uint32_t i = 0; i < ctx->n_inst; i++) {
+        const size_t before = tiles.size();
+        build_tiles(ctx->hparams[i], ctx->D, i, ctx->counts[i], cap, kTilePolicies[pol], tiles, ctx->tile_rows[i],
+                    ctx->tile_cols[i]);
+        ctx->tile_count[i] = (uint32_t)(tiles.size() - before);
+    }
+}
+
+crm_status upload_tiles(crm_ctx* ctx, const std::vector<ForceTile>& tiles) {
+    if (tiles.size() > ctx->tiles_cap) {
+        if (ctx->d_tiles) CRM_HIP(hipFree(ctx->d_tiles));
+        ctx->d_tiles = nullptr;
+        CRM_TRY(dalloc(ctx, ctx->d_tiles, tiles.size()));
+        ctx->tiles_cap = tiles.size();
+    }
+    CRM_HIP(hipMemcpy(ctx->d_tiles, tiles.data(), sizeof(ForceTile) * tiles.size(), hipMemcpyHostToDevice));
+    ctx->n_force_tiles = (uint32_t)tiles.size();
     return CRM_OK;
 }
 
@@ -821,9 +867,7 @@ crm_status crm_set_agents(crm_ctx* ctx, const uint64_t* counts, const double* po
     }
     ctx->n_work = (uint32_t)work.size();
     std::vector<ForceTile> tiles;
-    const uint32_t cap = ctx->cfg.precision == CRM_PRECISION_MIXED ? kLdsRecordsMixed : kLdsRecordsFP64;
-    for (uint32_t i = 0; i < ctx->n_inst; i++) build_tiles(ctx->hparams[i], ctx->D, i, counts[i], cap, tiles);
-    ctx->n_force_tiles = (uint32_t)tiles.size();
+    make_tiles(ctx, ctx->policy, tiles);
     CRM_HIP(hipMemcpy(ctx->d_ip, ctx->hparams.data(), sizeof(InstParams) * ctx->n_inst, hipMemcpyHostToDevice));
 
     const uint64_t Vt = ctx->Vt;
@@ -852,8 +896,7 @@ crm_status crm_set_agents(crm_ctx* ctx, const uint64_t* counts, const double* po
         CRM_TRY(dalloc(ctx, b.vox, N));
     }
     CRM_HIP(hipMemcpy(ctx->d_work, work.data(), sizeof(BlockWork) * work.size(), hipMemcpyHostToDevice));
-    CRM_TRY(dalloc(ctx, ctx->d_tiles, tiles.size()));
-    CRM_HIP(hipMemcpy(ctx->d_tiles, tiles.data(), sizeof(ForceTile) * tiles.size(), hipMemcpyHostToDevice));
+    CRM_TRY(upload_tiles(ctx, tiles));
     CRM_HIP(hipMemsetAsync(ctx->d_count, 0, sizeof(uint32_t) * std::max<uint64_t>(2 * Vt, 1), ctx->stream));
 
     // upload in agent-index order (slot = agent index before the first sort)
@@ -994,6 +1037,139 @@ crm_status crm_get_voxels(crm_ctx* ctx, uint32_t* voxel_of_agent, uint32_t* voxe
                                ctx->stream));
         CRM_HIP(hipStreamSynchronize(ctx->stream));
     }
+    return CRM_OK;
+}
+
+// ---- tiling autotuner ------------------------------------------------------
+namespace {
+
+std::mutex g_tune_mutex;
+std::map<std::string, uint32_t> g_tune_cache;  // workload key -> tiling policy
+
+std::string tune_key(const crm_ctx* ctx) {
+    char buf[256];
+    const InstParams& P = ctx->hparams[0];
+    std::snprintf(buf, sizeof buf, "dim=%u precision=%u potential=%u weighting=%u instances=%u agents=%llu grid=%ux%ux%u",
+                  ctx->D, ctx->cfg.precision, ctx->pot_kind, ctx->cfg.weighting, ctx->n_inst,
+                  (unsigned long long)ctx->N, P.n[0], P.n[1], P.n[2]);
+    return buf;
+}
+
+// The cache file is a flat JSON object {"<key>": <policy>, ...}.
+void read_tune_file(const char* path) {
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return;
+    std::string text;
+    char chunk[4096];
+    size_t n;
+    while ((n = std::fread(chunk, 1, sizeof chunk, f)) > 0) text.append(chunk, n);
+    std::fclose(f);
+    size_t i = 0;
+    while ((i = text.find('"', i)) != std::string::npos) {
+        const size_t j = text.find('"', i + 1);
+        if (j == std::string::npos) break;
+        const std::string key = text.substr(i + 1, j - i - 1);
+        const size_t c = text.find(':', j);
+        if (c == std::string::npos) break;
+        char* end = nullptr;
+        const unsigned long v = std::strtoul(text.c_str() + c + 1, &end, 10);
+        if (end != text.c_str() + c + 1 && v < kNumTilePolicies) g_tune_cache[key] = (uint32_t)v;
+        i = (size_t)(end - text.c_str());
+    }
+}
+
+void write_tune_file(const char* path) {
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return;
+    std::fputs("{\n", f);
+    size_t k = 0;
+    for (const auto& e : g_tune_cache)
+        std::fprintf(f, "  \"%s\": %u%s\n", e.first.c_str(), e.second, ++k < g_tune_cache.size() ? "," : "");
+    std::fputs("}\n", f);
+    std::fclose(f);
+}
+
+crm_status apply_policy(crm_ctx* ctx, uint32_t pol) {
+    std::vector<ForceTile> tiles;
+    make_tiles(ctx, pol, tiles);
+    CRM_TRY(upload_tiles(ctx, tiles));
+    ctx->policy = pol;
+    for (auto& g : ctx->graph)  // captured with the old tile table
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+    return CRM_OK;
+}
+
+}  // namespace
+
+crm_status crm_autotune(crm_ctx* ctx, const char* cache_path, uint32_t* chosen) {
+    if (!ctx || !ctx->have_agents) return CRM_INVALID_ARGUMENT;
+    if (ctx->halted) return CRM_INVALID_ARGUMENT;
+    CRM_HIP(hipSetDevice(ctx->device));
+    if (is_rod(ctx)) {  // rods do not use force tiles
+        if (chosen) *chosen = ctx->policy;
+        return CRM_OK;
+    }
+    const std::string key = tune_key(ctx);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mutex);
+        if (cache_path) read_tune_file(cache_path);
+        const auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end()) {
+            CRM_TRY(apply_policy(ctx, it->second));
+            if (chosen) *chosen = it->second;
+            return CRM_OK;
+        }
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    CRM_HIP(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        ctx->err = "hipEventCreate failed";
+        return CRM_DEVICE_ERROR;
+    }
+    uint32_t best = ctx->policy;
+    float best_ms = 0.0f;
+    crm_status st = CRM_OK;
+    for (uint32_t p = 0; p < kNumTilePolicies && st == CRM_OK; p++) {
+        st = apply_policy(ctx, p);
+        if (st) break;
+        st = launch_force(ctx);  // warm-up
+        if (st) break;
+        if (hipEventRecord(e0, ctx->stream) != hipSuccess) st = CRM_DEVICE_ERROR;
+        for (int r = 0; r < 3 && st == CRM_OK; r++) st = launch_force(ctx);
+        if (st) break;
+        float ms = 0.0f;
+        if (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+            st = CRM_DEVICE_ERROR;
+        if (st == CRM_OK && (p == 0 || ms < best_ms)) {
+            best_ms = ms;
+            best = p;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (st == CRM_DEVICE_ERROR && ctx->err.empty()) ctx->err = "HIP error while timing the tiling policies";
+    if (st) return st;
+    CRM_TRY(apply_policy(ctx, best));
+    CRM_TRY(check_errors(ctx, "autotune"));
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mutex);
+        g_tune_cache[key] = best;
+        if (cache_path) write_tune_file(cache_path);
+    }
+    if (chosen) *chosen = best;
+    return CRM_OK;
+}
+
+crm_status crm_get_tiling(crm_ctx* ctx, uint32_t inst, uint32_t* rows, uint32_t* cols, uint32_t* n_tiles) {
+    if (!ctx || !ctx->have_agents || inst >= ctx->n_inst || is_rod(ctx)) return CRM_INVALID_ARGUMENT;
+    if (rows) *rows = ctx->tile_rows[inst];
+    if (cols) *cols = ctx->tile_cols[inst];
+    if (n_tiles) *n_tiles = ctx->tile_count[inst];
     return CRM_OK;
 }
 

@@ -56,6 +56,7 @@ class MechanicsBackend:
         graph: bool = True,
         timing: bool = False,
         lds: bool = True,
+        autotune=False,
     ):
         self._lib = load()
         if not instances:
@@ -89,6 +90,10 @@ class MechanicsBackend:
         self._h = handle
         self.n_agents = 0
         self.counts: List[int] = []
+        # False: default tiling; True: tune in memory after each set_agents; a path:
+        # also read / update that JSON cache of earlier choices
+        self.autotune = autotune
+        self.tiling_policy: Optional[int] = None
 
     # -- helpers -----------------------------------------------------------
     def _message(self, h) -> str:
@@ -143,6 +148,11 @@ class MechanicsBackend:
             None if sp_a is None else sp_a.ctypes.data_as(_abi.I32P),
         )
         self._check(st)
+        if self.autotune:
+            path = None if self.autotune is True else str(self.autotune).encode()
+            chosen = C.c_uint32(0)
+            self._check(self._lib.crm_autotune(self._h, path, C.byref(chosen)))
+            self.tiling_policy = int(chosen.value)
         self.n_agents = n
         self.counts = [int(c) for c in counts_a]
 
@@ -185,6 +195,12 @@ class MechanicsBackend:
             self._lib.crm_get_voxels(self._h, vox.ctypes.data_as(_abi.U32P), start.ctypes.data_as(_abi.U32P))
         )
         return vox, start
+
+    def tiling(self, inst: int = 0) -> Dict[str, int]:
+        """Force tiling of instance `inst`: voxel rows and columns per tile, tile count."""
+        r, c, n = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        self._check(self._lib.crm_get_tiling(self._h, inst, C.byref(r), C.byref(c), C.byref(n)))
+        return {"rows": int(r.value), "cols": int(c.value), "tiles": int(n.value)}
 
     def timing(self) -> Dict[str, Dict[str, float]]:
         t = _abi.Timing()

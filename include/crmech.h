@@ -240,6 +240,19 @@ crm_status crm_get_voxels(crm_ctx* ctx, uint32_t* voxel_of_agent, uint32_t* voxe
 
 crm_status crm_get_info(crm_ctx* ctx, crm_info* info);
 
+/* Tiling autotuner (no counterpart in the reference, whose CPU backend has no
+ * work tiles): times the force kernel on the current agents under each of the
+ * library's tiling policies (share of a workgroup's lanes filled by the tile's
+ * own agents, density margin of the staged region) and keeps the fastest.  The
+ * choice is cached per workload key (dimension, precision, potential, weighting,
+ * instance count, agent count, grid) for the process and, when cache_path is not
+ * NULL, in that JSON file, which is read first and rewritten.  Forces do not
+ * depend on the tiling.  chosen (may be NULL) receives the policy index. */
+crm_status crm_autotune(crm_ctx* ctx, const char* cache_path, uint32_t* chosen);
+
+/* Force tiling of instance inst: voxel rows and columns per tile, tile count. */
+crm_status crm_get_tiling(crm_ctx* ctx, uint32_t inst, uint32_t* rows, uint32_t* cols, uint32_t* n_tiles);
+
 crm_status crm_get_timing(crm_ctx* ctx, crm_timing* timing);
 crm_status crm_reset_timing(crm_ctx* ctx);
 

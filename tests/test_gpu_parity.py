@@ -407,3 +407,28 @@ def test_full_size_c2_invariants():
     assert np.array_equal(np.bincount(vox, minlength=len(start) - 1), np.diff(start.astype(np.int64)))
     assert np.array_equal(vox.astype(np.int64), voxel_reference(wl.instances[0].domain, pos))
     assert np.isfinite(pos).all()
+
+
+def test_autotune_keeps_results_and_caches(tmp_path):
+    """The tiling autotuner changes only the work decomposition: forces and states are
+    bitwise those of the default tiling; its choice is written to and read from the
+    JSON cache."""
+    wl = workloads.c2_sorting_3d(n=60000, seed=40, steps=5)
+    for precision in ("mixed", "fp64"):
+        ref = MechanicsBackend(wl.instances, dt=wl.dt, precision=precision, noise="philox", seed=2)
+        ref.set_agents(wl.pos, wl.vel, wl.species)
+        path = tmp_path / f"tiling_{precision}.json"
+        tuned = MechanicsBackend(wl.instances, dt=wl.dt, precision=precision, noise="philox", seed=2,
+                                 autotune=str(path))
+        tuned.set_agents(wl.pos, wl.vel, wl.species)
+        assert tuned.tiling_policy is not None and path.exists()
+        assert f"precision={1 if precision == 'mixed' else 0}" in path.read_text()
+        t = tuned.tiling()
+        assert t["rows"] >= 1 and t["cols"] >= 1 and t["tiles"] >= 1
+        assert np.array_equal(tuned.compute_forces(), ref.compute_forces())
+        tuned.step(5)
+        ref.step(5)
+        assert np.array_equal(tuned.get_state()[0], ref.get_state()[0])
+        again = MechanicsBackend(wl.instances, dt=wl.dt, precision=precision, autotune=str(path))
+        again.set_agents(wl.pos, wl.vel, wl.species)
+        assert again.tiling_policy == tuned.tiling_policy
