@@ -152,6 +152,8 @@ SIGNATURES = {
     "crm_autotune": (C.c_int, [P, C.c_char_p, C.POINTER(C.c_uint32)]),
     "crm_get_tiling": (C.c_int, [P, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                  C.POINTER(C.c_uint32)]),
+    "crm_set_neighbor_sensing": (C.c_int, [P, C.c_uint32, C.c_double]),
+    "crm_get_neighbor_counts": (C.c_int, [P, U32P]),
     "crm_get_timing": (C.c_int, [P, C.POINTER(Timing)]),
     "crm_reset_timing": (C.c_int, [P]),
     "crm_last_error": (C.c_size_t, [P, C.c_char_p, C.c_size_t]),

@@ -166,6 +166,12 @@ struct crm_ctx {
     double* d_recrod = nullptr;  // rods: AoS vertex positions [N][K]
     double4* d_bnd = nullptr;    // rods: bounding balls [N]
     uint32_t rods_per_block = 0;
+    // neighbour sensing: counts by slot ([N][nv] for rods) of the last force
+    // evaluation, which read state buffer sense_parity (-1: none since enabled)
+    bool sense_on = false;
+    double sense_t = 0.0;
+    uint32_t* d_sense = nullptr;
+    int sense_parity = -1;
 
     uint64_t N = 0;
     std::vector<uint64_t> counts;
@@ -189,7 +195,8 @@ thread_local std::string g_create_error;
 
 void free_device(crm_ctx* ctx) {
     void* ptrs[] = {ctx->d_work, ctx->d_tiles, ctx->d_start, ctx->d_count, ctx->d_bsum, ctx->d_key, ctx->d_rank,
-                    ctx->d_perm, ctx->d_force, ctx->d_rec64, ctx->d_recq, ctx->d_noise, ctx->d_recrod, ctx->d_bnd};
+                    ctx->d_perm, ctx->d_force, ctx->d_rec64, ctx->d_recq, ctx->d_noise, ctx->d_recrod, ctx->d_bnd,
+                    ctx->d_sense};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& b : ctx->buf) {
@@ -208,6 +215,8 @@ void free_device(crm_ctx* ctx) {
     ctx->d_noise = nullptr;
     ctx->d_recrod = nullptr;
     ctx->d_bnd = nullptr;
+    ctx->d_sense = nullptr;
+    ctx->sense_parity = -1;
     ctx->noise_cap = 0;
     for (auto& g : ctx->graph)
         if (g) {
@@ -233,6 +242,8 @@ Arrays arrays(crm_ctx* ctx) {
     A.force = ctx->d_force;
     A.rec64 = ctx->d_rec64;
     A.recq = ctx->d_recq;
+    A.sense = ctx->sense_on ? ctx->d_sense : nullptr;
+    A.sense_t = ctx->sense_t;
     A.N = ctx->N;
     A.Vt = (uint32_t)ctx->Vt;
     return A;
@@ -340,6 +351,7 @@ crm_status launch_force_rod_d(crm_ctx* ctx, StateBuf S) {
 crm_status launch_force(crm_ctx* ctx) {
     CRM_TRY(t_begin(ctx));
     StateBuf S = ctx->buf[ctx->cur];
+    ctx->sense_parity = ctx->cur;
     crm_status st;
     if (is_rod(ctx)) st = ctx->D == 2 ? launch_force_rod_d<2>(ctx, S) : launch_force_rod_d<3>(ctx, S);
     else st = ctx->D == 2 ? launch_force_d<2>(ctx, S) : launch_force_d<3>(ctx, S);
@@ -886,6 +898,7 @@ crm_status crm_set_agents(crm_ctx* ctx, const uint64_t* counts, const double* po
     }
     else if (ctx->cfg.precision == CRM_PRECISION_MIXED) CRM_TRY(dalloc(ctx, ctx->d_recq, N));
     else CRM_TRY(dalloc(ctx, ctx->d_rec64, N));
+    if (ctx->sense_on) CRM_TRY(dalloc(ctx, ctx->d_sense, (size_t)ctx->nv * N));
     for (auto& b : ctx->buf) {
         CRM_TRY(dalloc(ctx, b.pos, (size_t)K * N));
         CRM_TRY(dalloc(ctx, b.vel, (size_t)K * N));
@@ -967,6 +980,8 @@ crm_status crm_step(crm_ctx* ctx, uint32_t n_steps, const double* noise) {
     for (; s < n_steps; s++) CRM_TRY(launch_step(ctx, use_noise_buf ? ctx->d_noise + per_step * s : nullptr));
     CRM_HIP(hipStreamSynchronize(ctx->stream));
     ctx->step += n_steps;
+    // the last step's forces (and counts) read the buffer before its gather
+    if (n_steps > 0) ctx->sense_parity = 1 - ctx->cur;
     return check_errors(ctx, "step");
 }
 
@@ -1162,6 +1177,56 @@ crm_status crm_autotune(crm_ctx* ctx, const char* cache_path, uint32_t* chosen) 
         if (cache_path) write_tune_file(cache_path);
     }
     if (chosen) *chosen = best;
+    return CRM_OK;
+}
+
+crm_status crm_set_neighbor_sensing(crm_ctx* ctx, uint32_t enable, double radius) {
+    if (!ctx) return CRM_INVALID_ARGUMENT;
+    if (enable && std::isnan(radius)) {
+        ctx->err = "the sensing radius is NaN";
+        return CRM_INVALID_ARGUMENT;
+    }
+    CRM_HIP(hipSetDevice(ctx->device));
+    CRM_HIP(hipStreamSynchronize(ctx->stream));
+    double t = 0.0;
+    if (enable && radius > 0.0) {
+        // smallest t with sqrt(t) >= radius: then sqrt(d2) < radius <=> d2 < t
+        // (sqrt is correctly rounded and monotone)
+        t = radius * radius;
+        while (std::sqrt(t) < radius) t = std::nextafter(t, HUGE_VAL);
+        for (double p = std::nextafter(t, 0.0); t > 0.0 && std::sqrt(p) >= radius; p = std::nextafter(t, 0.0))
+            t = p;
+    }
+    if (enable && ctx->have_agents && !ctx->d_sense) CRM_TRY(dalloc(ctx, ctx->d_sense, (size_t)ctx->nv * ctx->N));
+    ctx->sense_on = enable != 0;
+    ctx->sense_t = t;
+    ctx->sense_parity = -1;
+    for (auto& g : ctx->graph)  // captured with the old kernel arguments
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+    return CRM_OK;
+}
+
+crm_status crm_get_neighbor_counts(crm_ctx* ctx, uint32_t* counts) {
+    if (!ctx || !counts || !ctx->have_agents) return CRM_INVALID_ARGUMENT;
+    if (!ctx->sense_on || ctx->sense_parity < 0) {
+        ctx->err = "no force evaluation with neighbour sensing since it was enabled or the agents were set";
+        return CRM_INVALID_ARGUMENT;
+    }
+    CRM_HIP(hipSetDevice(ctx->device));
+    const uint64_t N = ctx->N, nv = ctx->nv;
+    std::vector<uint32_t> id(N), c((size_t)nv * N);
+    CRM_HIP(hipMemcpyAsync(id.data(), ctx->buf[ctx->sense_parity].id, N * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    CRM_HIP(hipMemcpyAsync(c.data(), ctx->d_sense, c.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    CRM_HIP(hipStreamSynchronize(ctx->stream));
+    for (uint64_t s = 0; s < N; s++) {
+        uint32_t n = 0;
+        for (uint64_t v = 0; v < nv; v++) n += c[s * nv + v];
+        counts[id[s]] = n;
+    }
     return CRM_OK;
 }
 

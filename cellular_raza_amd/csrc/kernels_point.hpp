@@ -47,9 +47,22 @@ struct Arrays {
     double* force;    // [K][N]
     double4* rec64;   // [N] (x, y, z, species) absolute, FP64 path
     int4* recq;       // [N] (qx, qy, qz, species | outside << 31) fixed point in the voxel, mixed path
+    uint32_t* sense;  // [N] (rods: [N][nv]) neighbour counts by slot; null when sensing is off
+    double sense_t;   // a pair is sensed when its squared distance is below this
     uint64_t N;
     uint32_t Vt;
 };
+
+// Squared norm summed in axis order with every product and sum rounded on its
+// own (no fused multiply-add), as nalgebra's norm_squared computes it.
+template <int D>
+__device__ __forceinline__ double sum_squares_unfused(const double* z) {
+#pragma clang fp contract(off)
+    double d2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; k++) d2 = d2 + z[k] * z[k];
+    return d2;
+}
 
 __device__ __forceinline__ void raise_error(Control* c, uint32_t kind, uint64_t step, uint64_t agent) {
     atomicOr(&c->error_flags, 1u << kind);
@@ -314,6 +327,33 @@ __device__ __forceinline__ void agent_force(const Arrays& A, const StateBuf& S, 
     double thr_d = rcd * rcd * (1.0 + 1e-12);
     if (!(thr_d >= 0.0)) thr_d = __builtin_inf();
 
+    // Neighbour sensing (NeighborSensing::accumulate_information, concepts
+    // interaction.rs:64-79, called per stencil pair at update_mechanics.rs:105,
+    // 199-202) counts the candidates with squared distance below sense_t, the
+    // smallest double whose square root reaches the radius, so that the test is
+    // the reference's norm < radius.  Pass 1 then marks up to the larger of the
+    // two radii, and pass 2 replays pass 1's force test for each marked pair.
+    // Mixed precision counts from the fixed-point distance outside a band that
+    // bounds its error (quantisation of both ends plus float rounding) and
+    // decides the band's pairs from the FP64 positions.
+    const bool sense = A.sense != nullptr;
+    uint32_t n_sense = 0;
+    float mark_f = thr_f, rs2 = 0.0f, band_s = 0.0f;
+    double mark_d = thr_d;
+    if (sense) {
+        if constexpr (MIXED) {
+            const float umax = fmaxf(fmaxf(unit[0], unit[1]), unit[D - 1]);
+            const float q = 2.0f * (float)D * umax;
+            rs2 = (float)A.sense_t;
+            band_s = 2e-6f * rs2 + 4.0f * sqrtf(rs2) * q + q * q;
+            float t = (rs2 + 2.0f * band_s) / (unit[0] * unit[0]);
+            if (!(t >= 0.0f) || flag_i) t = __builtin_inff();
+            mark_f = fmaxf(thr_f, t);
+        } else {
+            mark_d = fmax(thr_d, A.sense_t * (1.0 + 1e-12));
+        }
+    }
+
     // species-matrix row (own species) and column (own species as ext)
     Row4<T> srow, scol;
     {
@@ -365,13 +405,59 @@ __device__ __forceinline__ void agent_force(const Arrays& A, const StateBuf& S, 
         return d2;
     };
 
+    // Pass 1's squared distance: mixed in units of the first axis' fixed-point
+    // step, FP64 with fused products.
+    auto mark_dist = [&](const Rec& rj) -> T {
+        if constexpr (MIXED) {
+            const float c0 = (float)(me.x - rj.x);
+            const float c1 = (float)(me.y - rj.y) * r1;
+            float d2 = fmaf(c1, c1, c0 * c0);
+            if constexpr (D == 3) {
+                const float c2 = (float)(me.z - rj.z) * r2;
+                d2 = fmaf(c2, c2, d2);
+            }
+            return d2;
+        } else {
+            const double z0 = me.x - rj.x, z1 = me.y - rj.y;
+            double d2 = fma(z1, z1, z0 * z0);
+            if constexpr (D == 3) {
+                const double z2 = me.z - rj.z;
+                d2 = fma(z2, z2, d2);
+            }
+            return d2;
+        }
+    };
+
     // The pair with the candidate at walk position w, weighted as in the reference.
     auto evaluate = [&](const W& w) {
         const uint32_t j = w.j;
         const Rec rj = src.record(j, w);
-        const bool full = physical || (j >= ov.b && j < ov.e);
         T z[D];
         const T d2 = displacement(rj, z);
+        if (sense) {
+            bool in;
+            if constexpr (MIXED) {
+                if (flag_i || rj.w < 0 || fabsf(d2 - rs2) <= band_s) {
+                    const uint32_t gj = src.slot(j, w);
+                    double zd[D];
+#pragma unroll
+                    for (int k = 0; k < D; k++) zd[k] = S.pos[k * N + s] - S.pos[k * N + gj];
+                    in = sum_squares_unfused<D>(zd) < A.sense_t;
+                } else {
+                    in = d2 < rs2;
+                }
+            } else {
+                in = sum_squares_unfused<D>(z) < A.sense_t;
+            }
+            n_sense += in ? 1u : 0u;
+            // the force only for the pairs pass 1 marks without sensing
+            if constexpr (POT == CRM_POTENTIAL_NONE) return;
+            bool fmark;
+            if constexpr (MIXED) fmark = mark_dist(rj) <= thr_f || rj.w < 0;
+            else fmark = mark_dist(rj) <= thr_d;
+            if (!fmark) return;
+        }
+        const bool full = physical || (j >= ov.b && j < ov.e);
         int sp_j;
         int cut = -1;
         if constexpr (MIXED) {
@@ -417,7 +503,7 @@ __device__ __forceinline__ void agent_force(const Arrays& A, const StateBuf& S, 
         }
     };
 
-    bool live = POT != CRM_POTENTIAL_NONE;
+    bool live = POT != CRM_POTENTIAL_NONE || sense;
     if (live) live = cur.settle();
     while (live) {
         const W c0 = cur;
@@ -428,24 +514,8 @@ __device__ __forceinline__ void agent_force(const Arrays& A, const StateBuf& S, 
             const uint32_t j = cur.j;
             const Rec rj = src.record(j, cur);
             bool mark;
-            if constexpr (MIXED) {
-                const float c0 = (float)(me.x - rj.x);
-                const float c1 = (float)(me.y - rj.y) * r1;
-                float d2 = fmaf(c1, c1, c0 * c0);
-                if constexpr (D == 3) {
-                    const float c2 = (float)(me.z - rj.z) * r2;
-                    d2 = fmaf(c2, c2, d2);
-                }
-                mark = d2 <= thr_f || rj.w < 0;
-            } else {
-                const double z0 = me.x - rj.x, z1 = me.y - rj.y;
-                double d2 = fma(z1, z1, z0 * z0);
-                if constexpr (D == 3) {
-                    const double z2 = me.z - rj.z;
-                    d2 = fma(z2, z2, d2);
-                }
-                mark = d2 <= thr_d;
-            }
+            if constexpr (MIXED) mark = mark_dist(rj) <= mark_f || rj.w < 0;
+            else mark = mark_dist(rj) <= mark_d;
             mark = mark && j != self;
             const uint32_t bit = 1u << (p & 31u);
             if (p < 32u) mlo = mark ? (mlo | bit) : mlo;
@@ -475,6 +545,7 @@ __device__ __forceinline__ void agent_force(const Arrays& A, const StateBuf& S, 
     if (calc_err) raise_error(A.ctrl, CRM_CALC_ERROR, A.ctrl->step, S.id[s]);
 #pragma unroll
     for (int k = 0; k < D; k++) A.force[(uint64_t)k * N + s] = MIXED ? (double)ks[k] + (double)(-kc[k]) : accd[k];
+    if (sense) A.sense[s] = n_sense;
 }
 
 // A force tile: plane a0, rows [b0, b1), columns [c0, c1) of one instance.

@@ -356,6 +356,45 @@ __global__ __launch_bounds__(kBlock) void k_force_rod(Arrays A, StateBuf S, RodA
     if (calc_err) raise_error(A.ctrl, CRM_CALC_ERROR, A.ctrl->step, S.id[s]);
 #pragma unroll
     for (int k = 0; k < D; k++) A.force[(uint64_t)(v * D + k) * A.N + s] = acc[k];
+
+    // Neighbour sensing as in the rods example (bacterial_rods/src/main.rs:48-78):
+    // vertex v counts the vertices u of every other rod of the stencil with
+    // norm(v - u) < radius, i.e. squared distance below sense_t; rods whose
+    // bounding ball is out of the radius' reach are skipped.
+    if (A.sense) {
+        const double tsq = A.sense_t;
+        float sreach = (float)sqrt(tsq) * (1.0f + 1e-6f) + 1e-5f;
+        if (!(sreach >= 0.0f)) sreach = __builtin_inff();
+        uint32_t n = 0;
+        RodWalk sw;
+        bool more = first(sw);
+        while (more) {
+            const uint32_t j = sw.j;
+            if (j != s) {
+                const double4 bj = R.bnd[j];
+                const float lim = (float)bj.w * (1.0f + 1e-6f) + sreach;
+                float d2 = 0.0f;
+                const double cj[3] = {bj.x, bj.y, bj.z};
+#pragma unroll
+                for (int k = 0; k < D; k++) {
+                    const float d = (float)(pv[k] - cj[k]);
+                    d2 = fmaf(d, d, d2);
+                }
+                if (!(d2 > lim * lim)) {
+                    const double* ext = R.rec + (uint64_t)j * K;
+                    for (uint32_t u = 0; u < nv; u++) {
+                        double z[D];
+#pragma unroll
+                        for (int k = 0; k < D; k++) z[k] = pv[k] - ext[u * D + k];
+                        n += sum_squares_unfused<D>(z) < tsq ? 1u : 0u;
+                    }
+                }
+            }
+            ++sw.j;
+            more = settle(sw);
+        }
+        A.sense[s * nv + v] = n;
+    }
 }
 
 // Normal k of a rod's K normals for (agent, step): Philox block k / 2, element k % 2.

@@ -202,6 +202,22 @@ class MechanicsBackend:
         self._check(self._lib.crm_get_tiling(self._h, inst, C.byref(r), C.byref(c), C.byref(n)))
         return {"rows": int(r.value), "cols": int(c.value), "tiles": int(n.value)}
 
+    def set_neighbor_sensing(self, radius: Optional[float]) -> None:
+        """Count, in every force evaluation, each agent's (point or vertex) pairs
+        with the agents of its 3^D voxel stencil closer than `radius` (the
+        accumulator of bacterial_rods/src/main.rs:48-78, NeighborSensing in
+        concepts interaction.rs:64-79); None switches it off."""
+        if radius is None:
+            self._check(self._lib.crm_set_neighbor_sensing(self._h, 0, 0.0))
+        else:
+            self._check(self._lib.crm_set_neighbor_sensing(self._h, 1, float(radius)))
+
+    def neighbor_counts(self) -> np.ndarray:
+        """Counts of the last force evaluation by agent index (uint32)."""
+        out = np.empty(self.n_agents, dtype=np.uint32)
+        self._check(self._lib.crm_get_neighbor_counts(self._h, out.ctypes.data_as(_abi.U32P)))
+        return out
+
     def timing(self) -> Dict[str, Dict[str, float]]:
         t = _abi.Timing()
         self._check(self._lib.crm_get_timing(self._h, C.byref(t)))

@@ -253,6 +253,25 @@ crm_status crm_autotune(crm_ctx* ctx, const char* cache_path, uint32_t* chosen);
 /* Force tiling of instance inst: voxel rows and columns per tile, tile count. */
 crm_status crm_get_tiling(crm_ctx* ctx, uint32_t inst, uint32_t* rows, uint32_t* cols, uint32_t* n_tiles);
 
+/* Neighbour sensing fused into the force pass.  Replaces the NeighborSensing
+ * hooks of the chili pair loop (concepts interaction.rs:64-79 trait,
+ * update_mechanics.rs:125-139 neighbor_sensing_single, called per pair at
+ * update_mechanics.rs:105 and 199-202) for the accumulator of the rods example
+ * (bacterial_rods/src/main.rs:48-78): during every force evaluation each agent
+ * counts, over the other agents of its voxel and of the adjacent voxels, the
+ * (own point, other point) pairs -- points, or rod vertices -- with
+ * norm(own - other) < radius.  enable = 0 switches it off; a NaN radius is
+ * rejected; radius <= 0 counts nothing.  The setting survives crm_set_agents. */
+crm_status crm_set_neighbor_sensing(crm_ctx* ctx, uint32_t enable, double radius);
+
+/* Counts of the last force evaluation (the last step's, or crm_compute_forces'),
+ * by agent index: counts[n_agents].  Their positions are the state before that
+ * step's integration, as the reference's accumulator sees it when
+ * react_to_neighbors runs (update_mechanics.rs:561-576).  CRM_INVALID_ARGUMENT
+ * when no force evaluation with sensing on has run since it was enabled or the
+ * agents were set. */
+crm_status crm_get_neighbor_counts(crm_ctx* ctx, uint32_t* counts);
+
 crm_status crm_get_timing(crm_ctx* ctx, crm_timing* timing);
 crm_status crm_reset_timing(crm_ctx* ctx);
 

@@ -584,6 +584,7 @@ struct SimBase {
     virtual uint32_t history_out(double* dx, double* dv) = 0;
     virtual void voxels_out(uint32_t* voxel_of_agent, uint32_t* start_out, uint64_t* order_out) = 0;
     virtual uint64_t n_voxels_total() = 0;
+    virtual void neighbor_counts(double radius, uint32_t* counts) = 0;
 };
 
 template <int KM>
@@ -1127,6 +1128,38 @@ struct Sim : SimBase {
         return ns;
     }
 
+    // NeighborSensing as in the rods example (bacterial_rods/src/main.rs:48-78): every
+    // cell senses every other cell of its voxel and of the neighbouring voxels once
+    // per step (update_mechanics.rs:199-202 same voxel, 105 adjacent voxels), and the
+    // accumulator counts the (own point, ext point) pairs -- points, or rod
+    // vertices -- with norm(own - ext) < radius.
+    void neighbor_counts(double radius, uint32_t* counts) override {
+        std::vector<uint64_t> nb;
+        for (auto& I : inst) {
+            for (uint64_t v = 0; v < I.voxels.size(); v++) {
+                neighbors_of(I.grid, v, nb);
+                nb.push_back(v);
+                for (const auto& c : I.voxels[v]) {
+                    uint32_t n = 0;
+                    for (uint64_t w : nb)
+                        for (const auto& e : I.voxels[w]) {
+                            if (&e == &c) continue;
+                            for (int a = 0; a < nv; a++)
+                                for (int b = 0; b < nv; b++) {
+                                    double d2 = 0.0;
+                                    for (int k = 0; k < D; k++) {
+                                        const double d = c.pos[a * D + k] - e.pos[b * D + k];
+                                        d2 = d2 + d * d;
+                                    }
+                                    if (std::sqrt(d2) < radius) n++;
+                                }
+                        }
+                    counts[c.id] = n;
+                }
+            }
+        }
+    }
+
     void voxels_out(uint32_t* voxel_of_agent, uint32_t* start_out, uint64_t* order_out) override {
         uint64_t off = 0, pos = 0;
         for (auto& I : inst) {
@@ -1312,6 +1345,14 @@ int orc_get_voxels(void* h, uint32_t* voxel_of_agent, uint32_t* start_out, uint6
 }
 
 uint64_t orc_n_voxels_total(void* h) { return ((SimBase*)h)->n_voxels_total(); }
+
+// Neighbour count of every agent (by agent index) for the current positions.
+int orc_neighbor_counts(void* h, double radius, uint32_t* counts) {
+    SimBase& S = *(SimBase*)h;
+    if (!S.have_agents) return CRM_INVALID_ARGUMENT;
+    S.neighbor_counts(radius, counts);
+    return 0;
+}
 
 uint64_t orc_n_agents(void* h) { return ((SimBase*)h)->n_agents; }
 

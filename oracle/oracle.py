@@ -54,6 +54,7 @@ def load():
         "orc_get_state": (C.c_int, [P, DP, DP]),
         "orc_get_history": (C.c_int, [P, DP, DP, _abi.U32P]),
         "orc_get_voxels": (C.c_int, [P, _abi.U32P, _abi.U32P, _abi.U64P]),
+        "orc_neighbor_counts": (C.c_int, [P, C.c_double, _abi.U32P]),
         "orc_n_voxels_total": (C.c_uint64, [P]),
         "orc_n_agents": (C.c_uint64, [P]),
         "orc_last_error": (C.c_size_t, [P, C.c_char_p, C.c_size_t]),
@@ -179,6 +180,12 @@ class OracleSim:
         self._check(self._lib.orc_get_voxels(self._h, vox.ctypes.data_as(_abi.U32P),
                                              start.ctypes.data_as(_abi.U32P), order.ctypes.data_as(_abi.U64P)))
         return vox, start, order
+
+    def neighbor_counts(self, radius):
+        """Stencil neighbour count per agent: (point or vertex) pairs closer than radius."""
+        out = np.empty(self.n_agents, dtype=np.uint32)
+        self._check(self._lib.orc_neighbor_counts(self._h, float(radius), out.ctypes.data_as(_abi.U32P)))
+        return out
 
 
 # -- pure helpers ---------------------------------------------------------

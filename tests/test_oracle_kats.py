@@ -448,3 +448,59 @@ def test_langevin_msd_oracle():
         + 6 * kT * t / damping
     d2 = ((s.get_state()[0] - x0) ** 2).sum(axis=1)
     assert abs(expected - d2.mean()) <= 3.5 * d2.std()
+
+
+# ---------------------------------------------------------------------------
+# Neighbour sensing (bacterial_rods/src/main.rs:48-78 accumulator): brute force
+# over all pairs, with radii inside the voxel stencil, and the strict inequality
+# ---------------------------------------------------------------------------
+def test_neighbor_counts_brute_force_points():
+    from scipy.spatial.distance import cdist
+
+    from cellular_raza_amd import workloads
+
+    wl = workloads.c2_sorting_3d(n=3000, seed=3)
+    o = O.OracleSim(wl.instances, dt=wl.dt)
+    o.set_agents(wl.pos, wl.vel, wl.species)
+    d = cdist(wl.pos, wl.pos)
+    np.fill_diagonal(d, np.inf)
+    for radius in (0.8, 1.6, 2.5):  # voxel edge >= 2.5: every pair within radius is in the stencil
+        assert np.array_equal(o.neighbor_counts(radius), (d < radius).sum(1)), radius
+    o.close()
+
+
+def test_neighbor_counts_brute_force_rods():
+    from scipy.spatial.distance import cdist
+
+    from cellular_raza_amd import RodInteraction, workloads
+
+    rng = np.random.default_rng(4)
+    n, nv = 300, 6
+    pos = workloads.straight_rods(rng, n, 2, 30.0, nv, 0.5)
+    dom = CartesianCuboid.from_boundaries_and_interaction_range([0.0, 0.0], [30.0, 30.0], 4.0)
+    spec = [InstanceSpec(dom, RodInteraction(MorsePotential(radius=0.5, potential_stiffness=2.0, cutoff=1.5,
+                                                            strength=0.1)),
+                         RodMechanics(n_vertices=nv, diffusion_constant=0.0, spring_tension=10.0, rigidity=2.0,
+                                      spring_length=0.5, damping=1.5))]
+    o = O.OracleSim(spec, dt=0.002)
+    o.set_agents(pos)
+    flat = pos.reshape(-1, 2)
+    rod = np.repeat(np.arange(n), nv)
+    d = cdist(flat, flat)
+    d[rod[:, None] == rod[None, :]] = np.inf
+    for radius in (0.4, 1.0):  # rod half length 1.25 + radius <= voxel edge 4: stencil covers
+        want = (d < radius).reshape(n, nv, n * nv).sum((1, 2))
+        assert np.array_equal(o.neighbor_counts(radius), want), radius
+    o.close()
+
+
+def test_neighbor_counts_strict_inequality():
+    """Points at exact distance 1: norm < radius excludes radius 1, includes the next double."""
+    dom = CartesianCuboid.from_boundaries_and_interaction_range([0.0, 0.0], [8.0, 8.0], 2.0)
+    spec = [InstanceSpec(dom, NoInteraction(), NewtonDamped(damping_constant=1.0))]
+    o = O.OracleSim(spec, dt=0.01, mechanics_solver_order=1)
+    o.set_agents(np.array([[1.5, 1.5], [2.5, 1.5], [2.5, 2.5]]))
+    assert o.neighbor_counts(1.0).tolist() == [0, 0, 0]
+    assert o.neighbor_counts(math.nextafter(1.0, 2.0)).tolist() == [1, 2, 1]
+    assert o.neighbor_counts(math.nextafter(math.sqrt(2.0), 2.0)).tolist() == [2, 2, 2]
+    o.close()
