@@ -173,16 +173,16 @@ def workload_table():
     return {
         "c2": (lambda a: W.c2_sorting_3d(n=a.n or 1_000_000, seed=2, steps=a.steps), True,
                "3-D two-species cell sorting, Langevin, N=1,000,000, volume fraction 0.3",
-               "k_force_tiled<3, 3, true>"),
+               "k_force_tiled<3, 3, true, false>"),
         "c1": (lambda a: W.c1_morse_2d(n=a.n or 1000, seed=1, steps=a.steps), True,
-               "2-D Morse NewtonDamped, N=1000 in 250x250, Euler", "k_force_tiled<2, 0, true>"),
+               "2-D Morse NewtonDamped, N=1000 in 250x250, Euler", "k_force_tiled<2, 0, true, false>"),
         "c3": (lambda a: W.c3_rods_2d(n=a.n or 200_000, seed=3, steps=a.steps), True,
                "2-D bacterial rods, 200,000 rods x 8 vertices in 2000x2000, AB3", "k_force_rod<2, 0, true>"),
         "c4": (lambda a: W.c4_ensemble_2d(n_instances=a.n or 512, seed=4, steps=a.steps), True,
                "512-instance 2-D Morse ensemble (500..20000 agents each), Euler, one context",
-               "k_force_tiled<2, 0, true>"),
+               "k_force_tiled<2, 0, true, false>"),
         "c5": (lambda a: W.c5_mie_3d(n=a.n or 4_000_000, seed=5, steps=a.steps), True,
-               "3-D Mie lattice, N=4,000,000, AB3", "k_force_tiled<3, 1, true>"),
+               "3-D Mie lattice, N=4,000,000, AB3", "k_force_tiled<3, 1, true, false>"),
     }
 
 

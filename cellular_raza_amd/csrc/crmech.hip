@@ -292,12 +292,21 @@ crm_status launch_force_dp(crm_ctx* ctx, StateBuf S) {
     const uint32_t phys = ctx->cfg.weighting == CRM_WEIGHT_PHYSICAL;
     const uint32_t sym = ctx->symmetric;
     const uint32_t no_lds = (ctx->cfg.flags & CRM_FLAG_NO_LDS) ? 1u : 0u;
-    if (ctx->cfg.precision == CRM_PRECISION_MIXED)
-        hipLaunchKernelGGL((k_force_tiled<D, POT, true>), dim3(ctx->n_force_tiles), dim3(kBlock), 0, ctx->stream, A,
-                           S, ctx->d_tiles, phys, sym, no_lds);
+    const dim3 g(ctx->n_force_tiles), b(kBlock);
+    // sensing has kernels of its own, so the force-only kernels carry none of it
+    const bool mixed = ctx->cfg.precision == CRM_PRECISION_MIXED, sense = A.sense != nullptr;
+    if (mixed && sense)
+        hipLaunchKernelGGL((k_force_tiled<D, POT, true, true>), g, b, 0, ctx->stream, A, S, ctx->d_tiles, phys, sym,
+                           no_lds);
+    else if (mixed)
+        hipLaunchKernelGGL((k_force_tiled<D, POT, true, false>), g, b, 0, ctx->stream, A, S, ctx->d_tiles, phys, sym,
+                           no_lds);
+    else if (sense)
+        hipLaunchKernelGGL((k_force_tiled<D, POT, false, true>), g, b, 0, ctx->stream, A, S, ctx->d_tiles, phys, sym,
+                           no_lds);
     else
-        hipLaunchKernelGGL((k_force_tiled<D, POT, false>), dim3(ctx->n_force_tiles), dim3(kBlock), 0, ctx->stream, A,
-                           S, ctx->d_tiles, phys, sym, no_lds);
+        hipLaunchKernelGGL((k_force_tiled<D, POT, false, false>), g, b, 0, ctx->stream, A, S, ctx->d_tiles, phys, sym,
+                           no_lds);
     return CRM_OK;
 }
 

@@ -284,7 +284,7 @@ struct Row4 {
 // exactly zero, so the sums are those of the full loop without its zero terms.
 // The wave pays max-over-lanes of the marked count for the expensive pass
 // instead of max-over-lanes of the candidate count.
-template <int D, int POT, bool MIXED, typename Src>
+template <int D, int POT, bool MIXED, bool SENSE, typename Src>
 __device__ __forceinline__ void agent_force(const Arrays& A, const StateBuf& S, const InstParams& P, uint64_t s,
                                             uint32_t self, const typename RecOf<MIXED>::type me, Span ov,
                                             typename Src::W cur, uint32_t physical, uint32_t symmetric,
@@ -336,11 +336,11 @@ __device__ __forceinline__ void agent_force(const Arrays& A, const StateBuf& S, 
     // Mixed precision counts from the fixed-point distance outside a band that
     // bounds its error (quantisation of both ends plus float rounding) and
     // decides the band's pairs from the FP64 positions.
-    const bool sense = A.sense != nullptr;
+    constexpr bool sense = SENSE;
     uint32_t n_sense = 0;
     float mark_f = thr_f, rs2 = 0.0f, band_s = 0.0f;
     double mark_d = thr_d;
-    if (sense) {
+    if constexpr (SENSE) {
         if constexpr (MIXED) {
             const float umax = fmaxf(fmaxf(unit[0], unit[1]), unit[D - 1]);
             const float q = 2.0f * (float)D * umax;
@@ -353,6 +353,7 @@ __device__ __forceinline__ void agent_force(const Arrays& A, const StateBuf& S, 
             mark_d = fmax(thr_d, A.sense_t * (1.0 + 1e-12));
         }
     }
+    const bool wider = MIXED ? mark_f > thr_f : mark_d > thr_d;
 
     // species-matrix row (own species) and column (own species as ext)
     Row4<T> srow, scol;
@@ -434,7 +435,7 @@ __device__ __forceinline__ void agent_force(const Arrays& A, const StateBuf& S, 
         const Rec rj = src.record(j, w);
         T z[D];
         const T d2 = displacement(rj, z);
-        if (sense) {
+        if constexpr (SENSE) {
             bool in;
             if constexpr (MIXED) {
                 if (flag_i || rj.w < 0 || fabsf(d2 - rs2) <= band_s) {
@@ -450,12 +451,15 @@ __device__ __forceinline__ void agent_force(const Arrays& A, const StateBuf& S, 
                 in = sum_squares_unfused<D>(z) < A.sense_t;
             }
             n_sense += in ? 1u : 0u;
-            // the force only for the pairs pass 1 marks without sensing
+            // the force only for the pairs pass 1 marks without sensing (all of
+            // them unless the sensing radius widened the marks)
             if constexpr (POT == CRM_POTENTIAL_NONE) return;
-            bool fmark;
-            if constexpr (MIXED) fmark = mark_dist(rj) <= thr_f || rj.w < 0;
-            else fmark = mark_dist(rj) <= thr_d;
-            if (!fmark) return;
+            if (wider) {
+                bool fmark;
+                if constexpr (MIXED) fmark = mark_dist(rj) <= thr_f || rj.w < 0;
+                else fmark = mark_dist(rj) <= thr_d;
+                if (!fmark) return;
+            }
         }
         const bool full = physical || (j >= ov.b && j < ov.e);
         int sp_j;
@@ -545,7 +549,7 @@ __device__ __forceinline__ void agent_force(const Arrays& A, const StateBuf& S, 
     if (calc_err) raise_error(A.ctrl, CRM_CALC_ERROR, A.ctrl->step, S.id[s]);
 #pragma unroll
     for (int k = 0; k < D; k++) A.force[(uint64_t)k * N + s] = MIXED ? (double)ks[k] + (double)(-kc[k]) : accd[k];
-    if (sense) A.sense[s] = n_sense;
+    if constexpr (SENSE) A.sense[s] = n_sense;
 }
 
 // A force tile: plane a0, rows [b0, b1), columns [c0, c1) of one instance.
@@ -600,7 +604,7 @@ __device__ __forceinline__ uint32_t small_div(uint32_t a, float inv_d) {
 // starts) in LDS, then one lane per own agent sums its forces from LDS.  A tile
 // whose region does not fit (local crowding) reads global memory instead; the
 // candidate order and the arithmetic are the same.
-template <int D, int POT, bool MIXED>
+template <int D, int POT, bool MIXED, bool SENSE>
 __global__ __launch_bounds__(kBlock, MIXED ? 8 : 4) void k_force_tiled(Arrays A, StateBuf S,
                                                                      const ForceTile* __restrict__ tiles,
                                                                      uint32_t physical, uint32_t symmetric,
@@ -761,7 +765,7 @@ __global__ __launch_bounds__(kBlock, MIXED ? 8 : 4) void k_force_tiled(Arrays A,
                 Span ov;
                 Walk3 w;
                 const uint32_t s = own_agent(t, self, ov, w);
-                agent_force<D, POT, MIXED>(A, S, P, s, self, lds[self], ov, w, physical, symmetric, src, unit);
+                agent_force<D, POT, MIXED, SENSE>(A, S, P, s, self, lds[self], ov, w, physical, symmetric, src, unit);
             }
             return;
         }
@@ -792,7 +796,7 @@ __global__ __launch_bounds__(kBlock, MIXED ? 8 : 4) void k_force_tiled(Arrays A,
             WalkVox wm = w;  // the own record: its voxel (a0, b, c) in region coordinates
             wm.pa = T.a0 - w.a_lo;
             wm.rb = b - w.b_lo;
-            agent_force<D, POT, MIXED>(A, S, P, s, s, src.record(s, wm), ov, w, physical, symmetric, src, unit);
+            agent_force<D, POT, MIXED, SENSE>(A, S, P, s, s, src.record(s, wm), ov, w, physical, symmetric, src, unit);
         }
     }
 }

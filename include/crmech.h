@@ -246,8 +246,11 @@ crm_status crm_get_info(crm_ctx* ctx, crm_info* info);
  * own agents, density margin of the staged region) and keeps the fastest.  The
  * choice is cached per workload key (dimension, precision, potential, weighting,
  * instance count, agent count, grid) for the process and, when cache_path is not
- * NULL, in that JSON file, which is read first and rewritten.  Forces do not
- * depend on the tiling.  chosen (may be NULL) receives the policy index. */
+ * NULL, in that JSON file, which is read first and rewritten.  FP64 forces do
+ * not depend on the tiling; mixed-precision forces quantise positions with the
+ * tile region's fixed-point step and so move with the tiling at that step's
+ * level (reproducible for one tiling).  chosen (may be NULL) receives the policy
+ * index. */
 crm_status crm_autotune(crm_ctx* ctx, const char* cache_path, uint32_t* chosen);
 
 /* Force tiling of instance inst: voxel rows and columns per tile, tile count. */

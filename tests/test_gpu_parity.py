@@ -410,9 +410,11 @@ def test_full_size_c2_invariants():
 
 
 def test_autotune_keeps_results_and_caches(tmp_path):
-    """The tiling autotuner changes only the work decomposition: forces and states are
-    bitwise those of the default tiling; its choice is written to and read from the
-    JSON cache."""
+    """The tiling autotuner changes only the work decomposition.  FP64 forces and
+    states are bitwise those of the default tiling.  Mixed precision quantises
+    positions with the tile region's fixed-point step, so its forces move with the
+    tiling at that step's level (far inside the mixed bar); for one tiling they are
+    bitwise reproducible.  The choice is written to and read from the JSON cache."""
     wl = workloads.c2_sorting_3d(n=60000, seed=40, steps=5)
     for precision in ("mixed", "fp64"):
         ref = MechanicsBackend(wl.instances, dt=wl.dt, precision=precision, noise="philox", seed=2)
@@ -425,10 +427,21 @@ def test_autotune_keeps_results_and_caches(tmp_path):
         assert f"precision={1 if precision == 'mixed' else 0}" in path.read_text()
         t = tuned.tiling()
         assert t["rows"] >= 1 and t["cols"] >= 1 and t["tiles"] >= 1
-        assert np.array_equal(tuned.compute_forces(), ref.compute_forces())
-        tuned.step(5)
-        ref.step(5)
-        assert np.array_equal(tuned.get_state()[0], ref.get_state()[0])
-        again = MechanicsBackend(wl.instances, dt=wl.dt, precision=precision, autotune=str(path))
+        f_tuned, f_ref = tuned.compute_forces(), ref.compute_forces()
+        if precision == "fp64":
+            assert np.array_equal(f_tuned, f_ref)
+        else:
+            assert rel_err(f_tuned, f_ref) <= 1e-6
+        again = MechanicsBackend(wl.instances, dt=wl.dt, precision=precision, noise="philox", seed=2,
+                                 autotune=str(path))
         again.set_agents(wl.pos, wl.vel, wl.species)
         assert again.tiling_policy == tuned.tiling_policy
+        assert np.array_equal(again.compute_forces(), f_tuned)
+        tuned.step(5)
+        ref.step(5)
+        again.step(5)
+        assert np.array_equal(again.get_state()[0], tuned.get_state()[0])
+        if precision == "fp64":
+            assert np.array_equal(tuned.get_state()[0], ref.get_state()[0])
+        for h in (ref, tuned, again):
+            h.close()
