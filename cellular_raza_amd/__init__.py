@@ -32,3 +32,10 @@ from .simulation import (  # noqa: F401
     SimulationError,
     run_simulation,
 )
+from .storage import (  # noqa: F401
+    StorageAccess,
+    StorageBuilder,
+    StorageError,
+    StorageManager,
+    StorageOption,
+)

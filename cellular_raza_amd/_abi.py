@@ -15,6 +15,7 @@ CRM_RNG_ERROR = 4
 CRM_SETUP_ERROR = 5
 CRM_DEVICE_ERROR = 6
 CRM_INVALID_ARGUMENT = 7
+CRM_STORAGE_ERROR = 8
 
 STATUS_NAMES = {
     CRM_OK: "Ok",
@@ -25,6 +26,7 @@ STATUS_NAMES = {
     CRM_SETUP_ERROR: "SetupError",
     CRM_DEVICE_ERROR: "DeviceError",
     CRM_INVALID_ARGUMENT: "InvalidArgument",
+    CRM_STORAGE_ERROR: "StorageError",
 }
 
 # crm_potential_kind
@@ -132,6 +134,14 @@ class Timing(C.Structure):
     _fields_ = [("total_ms", C.c_double * 6), ("launches", C.c_uint64 * 6)]
 
 
+class JsonCells(C.Structure):
+    _fields_ = [
+        ("cell_templates", C.POINTER(C.c_char_p)),
+        ("n_templates", C.c_uint32),
+        ("storage_instance", C.c_uint32),
+    ]
+
+
 # Every symbol include/crmech.h declares, with (restype, argtypes).
 P = C.c_void_p
 DP = C.POINTER(C.c_double)
@@ -154,6 +164,8 @@ SIGNATURES = {
                                  C.POINTER(C.c_uint32)]),
     "crm_set_neighbor_sensing": (C.c_int, [P, C.c_uint32, C.c_double]),
     "crm_get_neighbor_counts": (C.c_int, [P, U32P]),
+    "crm_get_snapshot": (C.c_int, [P, DP, C.POINTER(C.c_int32), U32P]),
+    "crm_save_cells_json": (C.c_int, [P, C.c_char_p, C.c_uint64, C.POINTER(JsonCells)]),
     "crm_get_timing": (C.c_int, [P, C.POINTER(Timing)]),
     "crm_reset_timing": (C.c_int, [P]),
     "crm_last_error": (C.c_size_t, [P, C.c_char_p, C.c_size_t]),

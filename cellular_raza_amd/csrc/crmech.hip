@@ -7,17 +7,21 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <charconv>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
-#include <string>
 #include <cstdlib>
+#include <filesystem>
 #include <map>
 #include <mutex>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "crmech.h"
 #include "kernels_rod.hpp"
+#include "kernels_snapshot.hpp"
 
 namespace crm {
 
@@ -172,6 +176,10 @@ struct crm_ctx {
     double sense_t = 0.0;
     uint32_t* d_sense = nullptr;
     int sense_parity = -1;
+    // snapshot records (agent order) and species, allocated on the first save
+    double* d_snap = nullptr;
+    int32_t* d_snap_sp = nullptr;
+    size_t snap_cap = 0;
 
     uint64_t N = 0;
     std::vector<uint64_t> counts;
@@ -196,7 +204,7 @@ thread_local std::string g_create_error;
 void free_device(crm_ctx* ctx) {
     void* ptrs[] = {ctx->d_work, ctx->d_tiles, ctx->d_start, ctx->d_count, ctx->d_bsum, ctx->d_key, ctx->d_rank,
                     ctx->d_perm, ctx->d_force, ctx->d_rec64, ctx->d_recq, ctx->d_noise, ctx->d_recrod, ctx->d_bnd,
-                    ctx->d_sense};
+                    ctx->d_sense, ctx->d_snap, ctx->d_snap_sp};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& b : ctx->buf) {
@@ -217,6 +225,9 @@ void free_device(crm_ctx* ctx) {
     ctx->d_bnd = nullptr;
     ctx->d_sense = nullptr;
     ctx->sense_parity = -1;
+    ctx->d_snap = nullptr;
+    ctx->d_snap_sp = nullptr;
+    ctx->snap_cap = 0;
     ctx->noise_cap = 0;
     for (auto& g : ctx->graph)
         if (g) {
@@ -716,6 +727,150 @@ crm_status build_graph(crm_ctx* ctx, int parity) {
     return CRM_OK;
 }
 
+// History entries stored after ctx->step steps and their ring slots, oldest
+// first (the ring slot of push t is t % capacity).
+uint32_t history_slots(const crm_ctx* ctx, uint32_t slot[2]) {
+    const uint32_t cap = ctx->cfg.solver_order;
+    const uint64_t t = ctx->step;
+    const uint32_t size = cap == 0 ? 0u : (uint32_t)std::min<uint64_t>(t, cap);
+    const uint32_t oldest = (size < cap || cap == 0) ? 0u : (uint32_t)(t % cap);
+    for (uint32_t i = 0; i < 2; i++) slot[i] = cap ? (oldest + i) % cap : 0u;
+    return size;
+}
+
+// Gathers the snapshot records (kernels_snapshot.hpp) into agent order on the
+// device and copies them to host vectors.
+crm_status gather_snapshot(crm_ctx* ctx, std::vector<double>& rec, std::vector<int32_t>& species, uint32_t& n_hist) {
+    SnapshotArgs a{};
+    a.N = ctx->N;
+    a.K = ctx->K;
+    a.n_hist = history_slots(ctx, a.slot);
+    n_hist = a.n_hist;
+    const size_t len = (size_t)ctx->N * ctx->K * (2u + 2u * a.n_hist);
+    if (len > ctx->snap_cap) {
+        if (ctx->d_snap) CRM_HIP(hipFree(ctx->d_snap));
+        ctx->d_snap = nullptr;
+        ctx->snap_cap = 0;
+        CRM_TRY(dalloc(ctx, ctx->d_snap, len));
+        ctx->snap_cap = len;
+    }
+    if (!ctx->d_snap_sp) CRM_TRY(dalloc(ctx, ctx->d_snap_sp, ctx->N));
+    a.out = ctx->d_snap;
+    a.species = ctx->d_snap_sp;
+    hipLaunchKernelGGL(k_snapshot, dim3((uint32_t)((ctx->N + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream,
+                       ctx->buf[ctx->cur], a);
+    CRM_HIP(hipGetLastError());
+    rec.resize(len);
+    species.resize(ctx->N);
+    CRM_HIP(hipMemcpyAsync(rec.data(), ctx->d_snap, len * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    CRM_HIP(hipMemcpyAsync(species.data(), ctx->d_snap_sp, ctx->N * sizeof(int32_t), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    CRM_HIP(hipStreamSynchronize(ctx->stream));
+    return CRM_OK;
+}
+
+// A double as serde_json writes an f64: shortest round-trip digits, a fraction
+// or an exponent always present, the exponent without '+' or leading zeros;
+// non-finite values become null.
+void put_num(std::string& o, double x) {
+    if (!std::isfinite(x)) {
+        o += "null";
+        return;
+    }
+    char b[48];
+    const auto r = std::to_chars(b, b + sizeof b, x);
+    const char* e = std::find(b, r.ptr, 'e');
+    if (e == r.ptr) {
+        o.append(b, (size_t)(r.ptr - b));
+        if (std::find(b, r.ptr, '.') == r.ptr) o += ".0";
+        return;
+    }
+    o.append(b, (size_t)(e - b));
+    o += 'e';
+    const char* q = e + 1;
+    if (*q == '+') q++;
+    else if (*q == '-') o += *q++;
+    while (q + 1 < r.ptr && *q == '0') q++;
+    o.append(q, (size_t)(r.ptr - q));
+}
+
+// A position-like value: nalgebra SVector (points) serialises as [x, y, z];
+// the rods' MatrixXx{D} as [column-major data, rows, columns].
+void put_vec(std::string& o, const double* v, uint32_t nv, uint32_t D, bool matrix) {
+    if (!matrix) {
+        o += '[';
+        for (uint32_t k = 0; k < D; k++) {
+            if (k) o += ", ";
+            put_num(o, v[k]);
+        }
+        o += ']';
+        return;
+    }
+    o += "[[";
+    for (uint32_t k = 0; k < D; k++)
+        for (uint32_t i = 0; i < nv; i++) {
+            if (k || i) o += ", ";
+            put_num(o, v[i * D + k]);
+        }
+    o += "], " + std::to_string(nv) + ", " + std::to_string(D) + "]";
+}
+
+// A cell template split at its {pos} / {vel} placeholders.
+struct Template {
+    std::vector<std::string> lit;  // lit[0] hole[0] lit[1] hole[1] ... lit[n]
+    std::vector<int> hole;         // 0 pos, 1 vel
+};
+
+Template split_template(const char* t) {
+    Template T;
+    std::string cur;
+    for (const char* p = t; *p;) {
+        if (std::strncmp(p, "{pos}", 5) == 0 || std::strncmp(p, "{vel}", 5) == 0) {
+            T.lit.push_back(cur);
+            cur.clear();
+            T.hole.push_back(p[1] == 'p' ? 0 : 1);
+            p += 5;
+        } else {
+            cur += *p++;
+        }
+    }
+    T.lit.push_back(cur);
+    return T;
+}
+
+// One data entry of the batch file: CombinedSaveFormat { identifier, element:
+// (CellBox { identifier, parent, cell }, AuxStorage { mechanics }) }
+// (storage/concepts.rs:176-190, cell.rs:128-137, aux_storage.rs:41-46).
+void put_cell(std::string& o, uint64_t a, const double* r, const Template& T, uint32_t nv, uint32_t D,
+              uint32_t n_hist) {
+    const uint32_t K = nv * D;
+    const bool matrix = nv > 1;
+    const std::string ident = "{\"Initial\": " + std::to_string(a) + "}";
+    o += "    {\n      \"identifier\": " + ident + ",\n      \"element\": [\n        {\n          \"identifier\": " +
+         ident + ",\n          \"parent\": null,\n          \"cell\": ";
+    for (size_t i = 0; i < T.hole.size(); i++) {
+        o += T.lit[i];
+        put_vec(o, r + (T.hole[i] == 0 ? 0u : K), nv, D, matrix);
+    }
+    o += T.lit.back();
+    o += "\n        },\n        {\n          \"mechanics\": {\n            \"positions\": [";
+    for (uint32_t i = 0; i < n_hist; i++) {
+        if (i) o += ", ";
+        put_vec(o, r + 2u * K + i * K, nv, D, matrix);
+    }
+    o += "],\n            \"velocities\": [";
+    for (uint32_t i = 0; i < n_hist; i++) {
+        if (i) o += ", ";
+        put_vec(o, r + 2u * K + (n_hist + i) * K, nv, D, matrix);
+    }
+    std::vector<double> zero(K, 0.0);
+    o += "],\n            \"current_force\": ";
+    put_vec(o, zero.data(), nv, D, matrix);
+    o += ",\n            \"zero_force\": ";
+    put_vec(o, zero.data(), nv, D, matrix);
+    o += "\n          }\n        }\n      ]\n    }";
+}
+
 }  // namespace
 
 extern "C" {
@@ -1064,6 +1219,8 @@ crm_status crm_get_voxels(crm_ctx* ctx, uint32_t* voxel_of_agent, uint32_t* voxe
     return CRM_OK;
 }
 
+}  // extern "C"
+
 // ---- tiling autotuner ------------------------------------------------------
 namespace {
 
@@ -1127,6 +1284,8 @@ crm_status apply_policy(crm_ctx* ctx, uint32_t pol) {
 }
 
 }  // namespace
+
+extern "C" {
 
 crm_status crm_autotune(crm_ctx* ctx, const char* cache_path, uint32_t* chosen) {
     if (!ctx || !ctx->have_agents) return CRM_INVALID_ARGUMENT;
@@ -1235,6 +1394,97 @@ crm_status crm_get_neighbor_counts(crm_ctx* ctx, uint32_t* counts) {
         uint32_t n = 0;
         for (uint64_t v = 0; v < nv; v++) n += c[s * nv + v];
         counts[id[s]] = n;
+    }
+    return CRM_OK;
+}
+
+crm_status crm_get_snapshot(crm_ctx* ctx, double* records, int32_t* species, uint32_t* n_hist) {
+    if (!ctx || !ctx->have_agents) return CRM_INVALID_ARGUMENT;
+    CRM_HIP(hipSetDevice(ctx->device));
+    std::vector<double> rec;
+    std::vector<int32_t> sp;
+    uint32_t nh = 0;
+    CRM_TRY(gather_snapshot(ctx, rec, sp, nh));
+    if (records) std::memcpy(records, rec.data(), rec.size() * sizeof(double));
+    if (species) std::memcpy(species, sp.data(), sp.size() * sizeof(int32_t));
+    if (n_hist) *n_hist = nh;
+    return CRM_OK;
+}
+
+crm_status crm_save_cells_json(crm_ctx* ctx, const char* path, uint64_t iteration, const crm_json_cells* spec) {
+    if (!ctx || !path || !spec || !spec->cell_templates || spec->n_templates == 0) return CRM_INVALID_ARGUMENT;
+    if (!ctx->have_agents) return CRM_INVALID_ARGUMENT;
+    std::vector<Template> T;
+    for (uint32_t i = 0; i < spec->n_templates; i++) {
+        if (!spec->cell_templates[i]) {
+            ctx->err = "cell template " + std::to_string(i) + " is NULL";
+            return CRM_INVALID_ARGUMENT;
+        }
+        T.push_back(split_template(spec->cell_templates[i]));
+    }
+    CRM_HIP(hipSetDevice(ctx->device));
+    std::vector<double> rec;
+    std::vector<int32_t> sp;
+    uint32_t nh = 0;
+    CRM_TRY(gather_snapshot(ctx, rec, sp, nh));
+    const uint64_t N = ctx->N;
+    if (spec->n_templates > 1)
+        for (uint64_t a = 0; a < N; a++)
+            if (sp[a] < 0 || (uint32_t)sp[a] >= spec->n_templates) {
+                ctx->err = "agent " + std::to_string(a) + " has species " + std::to_string(sp[a]) +
+                           " but only " + std::to_string(spec->n_templates) + " cell templates were given";
+                return CRM_INVALID_ARGUMENT;
+            }
+    // <path>/<iteration, 20 digits>/batch_<instance, 20 digits>_<first free counter, 20 digits>.json
+    // (storage/concepts.rs:702-740)
+    namespace fs = std::filesystem;
+    char name[96];
+    std::snprintf(name, sizeof name, "%020llu", (unsigned long long)iteration);
+    const fs::path dir = fs::path(path) / name;
+    std::error_code ec;
+    fs::create_directories(dir, ec);
+    if (ec) {
+        ctx->err = "cannot create " + dir.string() + ": " + ec.message();
+        return CRM_STORAGE_ERROR;
+    }
+    fs::path file;
+    for (uint64_t c = 0;; c++) {
+        std::snprintf(name, sizeof name, "batch_%020llu_%020llu.json", (unsigned long long)spec->storage_instance,
+                      (unsigned long long)c);
+        file = dir / name;
+        if (!fs::exists(file)) break;
+    }
+    // format in parallel over contiguous agent ranges
+    const uint32_t nv = ctx->nv, D = ctx->D;
+    const size_t rec_len = (size_t)ctx->K * (2u + 2u * nh);
+    const uint32_t n_thr = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>({16, N / 4096 + 1,
+                                                          std::max(1u, std::thread::hardware_concurrency())}));
+    std::vector<std::string> part(n_thr);
+    auto work = [&](uint32_t t) {
+        const uint64_t a0 = N * t / n_thr, a1 = N * (t + 1) / n_thr;
+        std::string& o = part[t];
+        o.reserve((a1 - a0) * (400 + 60 * rec_len));
+        for (uint64_t a = a0; a < a1; a++) {
+            if (a) o += ",\n";
+            put_cell(o, a, rec.data() + a * rec_len, T[spec->n_templates > 1 ? sp[a] : 0], nv, D, nh);
+        }
+    };
+    std::vector<std::thread> pool;
+    for (uint32_t t = 1; t < n_thr; t++) pool.emplace_back(work, t);
+    work(0);
+    for (auto& th : pool) th.join();
+    FILE* f = std::fopen(file.string().c_str(), "wb");
+    if (!f) {
+        ctx->err = "cannot open " + file.string() + " for writing";
+        return CRM_STORAGE_ERROR;
+    }
+    bool ok = std::fputs("{\n  \"data\": [\n", f) >= 0;
+    for (const auto& p : part) ok = ok && std::fwrite(p.data(), 1, p.size(), f) == p.size();
+    ok = ok && std::fputs("\n  ]\n}", f) >= 0;
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok) {
+        ctx->err = "write to " + file.string() + " failed";
+        return CRM_STORAGE_ERROR;
     }
     return CRM_OK;
 }

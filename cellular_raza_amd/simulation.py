@@ -9,6 +9,7 @@ with `FixedStepsize` save points (cellular_raza-core/src/time.rs:93-265).
 from __future__ import annotations
 
 import ctypes as C
+import os
 from dataclasses import dataclass
 from typing import Dict, List, Optional, Sequence
 
@@ -218,6 +219,27 @@ class MechanicsBackend:
         self._check(self._lib.crm_get_neighbor_counts(self._h, out.ctypes.data_as(_abi.U32P)))
         return out
 
+    def snapshot(self) -> Dict[str, np.ndarray]:
+        """Agent-order snapshot gathered on the device (crm_get_snapshot): pos and vel
+        (n_agents, n_vertices * dim), dx / dv history (n_agents, n_hist, n_vertices * dim)
+        oldest first, species."""
+        nh = C.c_uint32()
+        self._check(self._lib.crm_get_snapshot(self._h, None, None, C.byref(nh)))
+        n, K, h = self.n_agents, self.K, int(nh.value)
+        rec = np.empty((n, K * (2 + 2 * h)), dtype=np.float64)
+        sp = np.empty(n, dtype=np.int32)
+        self._check(self._lib.crm_get_snapshot(self._h, rec.ctypes.data_as(_abi.DP),
+                                               sp.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(nh)))
+        return {"pos": rec[:, :K], "vel": rec[:, K:2 * K], "dx": rec[:, 2 * K:(2 + h) * K].reshape(n, h, K),
+                "dv": rec[:, (2 + h) * K:].reshape(n, h, K), "species": sp}
+
+    def save_cells_json(self, path: str, iteration: int, templates: Sequence[str], instance: int = 0) -> None:
+        """Writes the cells in the reference's JSON batch layout (crm_save_cells_json)."""
+        enc = [t.encode() for t in templates]
+        arr = (C.c_char_p * len(enc))(*enc)
+        spec = _abi.JsonCells(C.cast(arr, C.POINTER(C.c_char_p)), len(enc), instance)
+        self._check(self._lib.crm_save_cells_json(self._h, str(path).encode(), int(iteration), C.byref(spec)))
+
     def timing(self) -> Dict[str, Dict[str, float]]:
         t = _abi.Timing()
         self._check(self._lib.crm_get_timing(self._h, C.byref(t)))
@@ -265,6 +287,7 @@ class Settings:
     time: FixedStepsize
     n_threads: int = 1
     progressbar: Optional[str] = None
+    storage: Optional[object] = None  # storage.StorageBuilder: store cells and subdomains
 
 
 def run_simulation(
@@ -278,8 +301,11 @@ def run_simulation(
     mechanics_solver_order: int = 2,
     **backend_kwargs,
 ) -> Dict[int, np.ndarray]:
-    """Runs the [Mechanics, Interaction] aspects and returns {iteration: positions}
-    at the save iterations (iteration 0 is the initial state, run_sim.rs:363-367)."""
+    """Runs the [Mechanics, Interaction] aspects.  Without settings.storage it returns
+    {iteration: positions} at the save iterations (iteration 0 is the initial state,
+    run_sim.rs:363-367); with a StorageBuilder it stores the cells and the subdomain
+    at those iterations as run_simulation! does (run_sim.rs:375-378,397-398,430-445)
+    and returns the StorageAccess."""
     be = MechanicsBackend(
         [InstanceSpec(domain, interaction, mechanics)],
         dt=settings.time.dt,
@@ -289,13 +315,62 @@ def run_simulation(
     try:
         be.set_agents(positions, velocities, species)
         out: Dict[int, np.ndarray] = {}
+        access = None
+        if settings.storage is not None:
+            from . import storage as st
+
+            builder = settings.storage if settings.storage.initialized else settings.storage.init()
+            base = builder.get_suffix()
+            access = st.StorageAccess(
+                cells=st.StorageManager.open_or_create(builder.suffix(os.path.join(base, "cells")), 0),
+                subdomains=st.StorageManager.open_or_create(builder.suffix(os.path.join(base, "subdomains")), 0),
+            )
+            n_sp = int(np.max(species)) + 1 if species is not None and len(species) else 1
+            templates = st.cell_templates(mechanics, interaction, n_species=n_sp)
+            sub = st.subdomain_element(domain)
         it = 0
         for target in settings.time.save_iterations:
             target = min(target, settings.time.maximum_iterations)
             if target > it:
                 be.step(target - it)
                 it = target
-            out[it] = be.get_state()[0]
-        return out
+            if access is None:
+                out[it] = be.get_state()[0]
+                continue
+            _store_cells(be, access.cells, it, templates)
+            access.subdomains.store_single_element(it, 0, sub)
+        return out if access is None else access
     finally:
         be.close()
+
+
+def _store_cells(be: MechanicsBackend, cells, iteration: int, templates: Sequence[str]) -> None:
+    """Cells of one save point into every option of the manager: JSON through the
+    library's writer, Memory as Python objects of the same structure."""
+    from . import storage as st
+
+    for opt, store in cells._stores.items():
+        if opt is st.StorageOption.SerdeJson:
+            be.save_cells_json(store.path, iteration, templates, cells.instance)
+            continue
+        snap = be.snapshot()
+        nv, dim = be.n_vertices, be.dim
+        matrix = nv > 1
+
+        def shaped(x):
+            return x.reshape(nv, dim) if matrix else x
+
+        def vec(x):  # SVector: [x, y, z]; dynamic matrix: [column-major data, rows, columns]
+            return [x.reshape(nv, dim).T.ravel().tolist(), nv, dim] if matrix else x.tolist()
+
+        zero = vec(np.zeros(be.K))
+        pairs = []
+        for a in range(be.n_agents):
+            t = templates[int(snap["species"][a]) if len(templates) > 1 else 0]
+            cellbox = {"identifier": {"Initial": a}, "parent": None,
+                       "cell": st.fill_template(t, shaped(snap["pos"][a]), shaped(snap["vel"][a]), matrix)}
+            aux = {"mechanics": {"positions": [vec(x) for x in snap["dx"][a]],
+                                 "velocities": [vec(x) for x in snap["dv"][a]],
+                                 "current_force": zero, "zero_force": zero}}
+            pairs.append((("Initial", a), [cellbox, aux]))
+        store.store_batch_elements(iteration, pairs)

@@ -51,7 +51,8 @@ typedef enum crm_status {
     CRM_RNG_ERROR = 4,        /* RngError       (mechanics.rs:211-226) */
     CRM_SETUP_ERROR = 5,      /* DecomposeError / TimeError during setup */
     CRM_DEVICE_ERROR = 6,     /* HIP runtime failure */
-    CRM_INVALID_ARGUMENT = 7  /* caller error: bad sizes, null pointers, unsupported combination */
+    CRM_INVALID_ARGUMENT = 7, /* caller error: bad sizes, null pointers, unsupported combination */
+    CRM_STORAGE_ERROR = 8     /* StorageError (storage/concepts.rs:16-42): a snapshot file could not be written */
 } crm_status;
 
 /* Interaction kinds (cellular_raza-building-blocks/src/cell_building_blocks/interaction.rs). */
@@ -274,6 +275,36 @@ crm_status crm_set_neighbor_sensing(crm_ctx* ctx, uint32_t enable, double radius
  * when no force evaluation with sensing on has run since it was enabled or the
  * agents were set. */
 crm_status crm_get_neighbor_counts(crm_ctx* ctx, uint32_t* counts);
+
+/* Snapshot of the current state in agent-index order, gathered on the device in
+ * one kernel and one copy.  Replaces the per-save-point walk of every voxel's
+ * cells in SubDomainBox::save_cells (datastructures.rs:371-396).  records[N *
+ * K * (2 + 2 * n_hist)] (may be NULL), K = n_vertices * dim, per agent:
+ *   pos[K], vel[K], dx history[n_hist][K], dv history[n_hist][K]
+ * with the history oldest first (AuxStorageMechanics positions / velocities
+ * rings, aux_storage.rs:41-46,107-115); species[N] (may be NULL); n_hist
+ * receives the number of stored history entries (0..2), which the caller can
+ * size records with by calling first with records == NULL. */
+crm_status crm_get_snapshot(crm_ctx* ctx, double* records, int32_t* species, uint32_t* n_hist);
+
+/* Cells of one save point written in the reference's JSON storage layout
+ * (JsonStorageInterface, storage/serde_json.rs:10-74, StorageWrapper::
+ * store_batch_elements and the file naming of storage/concepts.rs:702-790):
+ *   <path>/<iteration:020>/batch_<storage_instance:020>_<n:020>.json
+ * with n the first free counter, holding {"data": [CombinedSaveFormat ...]} with
+ * one entry per agent: identifier {"Initial": agent index}, element [CellBox
+ * {identifier, parent: null, cell}, AuxStorage {mechanics: {positions,
+ * velocities, current_force, zero_force}}].  `cell` is the caller's JSON text
+ * of the agent struct with the placeholders {pos} and {vel}, one template per
+ * species (n_templates > 1) or one for all; points write vectors as [x, y, z],
+ * rods as nalgebra's dynamic matrix [column-major data, rows, columns].  path is
+ * the cells' json folder (<storage location>/cells/json). */
+typedef struct crm_json_cells {
+    const char* const* cell_templates;
+    uint32_t n_templates;
+    uint32_t storage_instance;
+} crm_json_cells;
+crm_status crm_save_cells_json(crm_ctx* ctx, const char* path, uint64_t iteration, const crm_json_cells* spec);
 
 crm_status crm_get_timing(crm_ctx* ctx, crm_timing* timing);
 crm_status crm_reset_timing(crm_ctx* ctx);
