@@ -73,6 +73,37 @@ def force_bytes_per_agent(cbar, record_bytes, dim, stencil, force_bytes):
     return record_bytes + 8 * stencil + cbar * record_bytes + force_bytes
 
 
+def time_snapshot(wl, precision):
+    """One save point after the history ring has filled: crm_get_snapshot (device gather
+    + copy of the agent-order records) and crm_save_cells_json (gather + threaded JSON
+    formatting + write), wall clock; the JSON goes to a temporary folder."""
+    import shutil
+    import tempfile
+
+    from cellular_raza_amd import storage as st
+
+    g = make_backend(wl, precision, graph=True, timing=False, seed=1)
+    g.step(4)
+    g.snapshot()  # first call allocates the record buffer
+    t0 = time.perf_counter()
+    snap = g.snapshot()
+    t_gather = time.perf_counter() - t0
+    rec_bytes = sum(v.nbytes for v in snap.values())
+    n_sp = int(wl.species.max()) + 1 if wl.species is not None else 1
+    templates = st.cell_templates(wl.instances[0].mechanics, wl.instances[0].interaction, n_species=n_sp)
+    tmp = tempfile.mkdtemp(prefix="crm_snapshot_")
+    try:
+        t0 = time.perf_counter()
+        g.save_cells_json(tmp, 4, templates)
+        t_json = time.perf_counter() - t0
+        size = sum(os.path.getsize(os.path.join(r, f)) for r, _, fs in os.walk(tmp) for f in fs)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+        g.close()
+    return {"gather_and_copy_s": t_gather, "record_bytes": rec_bytes, "json_write_s": t_json, "json_bytes": size,
+            "json_agents_per_s": wl.n_agents / t_json}
+
+
 def make_backend(wl, precision, graph, timing, seed):
     """Context with the tiling chosen by the library's autotuner (timed on these agents)."""
     from cellular_raza_amd import MechanicsBackend
@@ -197,6 +228,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--seq-steps", type=int, default=100, help="c4: steps of the one-after-another run")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--snapshot", action="store_true",
+                    help="also time one save point: the device gather into agent order with its copy to the host, "
+                         "and the cells written in the reference's JSON layout (to a temporary folder)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy (reference for the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -252,6 +286,9 @@ def main():
         extra["batched_over_sequential"] = value / seq_value
         extra["sequential_sample"] = f"{len(wl.instances)} contexts one after another, {args.seq_steps} steps each"
         print(f"bench: sequential {seq_value:.4e} agent-steps/s", file=sys.stderr, flush=True)
+    if args.snapshot:
+        extra["snapshot"] = time_snapshot(wl, main_prec)
+        print(f"bench: snapshot {extra['snapshot']}", file=sys.stderr, flush=True)
     if args.workload in ("c1", "c3"):
         # force error of the benchmarked path against the oracle on the benchmarked input
         extra["force_rel_err_vs_oracle"] = rel_err(f_main, oracle_forces(wl))

@@ -21,7 +21,9 @@ from .building_blocks import (  # noqa: F401
     NewtonDamped3D,
     NoInteraction,
     RodInteraction,
+    RodGelForce,
     RodMechanics,
+    DomainSpring,
     SpeciesMorsePotential,
 )
 from .simulation import (  # noqa: F401

@@ -101,6 +101,23 @@ class Instance(C.Structure):
     _fields_ = [("domain", Domain), ("potential", Potential), ("mechanics", Mechanics)]
 
 
+DOMAIN_FORCE_NONE = 0
+DOMAIN_FORCE_SPRING = 1
+DOMAIN_FORCE_ROD_GEL = 2
+
+
+class DomainForce(C.Structure):
+    _fields_ = [
+        ("kind", C.c_uint32),
+        ("pad", C.c_uint32),
+        ("strength", C.c_double * 3),
+        ("center", C.c_double * 3),
+        ("gel_pressure", C.c_double),
+        ("surface_friction", C.c_double),
+        ("surface_friction_distance", C.c_double),
+    ]
+
+
 class Config(C.Structure):
     _fields_ = [
         ("dim", C.c_uint32),
@@ -165,6 +182,7 @@ SIGNATURES = {
     "crm_set_neighbor_sensing": (C.c_int, [P, C.c_uint32, C.c_double]),
     "crm_get_neighbor_counts": (C.c_int, [P, U32P]),
     "crm_get_snapshot": (C.c_int, [P, DP, C.POINTER(C.c_int32), U32P]),
+    "crm_set_domain_force": (C.c_int, [P, C.c_uint32, C.POINTER(DomainForce)]),
     "crm_save_cells_json": (C.c_int, [P, C.c_char_p, C.c_uint64, C.POINTER(JsonCells)]),
     "crm_get_timing": (C.c_int, [P, C.POINTER(Timing)]),
     "crm_reset_timing": (C.c_int, [P]),

@@ -293,6 +293,46 @@ Langevin2D = Langevin3D = Langevin
 Brownian2D = Brownian3D = Brownian
 
 
+# --------------------------------------------------------------------------
+# Domain forces (the DomainForce aspect, SubDomainForce)
+# --------------------------------------------------------------------------
+@dataclass
+class DomainSpring:
+    """A linear spring of the domain on every point / rod vertex,
+    f_k = (-strength_k) * (x_k - center_k): the test domain of
+    cellular_raza/tests/domain_force_spring.rs:56-67, per axis."""
+
+    strength: Sequence[float]
+    center: Sequence[float] = (0.0, 0.0, 0.0)
+
+    def to_struct(self) -> _abi.DomainForce:
+        f = _abi.DomainForce()
+        f.kind = _abi.DOMAIN_FORCE_SPRING
+        for k, v in enumerate(self.strength):
+            f.strength[k] = float(v)
+        for k, v in enumerate(self.center):
+            f.center[k] = float(v)
+        return f
+
+
+@dataclass
+class RodGelForce:
+    """CartesianSubDomainRods' gel pressure and surface friction on 3-D rods
+    (cell_building_blocks/bacterial_rods.rs:425-479)."""
+
+    gel_pressure: float
+    surface_friction: float
+    surface_friction_distance: float
+
+    def to_struct(self) -> _abi.DomainForce:
+        f = _abi.DomainForce()
+        f.kind = _abi.DOMAIN_FORCE_ROD_GEL
+        f.gel_pressure = self.gel_pressure
+        f.surface_friction = self.surface_friction
+        f.surface_friction_distance = self.surface_friction_distance
+        return f
+
+
 def instance_struct(domain: CartesianCuboid, interaction, mechanics) -> _abi.Instance:
     inst = _abi.Instance()
     inst.domain = domain.to_struct()

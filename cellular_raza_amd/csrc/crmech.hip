@@ -738,9 +738,9 @@ uint32_t history_slots(const crm_ctx* ctx, uint32_t slot[2]) {
     return size;
 }
 
-// Gathers the snapshot records (kernels_snapshot.hpp) into agent order on the
-// device and copies them to host vectors.
-crm_status gather_snapshot(crm_ctx* ctx, std::vector<double>& rec, std::vector<int32_t>& species, uint32_t& n_hist) {
+// Gathers the snapshot records (kernels_snapshot.hpp) into agent order in
+// d_snap / d_snap_sp; returns their length in doubles.
+crm_status gather_snapshot_device(crm_ctx* ctx, uint32_t& n_hist, size_t& len_out) {
     SnapshotArgs a{};
     a.N = ctx->N;
     a.K = ctx->K;
@@ -760,6 +760,14 @@ crm_status gather_snapshot(crm_ctx* ctx, std::vector<double>& rec, std::vector<i
     hipLaunchKernelGGL(k_snapshot, dim3((uint32_t)((ctx->N + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream,
                        ctx->buf[ctx->cur], a);
     CRM_HIP(hipGetLastError());
+    len_out = len;
+    return CRM_OK;
+}
+
+// The same, copied to host vectors.
+crm_status gather_snapshot(crm_ctx* ctx, std::vector<double>& rec, std::vector<int32_t>& species, uint32_t& n_hist) {
+    size_t len = 0;
+    CRM_TRY(gather_snapshot_device(ctx, n_hist, len));
     rec.resize(len);
     species.resize(ctx->N);
     CRM_HIP(hipMemcpyAsync(rec.data(), ctx->d_snap, len * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -1348,6 +1356,31 @@ crm_status crm_autotune(crm_ctx* ctx, const char* cache_path, uint32_t* chosen) 
     return CRM_OK;
 }
 
+crm_status crm_set_domain_force(crm_ctx* ctx, uint32_t inst, const crm_domain_force* f) {
+    if (!ctx || !f || inst >= ctx->n_inst) return CRM_INVALID_ARGUMENT;
+    if (f->kind > CRM_DOMAIN_FORCE_ROD_GEL) {
+        ctx->err = "unknown domain force kind";
+        return CRM_INVALID_ARGUMENT;
+    }
+    if (f->kind == CRM_DOMAIN_FORCE_ROD_GEL && !(is_rod(ctx) && ctx->D == 3)) {
+        ctx->err = "the rod gel force (CartesianSubDomainRods) acts on 3-D rods only";
+        return CRM_INVALID_ARGUMENT;
+    }
+    InstParams& P = ctx->hparams[inst];
+    P.dforce = f->kind;
+    for (int k = 0; k < 3; k++) {
+        P.dk[k] = f->strength[k];
+        P.dc[k] = f->center[k];
+    }
+    P.gel = f->gel_pressure;
+    P.fric = f->surface_friction;
+    P.fric_d = f->surface_friction_distance;
+    CRM_HIP(hipSetDevice(ctx->device));
+    CRM_HIP(hipStreamSynchronize(ctx->stream));
+    CRM_HIP(hipMemcpy(ctx->d_ip + inst, &P, sizeof(InstParams), hipMemcpyHostToDevice));
+    return CRM_OK;
+}
+
 crm_status crm_set_neighbor_sensing(crm_ctx* ctx, uint32_t enable, double radius) {
     if (!ctx) return CRM_INVALID_ARGUMENT;
     if (enable && std::isnan(radius)) {
@@ -1400,14 +1433,20 @@ crm_status crm_get_neighbor_counts(crm_ctx* ctx, uint32_t* counts) {
 
 crm_status crm_get_snapshot(crm_ctx* ctx, double* records, int32_t* species, uint32_t* n_hist) {
     if (!ctx || !ctx->have_agents) return CRM_INVALID_ARGUMENT;
-    CRM_HIP(hipSetDevice(ctx->device));
-    std::vector<double> rec;
-    std::vector<int32_t> sp;
-    uint32_t nh = 0;
-    CRM_TRY(gather_snapshot(ctx, rec, sp, nh));
-    if (records) std::memcpy(records, rec.data(), rec.size() * sizeof(double));
-    if (species) std::memcpy(species, sp.data(), sp.size() * sizeof(int32_t));
+    uint32_t slot[2];
+    const uint32_t nh = history_slots(ctx, slot);
     if (n_hist) *n_hist = nh;
+    if (!records && !species) return CRM_OK;  // sizing call
+    CRM_HIP(hipSetDevice(ctx->device));
+    uint32_t nh2 = 0;
+    size_t len = 0;
+    CRM_TRY(gather_snapshot_device(ctx, nh2, len));
+    if (records)
+        CRM_HIP(hipMemcpyAsync(records, ctx->d_snap, len * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (species)
+        CRM_HIP(hipMemcpyAsync(species, ctx->d_snap_sp, ctx->N * sizeof(int32_t), hipMemcpyDeviceToHost,
+                               ctx->stream));
+    CRM_HIP(hipStreamSynchronize(ctx->stream));
     return CRM_OK;
 }
 

@@ -547,8 +547,15 @@ __device__ __forceinline__ void agent_force(const Arrays& A, const StateBuf& S, 
         }
     }
     if (calc_err) raise_error(A.ctrl, CRM_CALC_ERROR, A.ctrl->step, S.id[s]);
+    // the domain force follows the pair forces (calculate_custom_domain_force after
+    // the subdomain's pairs, run_sim.rs:277-279, update_mechanics.rs:244-274)
+    const bool spring = P.dforce == CRM_DOMAIN_FORCE_SPRING;
 #pragma unroll
-    for (int k = 0; k < D; k++) A.force[(uint64_t)k * N + s] = MIXED ? (double)ks[k] + (double)(-kc[k]) : accd[k];
+    for (int k = 0; k < D; k++) {
+        double f = MIXED ? (double)ks[k] + (double)(-kc[k]) : accd[k];
+        if (spring) f = f + (-P.dk[k]) * (S.pos[(uint64_t)k * N + s] - P.dc[k]);
+        A.force[(uint64_t)k * N + s] = f;
+    }
     if constexpr (SENSE) A.sense[s] = n_sense;
 }
 

@@ -354,6 +354,29 @@ __global__ __launch_bounds__(kBlock) void k_force_rod(Arrays A, StateBuf S, RodA
         }
     }
     if (calc_err) raise_error(A.ctrl, CRM_CALC_ERROR, A.ctrl->step, S.id[s]);
+    // domain force after the pair forces (run_sim.rs:277-279, update_mechanics.rs:244-274)
+    if (P.dforce == CRM_DOMAIN_FORCE_SPRING) {
+#pragma unroll
+        for (int k = 0; k < D; k++) acc[k] = acc[k] + (-P.dk[k]) * (pv[k] - P.dc[k]);
+    } else if (D == 3 && P.dforce == CRM_DOMAIN_FORCE_ROD_GEL) {
+        // CartesianSubDomainRods::calculate_custom_force (bacterial_rods.rs:447-477)
+        double f[D], vv[D];
+        bool near = false;
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+            f[k] = 0.0;
+            vv[k] = S.vel[(uint64_t)(v * D + k) * A.N + s];
+            near = near || fabs(pv[k] - P.min[k]) <= P.fric_d || fabs(pv[k] - P.max[k]) <= P.fric_d;
+        }
+        f[D - 1] = -P.gel;
+        const double q = sqrt(sum_squares_unfused<D>(vv));
+        if (q != 0.0 && near) {
+#pragma unroll
+            for (int k = 0; k < D; k++) f[k] = f[k] + ((-(vv[k] / q)) * P.gel) * P.fric;
+        }
+#pragma unroll
+        for (int k = 0; k < D; k++) acc[k] = acc[k] + f[k];
+    }
 #pragma unroll
     for (int k = 0; k < D; k++) A.force[(uint64_t)(v * D + k) * A.N + s] = acc[k];
 

@@ -43,6 +43,11 @@ struct InstParams {
     double langevin_coeff;  // sqrt(2 * damping * kT / mass) (mechanics.rs:439-445)
     double brownian_q;      // diffusion / kT (mechanics.rs:352)
     double brownian_coeff;  // sqrt(2 * diffusion) (mechanics.rs:341)
+    // domain force (SubDomainForce), added after the pair forces
+    uint32_t dforce;        // crm_domain_force_kind
+    uint32_t dpad;
+    double dk[3], dc[3];    // spring: strength and centre per axis, 3-D layout
+    double gel, fric, fric_d;  // rods gel: pressure, surface friction, friction distance
 };
 
 // Work unit of the agent-parallel kernels: a block of consecutive slots inside

@@ -38,6 +38,7 @@ class InstanceSpec:
     domain: CartesianCuboid
     interaction: object
     mechanics: object
+    domain_force: Optional[object] = None  # DomainSpring / RodGelForce (the DomainForce aspect)
 
 
 class MechanicsBackend:
@@ -91,6 +92,9 @@ class MechanicsBackend:
         self._h = handle
         self.n_agents = 0
         self.counts: List[int] = []
+        for i, spec in enumerate(instances):
+            if spec.domain_force is not None:
+                self.set_domain_force(spec.domain_force, i)
         # False: default tiling; True: tune in memory after each set_agents; a path:
         # also read / update that JSON cache of earlier choices
         self.autotune = autotune
@@ -203,6 +207,11 @@ class MechanicsBackend:
         self._check(self._lib.crm_get_tiling(self._h, inst, C.byref(r), C.byref(c), C.byref(n)))
         return {"rows": int(r.value), "cols": int(c.value), "tiles": int(n.value)}
 
+    def set_domain_force(self, force, inst: int = 0) -> None:
+        """Domain force of instance `inst` (crm_set_domain_force); None removes it."""
+        f = _abi.DomainForce() if force is None else force.to_struct()
+        self._check(self._lib.crm_set_domain_force(self._h, inst, C.byref(f)))
+
     def set_neighbor_sensing(self, radius: Optional[float]) -> None:
         """Count, in every force evaluation, each agent's (point or vertex) pairs
         with the agents of its 3^D voxel stencil closer than `radius` (the
@@ -299,6 +308,7 @@ def run_simulation(
     velocities=None,
     species=None,
     mechanics_solver_order: int = 2,
+    domain_force=None,
     **backend_kwargs,
 ) -> Dict[int, np.ndarray]:
     """Runs the [Mechanics, Interaction] aspects.  Without settings.storage it returns
@@ -307,7 +317,7 @@ def run_simulation(
     at those iterations as run_simulation! does (run_sim.rs:375-378,397-398,430-445)
     and returns the StorageAccess."""
     be = MechanicsBackend(
-        [InstanceSpec(domain, interaction, mechanics)],
+        [InstanceSpec(domain, interaction, mechanics, domain_force)],
         dt=settings.time.dt,
         mechanics_solver_order=mechanics_solver_order,
         **backend_kwargs,

@@ -257,6 +257,36 @@ crm_status crm_autotune(crm_ctx* ctx, const char* cache_path, uint32_t* chosen);
 /* Force tiling of instance inst: voxel rows and columns per tile, tile count. */
 crm_status crm_get_tiling(crm_ctx* ctx, uint32_t inst, uint32_t* rows, uint32_t* cols, uint32_t* n_tiles);
 
+/* Force of the domain on every agent (the DomainForce aspect: SubDomainForce::
+ * calculate_custom_force, concepts domain.rs:250-285, applied per cell after the
+ * subdomain's pair forces by calculate_custom_domain_force,
+ * update_mechanics.rs:244-274, run_sim.rs:277-279).  Per point or rod vertex:
+ *   SPRING    f_k = (-strength[k]) * (x_k - center[k])   (the form of the
+ *             reference's test domain, cellular_raza/tests/domain_force_spring.rs:56-67)
+ *   ROD_GEL   CartesianSubDomainRods (bacterial_rods.rs:437-479), 3-D rods only:
+ *             (0, 0, -gel_pressure), plus -(v / |v|) * gel_pressure *
+ *             surface_friction when |v| != 0 and the vertex is within
+ *             surface_friction_distance of a domain face on any axis.
+ * Vectors are in the caller's axis order (x, y[, z]). */
+typedef enum crm_domain_force_kind {
+    CRM_DOMAIN_FORCE_NONE = 0,
+    CRM_DOMAIN_FORCE_SPRING = 1,
+    CRM_DOMAIN_FORCE_ROD_GEL = 2
+} crm_domain_force_kind;
+
+typedef struct crm_domain_force {
+    uint32_t kind;
+    uint32_t pad;
+    double strength[3];
+    double center[3];
+    double gel_pressure;
+    double surface_friction;
+    double surface_friction_distance;
+} crm_domain_force;
+
+/* Sets the domain force of instance `inst` (default NONE). */
+crm_status crm_set_domain_force(crm_ctx* ctx, uint32_t inst, const crm_domain_force* force);
+
 /* Neighbour sensing fused into the force pass.  Replaces the NeighborSensing
  * hooks of the chili pair loop (concepts interaction.rs:64-79 trait,
  * update_mechanics.rs:125-139 neighbor_sensing_single, called per pair at
@@ -283,8 +313,8 @@ crm_status crm_get_neighbor_counts(crm_ctx* ctx, uint32_t* counts);
  *   pos[K], vel[K], dx history[n_hist][K], dv history[n_hist][K]
  * with the history oldest first (AuxStorageMechanics positions / velocities
  * rings, aux_storage.rs:41-46,107-115); species[N] (may be NULL); n_hist
- * receives the number of stored history entries (0..2), which the caller can
- * size records with by calling first with records == NULL. */
+ * receives the number of stored history entries (0..2).  With records and
+ * species both NULL only n_hist is returned (to size records). */
 crm_status crm_get_snapshot(crm_ctx* ctx, double* records, int32_t* species, uint32_t* n_hist);
 
 /* Cells of one save point written in the reference's JSON storage layout

@@ -585,6 +585,7 @@ struct SimBase {
     virtual void voxels_out(uint32_t* voxel_of_agent, uint32_t* start_out, uint64_t* order_out) = 0;
     virtual uint64_t n_voxels_total() = 0;
     virtual void neighbor_counts(double radius, uint32_t* counts) = 0;
+    virtual int set_domain_force(uint32_t ii, const crm_domain_force& f) = 0;
 };
 
 template <int KM>
@@ -593,6 +594,7 @@ struct Sim : SimBase {
         Grid grid;
         crm_potential pot;
         crm_mechanics mech;
+        crm_domain_force dforce{};                      // kind NONE unless set
         std::vector<std::vector<Cell<KM>>> voxels;      // indexed by plain index
         std::vector<uint32_t> owner;                    // subdomain of each voxel
         std::vector<std::vector<uint64_t>> sub_voxels;  // ascending plain indices per subdomain
@@ -706,7 +708,54 @@ struct Sim : SimBase {
                     for (int k = 0; k < K; k++) c1.force[k] = c1.force[k] + acc[k];
             }
         }
+        // calculate_custom_domain_force runs in step 1 after the subdomain's own
+        // pairs (run_sim.rs:277-279, update_mechanics.rs:244-274): aux.add_force(f)
+        if (I.dforce.kind != CRM_DOMAIN_FORCE_NONE)
+            for (uint64_t v : I.sub_voxels[sub])
+                for (auto& c : I.voxels[v]) {
+                    double f[KM] = {};
+                    domain_force(I, c, f);
+                    for (int k = 0; k < K; k++) c.force[k] = c.force[k] + f[k];
+                }
         return 0;
+    }
+
+    // SubDomainForce::calculate_custom_force per point / rod vertex.
+    //   spring: the test domain of cellular_raza/tests/domain_force_spring.rs:56-67,
+    //           generalised per axis: (-strength_k) * (x_k - center_k);
+    //   rod gel: CartesianSubDomainRods (bacterial_rods.rs:447-477): the row starts
+    //           at (0, 0, -gel_pressure); when |v| != 0 and |p_k - min_k| <= d or
+    //           |p_k - max_k| <= d on some axis, it gets += (-(v / |v|) * gel) * friction.
+    void domain_force(const Instance& I, const Cell<KM>& c, double* f) const {
+        const crm_domain_force& F = I.dforce;
+        for (int v = 0; v < nv; v++) {
+            const double* p = c.pos + v * D;
+            double* o = f + v * D;
+            if (F.kind == CRM_DOMAIN_FORCE_SPRING) {
+                for (int k = 0; k < D; k++) o[k] = (-F.strength[k]) * (p[k] - F.center[k]);
+                continue;
+            }
+            const double* vel = c.vel + v * D;
+            for (int k = 0; k < D; k++) o[k] = 0.0;
+            o[D - 1] = -F.gel_pressure;
+            bool near = false;
+            for (int k = 0; k < D; k++) {
+                near = near || std::fabs(p[k] - I.grid.min[k]) <= F.surface_friction_distance;
+                near = near || std::fabs(p[k] - I.grid.max[k]) <= F.surface_friction_distance;
+            }
+            double q2 = 0.0;
+            for (int k = 0; k < D; k++) q2 = q2 + vel[k] * vel[k];
+            const double q = std::sqrt(q2);
+            if (q != 0.0 && near)
+                for (int k = 0; k < D; k++) o[k] = o[k] + ((-(vel[k] / q)) * F.gel_pressure) * F.surface_friction;
+        }
+    }
+
+    int set_domain_force(uint32_t ii, const crm_domain_force& f) override {
+        if (ii >= inst.size() || f.kind > CRM_DOMAIN_FORCE_ROD_GEL) return CRM_INVALID_ARGUMENT;
+        if (f.kind == CRM_DOMAIN_FORCE_ROD_GEL && !(nv > 1 && D == 3)) return CRM_INVALID_ARGUMENT;
+        inst[ii].dforce = f;
+        return CRM_OK;
     }
 
     // update_mechanics_interaction_step_2 (update_mechanics.rs:285-340)
@@ -1347,6 +1396,11 @@ int orc_get_voxels(void* h, uint32_t* voxel_of_agent, uint32_t* start_out, uint6
 uint64_t orc_n_voxels_total(void* h) { return ((SimBase*)h)->n_voxels_total(); }
 
 // Neighbour count of every agent (by agent index) for the current positions.
+int orc_set_domain_force(void* h, uint32_t inst, const crm_domain_force* f) {
+    if (!h || !f) return CRM_INVALID_ARGUMENT;
+    return static_cast<SimBase*>(h)->set_domain_force(inst, *f);
+}
+
 int orc_neighbor_counts(void* h, double radius, uint32_t* counts) {
     SimBase& S = *(SimBase*)h;
     if (!S.have_agents) return CRM_INVALID_ARGUMENT;

@@ -55,6 +55,7 @@ def load():
         "orc_get_history": (C.c_int, [P, DP, DP, _abi.U32P]),
         "orc_get_voxels": (C.c_int, [P, _abi.U32P, _abi.U32P, _abi.U64P]),
         "orc_neighbor_counts": (C.c_int, [P, C.c_double, _abi.U32P]),
+        "orc_set_domain_force": (C.c_int, [P, C.c_uint32, C.POINTER(_abi.DomainForce)]),
         "orc_n_voxels_total": (C.c_uint64, [P]),
         "orc_n_agents": (C.c_uint64, [P]),
         "orc_last_error": (C.c_size_t, [P, C.c_char_p, C.c_size_t]),
@@ -112,6 +113,13 @@ class OracleSim:
             raise OracleError(st, "orc_create failed")
         self._h = h
         self.n_agents = 0
+        for i, spec in enumerate(instances):
+            if getattr(spec, "domain_force", None) is not None:
+                self.set_domain_force(spec.domain_force, i)
+
+    def set_domain_force(self, force, inst=0):
+        f = _abi.DomainForce() if force is None else force.to_struct()
+        self._check(self._lib.orc_set_domain_force(self._h, inst, C.byref(f)))
 
     @property
     def K(self):
