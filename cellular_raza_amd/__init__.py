@@ -22,6 +22,7 @@ from .building_blocks import (  # noqa: F401
     NoInteraction,
     RodInteraction,
     RodGelForce,
+    RodGrowthCycle,
     RodMechanics,
     DomainSpring,
     SpeciesMorsePotential,

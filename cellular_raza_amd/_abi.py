@@ -16,6 +16,7 @@ CRM_SETUP_ERROR = 5
 CRM_DEVICE_ERROR = 6
 CRM_INVALID_ARGUMENT = 7
 CRM_STORAGE_ERROR = 8
+CRM_DIVISION_ERROR = 9
 
 STATUS_NAMES = {
     CRM_OK: "Ok",
@@ -27,6 +28,7 @@ STATUS_NAMES = {
     CRM_DEVICE_ERROR: "DeviceError",
     CRM_INVALID_ARGUMENT: "InvalidArgument",
     CRM_STORAGE_ERROR: "StorageError",
+    CRM_DIVISION_ERROR: "DivisionError",
 }
 
 # crm_potential_kind
@@ -99,6 +101,17 @@ class Mechanics(C.Structure):
 
 class Instance(C.Structure):
     _fields_ = [("domain", Domain), ("potential", Potential), ("mechanics", Mechanics)]
+
+
+class RodCycle(C.Structure):
+    _fields_ = [
+        ("growth_base_rate", C.c_double),
+        ("division_threshold", C.c_double),
+        ("division_radius", C.c_double),
+        ("sensing_radius", C.c_double),
+        ("neighbor_saturation", C.c_double),
+        ("max_agents", C.c_uint64),
+    ]
 
 
 DOMAIN_FORCE_NONE = 0
@@ -183,6 +196,9 @@ SIGNATURES = {
     "crm_get_neighbor_counts": (C.c_int, [P, U32P]),
     "crm_get_snapshot": (C.c_int, [P, DP, C.POINTER(C.c_int32), U32P]),
     "crm_set_domain_force": (C.c_int, [P, C.c_uint32, C.POINTER(DomainForce)]),
+    "crm_set_rod_cycle": (C.c_int, [P, C.POINTER(RodCycle)]),
+    "crm_set_rod_spring_lengths": (C.c_int, [P, DP]),
+    "crm_get_cycle_state": (C.c_int, [P, DP, DP, U64P, U64P]),
     "crm_save_cells_json": (C.c_int, [P, C.c_char_p, C.c_uint64, C.POINTER(JsonCells)]),
     "crm_get_timing": (C.c_int, [P, C.POINTER(Timing)]),
     "crm_reset_timing": (C.c_int, [P]),

@@ -333,6 +333,31 @@ class RodGelForce:
         return f
 
 
+@dataclass
+class RodGrowthCycle:
+    """The Cycle + NeighborSensing of the rods example (bacterial_rods/src/main.rs:48-111):
+    growth rate from the neighbour count, spring-length growth, division of the rod
+    (RodMechanics::divide) at a threshold.  The example uses sensing_radius = 2 R,
+    division_radius = R (R the interaction radius) and neighbor_saturation = 8."""
+
+    growth_base_rate: float
+    division_threshold: float
+    division_radius: float
+    sensing_radius: float
+    neighbor_saturation: float = 8.0
+    max_agents: int = 1 << 20
+
+    def to_struct(self) -> _abi.RodCycle:
+        y = _abi.RodCycle()
+        y.growth_base_rate = self.growth_base_rate
+        y.division_threshold = self.division_threshold
+        y.division_radius = self.division_radius
+        y.sensing_radius = self.sensing_radius
+        y.neighbor_saturation = self.neighbor_saturation
+        y.max_agents = int(self.max_agents)
+        return y
+
+
 def instance_struct(domain: CartesianCuboid, interaction, mechanics) -> _abi.Instance:
     inst = _abi.Instance()
     inst.domain = domain.to_struct()

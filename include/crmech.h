@@ -52,7 +52,8 @@ typedef enum crm_status {
     CRM_SETUP_ERROR = 5,      /* DecomposeError / TimeError during setup */
     CRM_DEVICE_ERROR = 6,     /* HIP runtime failure */
     CRM_INVALID_ARGUMENT = 7, /* caller error: bad sizes, null pointers, unsupported combination */
-    CRM_STORAGE_ERROR = 8     /* StorageError (storage/concepts.rs:16-42): a snapshot file could not be written */
+    CRM_STORAGE_ERROR = 8,    /* StorageError (storage/concepts.rs:16-42): a snapshot file could not be written */
+    CRM_DIVISION_ERROR = 9    /* DivisionError (concepts errors.rs): RodMechanics::divide failed */
 } crm_status;
 
 /* Interaction kinds (cellular_raza-building-blocks/src/cell_building_blocks/interaction.rs). */
@@ -286,6 +287,44 @@ typedef struct crm_domain_force {
 
 /* Sets the domain force of instance `inst` (default NONE). */
 crm_status crm_set_domain_force(crm_ctx* ctx, uint32_t inst, const crm_domain_force* force);
+
+/* Cycle of the rods example (aspects [Mechanics, Interaction, Cycle,
+ * NeighborSensing], bacterial_rods/src/main.rs:48-111) with chili's division
+ * bookkeeping (update_cycle.rs:10-69).  Every step, after the mechanics update:
+ *   react_to_neighbors: n = vertex pairs of the stencil closer than
+ *     sensing_radius (the NeighborSensing count of this step's force pass);
+ *     growth_rate = ((growth_base_rate * (neighbor_saturation - n)) /
+ *     neighbor_saturation).max(0) if n > 0, else growth_base_rate;
+ *   update_cycle: spring_length += growth_rate * dt; Division when
+ *     spring_length > division_threshold;
+ *   update_cell_cycle_4: RodMechanics::divide(division_radius) (bacterial_rods.rs:
+ *     586-662) splits the mother in place and adds a daughter (a clone with the
+ *     other half, empty history); identifiers CellIdentifier::Division(voxel,
+ *     counter) from the voxel's counter, mothers first, the parent the mother's
+ *     old identifier;
+ * then the boundary and the re-sort.  Daughters get agent indices n_agents,
+ * n_agents + 1, ... in voxel order.  The context holds at most max_agents rods
+ * (CRM_INVALID_ARGUMENT past it); rods only.  Enabling it switches neighbour
+ * sensing on with sensing_radius and sets every growth_rate to the base rate. */
+typedef struct crm_rod_cycle {
+    double growth_base_rate;
+    double division_threshold;
+    double division_radius;
+    double sensing_radius;
+    double neighbor_saturation;
+    uint64_t max_agents;
+} crm_rod_cycle;
+crm_status crm_set_rod_cycle(crm_ctx* ctx, const crm_rod_cycle* cycle);
+
+/* Per-rod spring lengths (RodMechanics::spring_length, bacterial_rods.rs:104), by
+ * agent index; crm_set_agents initialises them from the instance's mechanics. */
+crm_status crm_set_rod_spring_lengths(crm_ctx* ctx, const double* spring_length);
+
+/* Per-rod cycle state by agent index (any pointer may be NULL): spring_length[N],
+ * growth_rate[N], identifier[3 N] and parent[3 N] as (kind, a, b) with kind 0
+ * Initial(a), 1 Division(a = voxel plain index, b = counter), 3 no parent. */
+crm_status crm_get_cycle_state(crm_ctx* ctx, double* spring_length, double* growth_rate, uint64_t* identifier,
+                               uint64_t* parent);
 
 /* Neighbour sensing fused into the force pass.  Replaces the NeighborSensing
  * hooks of the chili pair loop (concepts interaction.rs:64-79 trait,

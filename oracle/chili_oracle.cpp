@@ -510,6 +510,14 @@ struct Cell {
     double ring_dv[2][KM];
     int ring_size;  // entries stored, 0..cap
     int ring_head;  // index of the oldest entry
+    // rods: RodMechanics::spring_length, per agent (bacterial_rods.rs:82-107)
+    double spring_length;
+    // Cycle of the rods example (bacterial_rods/src/main.rs:48-111)
+    double growth_rate;
+    uint32_t sensed;       // NeighborSensing accumulator of the current step
+    bool divide;           // CycleEvent::Division pending
+    uint64_t ident[3];     // CellIdentifier: kind (0 Initial(a), 1 Division(a = voxel, b = counter)), a, b
+    uint64_t parent[3];    // kind 3: no parent
 };
 
 template <int KM>
@@ -586,6 +594,11 @@ struct SimBase {
     virtual uint64_t n_voxels_total() = 0;
     virtual void neighbor_counts(double radius, uint32_t* counts) = 0;
     virtual int set_domain_force(uint32_t ii, const crm_domain_force& f) = 0;
+    virtual int set_rod_cycle(const crm_rod_cycle& y) = 0;
+    virtual int set_spring_lengths(const double* sl) = 0;
+    virtual void cycle_out(double* spring_length, double* growth_rate, uint64_t* ident, uint64_t* parent) = 0;
+    bool cycle_on = false;
+    crm_rod_cycle cyc{};
 };
 
 template <int KM>
@@ -596,6 +609,7 @@ struct Sim : SimBase {
         crm_mechanics mech;
         crm_domain_force dforce{};                      // kind NONE unless set
         std::vector<std::vector<Cell<KM>>> voxels;      // indexed by plain index
+        std::vector<uint64_t> id_counter;               // Voxel::id_counter per plain index
         std::vector<uint32_t> owner;                    // subdomain of each voxel
         std::vector<std::vector<uint64_t>> sub_voxels;  // ascending plain indices per subdomain
     };
@@ -852,7 +866,9 @@ struct Sim : SimBase {
                 break;
             }
             case CRM_MECHANICS_ROD: {  // bacterial_rods.rs:157-240
-                rod_increment(D, nv, M, c.pos, c.vel, force, dx, dv);
+                crm_mechanics Mc = M;
+                Mc.spring_length = c.spring_length;
+                rod_increment(D, nv, Mc, c.pos, c.vel, force, dx, dv);
                 if (have_noise) {
                     double sd = std::sqrt(dt);
                     for (int k = 0; k < K; k++)
@@ -1022,17 +1038,173 @@ struct Sim : SimBase {
         if (Ef.code.load()) return;
         for (uint64_t v : I.sub_voxels[sub])
             for (auto& c : I.voxels[v]) integrate(I, c);
+        if (cycle_on) {
+            // local funcs after the mechanics update: react_to_neighbors, then the
+            // cycle update (run_sim.rs:259-283); then update_cell_cycle_4 (step 4)
+            for (uint64_t v : I.sub_voxels[sub])
+                for (auto& c : I.voxels[v]) react_and_grow(c);
+            for (uint64_t v : I.sub_voxels[sub])
+                if (cycle_voxel(I, v)) Em.set(CRM_DIVISION_ERROR, "DivisionError: Could not calculate new segment lengths");
+        }
         for (uint64_t v : I.sub_voxels[sub])
             for (auto& c : I.voxels[v])
                 if (apply_boundary(I, c)) Em.set(CRM_BOUNDARY_ERROR, "BoundaryError: Particle is out of domain");
         sync();
         if (Em.code.load()) return;
+        if (cycle_on) {
+            if (sub == 0) number_new_cells(I);
+            sync();
+        }
         sort_step1(ii, sub);
         sync();
         sort_step2(ii, sub);
     }
 
+    // The accumulator of the rods example: (own vertex, other vertex) pairs of the
+    // stencil closer than the sensing radius, on the positions the forces see.
+    void sense_all() {
+        std::vector<uint32_t> cnt(n_agents);
+        neighbor_counts(cyc.sensing_radius, cnt.data());
+        for_each_cell([&](Cell<KM>& c) { c.sensed = cnt[c.id]; });
+    }
+
+    // react_to_neighbors then Cycle::update_cycle (bacterial_rods/src/main.rs:66-73,
+    // 88-100): growth = ((base * (sat - n)) / sat).max(0) when n > 0, else base;
+    // spring_length += growth * dt; Division when spring_length > threshold.
+    void react_and_grow(Cell<KM>& c) {
+        if (c.sensed > 0) {
+            const double g = (cyc.growth_base_rate * (cyc.neighbor_saturation - (double)c.sensed)) /
+                             cyc.neighbor_saturation;
+            c.growth_rate = std::fmax(g, 0.0);
+        } else {
+            c.growth_rate = cyc.growth_base_rate;
+        }
+        c.sensed = 0;  // clear_accumulator
+        c.spring_length = c.spring_length + c.growth_rate * cfg.dt;
+        if (c.spring_length > cyc.division_threshold) c.divide = true;
+    }
+
+    // RodMechanics::divide (bacterial_rods.rs:586-662) on c (the mother, changed in
+    // place) into d (the daughter, otherwise a clone).  Returns 1 on DivisionError.
+    int rod_divide(Cell<KM>& c, double radius, Cell<KM>& d) const {
+        const int n = nv;
+        double pos[KM];
+        std::memcpy(pos, c.pos, sizeof(double) * K);
+        d = c;
+        const double two = 1.0 + 1.0, one_half = 1.0 / two;
+        const double div_factor = one_half - radius / ((double)n * c.spring_length);
+        const double nsl = div_factor * c.spring_length;
+        c.spring_length = nsl;
+        d.spring_length = nsl;
+        double seg[CRM_MAX_ROD_VERTICES] = {};
+        for (int i = 0; i + 1 < n; i++) {
+            double z[3] = {0.0, 0.0, 0.0};
+            for (int k = 0; k < D; k++) z[k] = pos[i * D + k] - pos[(i + 1) * D + k];
+            double q2 = 0.0;
+            for (int k = 0; k < D; k++) q2 = q2 + z[k] * z[k];
+            seg[i] = std::sqrt(q2);
+        }
+        auto head = [&](int m) {  // segments.iter().take(m).sum()
+            double t = 0.0;
+            for (int i = 0; i < m; i++) t = t + seg[i];
+            return t;
+        };
+        auto tail = [&](int m) {  // segments.iter().rev().take(m).sum()
+            double t = 0.0;
+            for (int i = 0; i < m; i++) t = t + seg[n - 2 - i];
+            return t;
+        };
+        const double seg_len = (head(n - 1) - two * radius) / (double)(n - 1) / two;
+        for (int v = 0; v < n; v++) {
+            const double L = (double)v * seg_len;
+            int k = -1;
+            for (int m = 0; m < n - 1; m++)
+                if (head(m) <= L) k = m;
+            if (k < 0) return 1;
+            const double q = (L - head(k)) / seg[k];
+            for (int j = 0; j < D; j++) c.pos[v * D + j] = pos[k * D + j] * (1.0 - q) + pos[(k + 1) * D + j] * q;
+            int m = -1;
+            for (int mm = 0; mm < n; mm++)
+                if (tail(mm) <= L) m = mm;
+            if (m < 0 || n - m - 2 < 0) return 1;
+            const double p = (L - tail(m)) / seg[n - m - 2];
+            for (int j = 0; j < D; j++)
+                d.pos[(n - v - 1) * D + j] = pos[(n - m - 1) * D + j] * (1.0 - p) + pos[(n - m - 2) * D + j] * p;
+        }
+        return 0;
+    }
+
+    // Voxel::update_cell_cycle_4 (update_cycle.rs:10-69): mothers in voxel order get
+    // Division(voxel, ++counter) and their old identifier as parent; the daughters
+    // are appended after the loop with Division(voxel, ++counter), the same parent
+    // and a fresh AuxStorage (empty history, zero force).
+    int cycle_voxel(Instance& I, uint64_t v) {
+        auto& cells = I.voxels[v];
+        std::vector<Cell<KM>> born;
+        for (auto& c : cells) {
+            if (!c.divide) continue;
+            c.divide = false;
+            Cell<KM> d;
+            if (rod_divide(c, cyc.division_radius, d)) return 1;
+            uint64_t par[3] = {c.ident[0], c.ident[1], c.ident[2]};
+            I.id_counter[v] += 1;
+            c.ident[0] = 1;
+            c.ident[1] = v;
+            c.ident[2] = I.id_counter[v];
+            std::memcpy(c.parent, par, sizeof(par));
+            std::memcpy(d.parent, par, sizeof(par));
+            d.divide = false;
+            d.sensed = 0;
+            d.ring_size = 0;
+            d.ring_head = 0;
+            for (int k = 0; k < K; k++) d.force[k] = 0.0;
+            d.id = UINT64_MAX;  // numbered after every voxel has divided
+            born.push_back(d);
+        }
+        for (auto& d : born) {
+            I.id_counter[v] += 1;
+            d.ident[0] = 1;
+            d.ident[1] = v;
+            d.ident[2] = I.id_counter[v];
+            cells.push_back(d);
+        }
+        return 0;
+    }
+
+    // Agent indices of the cells born this step: n_agents, n_agents + 1, ... in
+    // voxel order, then order inside the voxel.
+    void number_new_cells(Instance& I) {
+        for (auto& cells : I.voxels)
+            for (auto& c : cells)
+                if (c.id == UINT64_MAX) c.id = n_agents++;
+    }
+
+    int set_rod_cycle(const crm_rod_cycle& y) override {
+        if (nv < 2) return CRM_INVALID_ARGUMENT;
+        cyc = y;
+        cycle_on = true;
+        for_each_cell([&](Cell<KM>& c) { c.growth_rate = y.growth_base_rate; });
+        return CRM_OK;
+    }
+
+    int set_spring_lengths(const double* sl) override {
+        for_each_cell([&](Cell<KM>& c) { c.spring_length = sl[c.id]; });
+        return CRM_OK;
+    }
+
+    void cycle_out(double* spring_length, double* growth_rate, uint64_t* ident, uint64_t* parent) override {
+        for_each_cell([&](Cell<KM>& c) {
+            if (spring_length) spring_length[c.id] = c.spring_length;
+            if (growth_rate) growth_rate[c.id] = c.growth_rate;
+            for (int j = 0; j < 3; j++) {
+                if (ident) ident[c.id * 3 + j] = c.ident[j];
+                if (parent) parent[c.id * 3 + j] = c.parent[j];
+            }
+        });
+    }
+
     int run_phase(bool forces_only) override {
+        if (cycle_on && !forces_only) sense_all();
         for (size_t ii = 0; ii < inst.size(); ii++) {
             StepError Ef, Em;
             if (n_sub == 1) {
@@ -1108,6 +1280,11 @@ struct Sim : SimBase {
                 std::memset(&c, 0, sizeof(c));
                 c.id = a;
                 c.species = species ? species[a] : 0;
+                c.spring_length = I.mech.spring_length;
+                c.growth_rate = cyc.growth_base_rate;
+                c.ident[0] = 0;
+                c.ident[1] = j;  // CellBox::new_initial(n_cell): index in this simulation's agents
+                c.parent[0] = 3;
                 for (int k = 0; k < K; k++) {
                     c.pos[k] = pos[a * K + k];
                     c.vel[k] = vel ? vel[a * K + k] : 0.0;
@@ -1130,6 +1307,9 @@ struct Sim : SimBase {
                 }
                 I.voxels[plain_of(I.grid, idx)].push_back(c);
             }
+            // insert_initial_cells counts the initial cells of each voxel (datastructures.rs:286-297)
+            I.id_counter.assign(I.voxels.size(), 0);
+            for (size_t v = 0; v < I.voxels.size(); v++) I.id_counter[v] = I.voxels[v].size();
         }
         n_agents = a;
         step = 0;
@@ -1396,6 +1576,22 @@ int orc_get_voxels(void* h, uint32_t* voxel_of_agent, uint32_t* start_out, uint6
 uint64_t orc_n_voxels_total(void* h) { return ((SimBase*)h)->n_voxels_total(); }
 
 // Neighbour count of every agent (by agent index) for the current positions.
+int orc_set_rod_cycle(void* h, const crm_rod_cycle* y) {
+    if (!h || !y) return CRM_INVALID_ARGUMENT;
+    return static_cast<SimBase*>(h)->set_rod_cycle(*y);
+}
+
+int orc_set_spring_lengths(void* h, const double* sl) {
+    if (!h || !sl) return CRM_INVALID_ARGUMENT;
+    return static_cast<SimBase*>(h)->set_spring_lengths(sl);
+}
+
+int orc_cycle_out(void* h, double* spring_length, double* growth_rate, uint64_t* ident, uint64_t* parent) {
+    if (!h) return CRM_INVALID_ARGUMENT;
+    static_cast<SimBase*>(h)->cycle_out(spring_length, growth_rate, ident, parent);
+    return CRM_OK;
+}
+
 int orc_set_domain_force(void* h, uint32_t inst, const crm_domain_force* f) {
     if (!h || !f) return CRM_INVALID_ARGUMENT;
     return static_cast<SimBase*>(h)->set_domain_force(inst, *f);

@@ -56,6 +56,9 @@ def load():
         "orc_get_voxels": (C.c_int, [P, _abi.U32P, _abi.U32P, _abi.U64P]),
         "orc_neighbor_counts": (C.c_int, [P, C.c_double, _abi.U32P]),
         "orc_set_domain_force": (C.c_int, [P, C.c_uint32, C.POINTER(_abi.DomainForce)]),
+        "orc_set_rod_cycle": (C.c_int, [P, C.POINTER(_abi.RodCycle)]),
+        "orc_set_spring_lengths": (C.c_int, [P, _abi.DP]),
+        "orc_cycle_out": (C.c_int, [P, _abi.DP, _abi.DP, _abi.U64P, _abi.U64P]),
         "orc_n_voxels_total": (C.c_uint64, [P]),
         "orc_n_agents": (C.c_uint64, [P]),
         "orc_last_error": (C.c_size_t, [P, C.c_char_p, C.c_size_t]),
@@ -155,7 +158,24 @@ class OracleSim:
 
     def step(self, n_steps=1, noise=None):
         nz = None if noise is None else np.ascontiguousarray(noise, dtype=np.float64)
-        self._check(self._lib.orc_step(self._h, n_steps, _dp(nz)))
+        st = self._lib.orc_step(self._h, n_steps, _dp(nz))
+        self.n_agents = int(self._lib.orc_n_agents(self._h))  # division adds agents
+        self._check(st)
+
+    def set_rod_cycle(self, cycle):
+        self._check(self._lib.orc_set_rod_cycle(self._h, C.byref(cycle.to_struct())))
+
+    def set_spring_lengths(self, spring_length):
+        sl = np.ascontiguousarray(spring_length, dtype=np.float64)
+        self._check(self._lib.orc_set_spring_lengths(self._h, _dp(sl)))
+
+    def cycle_state(self):
+        n = self.n_agents
+        sl, gr = np.empty(n), np.empty(n)
+        ident, parent = np.empty((n, 3), dtype=np.uint64), np.empty((n, 3), dtype=np.uint64)
+        self._check(self._lib.orc_cycle_out(self._h, _dp(sl), _dp(gr), ident.ctypes.data_as(_abi.U64P),
+                                            parent.ctypes.data_as(_abi.U64P)))
+        return {"spring_length": sl, "growth_rate": gr, "identifier": ident, "parent": parent}
 
     def step_timed(self, n_steps):
         sec = C.c_double()

@@ -560,3 +560,71 @@ def test_domain_force_rod_gel_closed_form():
                 want[a, v] += -(vel[a, v] / q) * gel * fric
     assert np.allclose(f, want, rtol=0, atol=1e-15)
     o.close()
+
+
+# ---------------------------------------------------------------------------
+# Cycle of the rods example: RodMechanics::divide's doctest, the identifier
+# bookkeeping of update_cell_cycle_4, the growth-rate reaction
+# ---------------------------------------------------------------------------
+def _rod_cycle_sim(pos, spring_length, cycle, L=40.0, irange=5.0, sp=None):
+    from cellular_raza_amd import RodInteraction
+
+    n, nv, dim = pos.shape
+    dom = CartesianCuboid.from_boundaries_and_interaction_range([0.0] * dim, [L] * dim, irange)
+    spec = [InstanceSpec(dom, RodInteraction(NoInteraction()),
+                         RodMechanics(n_vertices=nv, diffusion_constant=0.0, spring_tension=0.1, rigidity=0.05,
+                                      spring_length=spring_length, damping=0.0))]
+    o = O.OracleSim(spec, dt=0.1, mechanics_solver_order=2)
+    o.set_agents(pos.reshape(n, -1))
+    o.set_rod_cycle(cycle)
+    return o
+
+
+def test_rod_divide_doctest():
+    """bacterial_rods.rs:560-585: 7 vertices 0.5 apart, radius 0.25: the mother's last
+    vertex and the daughter's first are 2 radius apart.  Closed form: the mother keeps
+    v * l from the start, the daughter v * l from the end, l = (3 - 2 r) / 6 / 2; both
+    spring lengths (1/2 - r / (7 * 0.5)) * 0.5; identifiers Division(voxel, 2) and
+    Division(voxel, 3) (the voxel counted its initial rod), parent Initial(0)."""
+    from cellular_raza_amd import RodGrowthCycle
+
+    pos = np.zeros((1, 7, 2))
+    pos[0, :, 0] = 10.0 + 0.5 * np.arange(7)
+    pos[0, :, 1] = 10.0
+    cyc = RodGrowthCycle(growth_base_rate=0.0, division_threshold=0.4, division_radius=0.25, sensing_radius=0.5)
+    o = _rod_cycle_sim(pos, 0.5, cyc)
+    o.step(1)
+    assert o.n_agents == 2
+    p = o.get_state()[0].reshape(2, 7, 2)
+    l = (3.0 - 0.5) / 6 / 2
+    assert np.allclose(p[0, :, 0], 10.0 + l * np.arange(7), rtol=0, atol=1e-12)
+    assert np.allclose(p[1, :, 0], 13.0 - l * np.arange(7)[::-1], rtol=0, atol=1e-12)
+    assert abs(np.linalg.norm(p[0, 6] - p[1, 0]) - 0.5) < 1e-3
+    st = o.cycle_state()
+    assert np.allclose(st["spring_length"], (0.5 - 0.25 / 3.5) * 0.5, rtol=0, atol=1e-15)
+    vox = st["identifier"][0, 1]
+    assert st["identifier"].tolist() == [[1, vox, 2], [1, vox, 3]]
+    assert st["parent"].tolist() == [[0, 0, 0], [0, 0, 0]]
+    o.close()
+
+
+def test_rod_growth_reacts_to_neighbours():
+    """react_to_neighbors (bacterial_rods/src/main.rs:66-73): no neighbour -> base rate;
+    n vertex pairs within 2 R -> ((base (8 - n)) / 8).max(0); the spring length grows
+    by rate * dt."""
+    from cellular_raza_amd import RodGrowthCycle
+
+    base, dt = 0.2, 0.1
+    pos = np.zeros((3, 3, 2))
+    pos[:, :, 0] = 0.5 * np.arange(3)
+    pos[0] += [5.0, 5.0]
+    pos[1] += [5.0, 5.4]    # parallel at 0.4: 3 vertex pairs within 0.5 for each rod
+    pos[2] += [20.0, 20.0]  # alone
+    cyc = RodGrowthCycle(growth_base_rate=base, division_threshold=10.0, division_radius=0.25, sensing_radius=0.5)
+    o = _rod_cycle_sim(pos, 0.5, cyc)
+    o.step(1)
+    st = o.cycle_state()
+    want = np.array([(base * (8.0 - 3.0)) / 8.0, (base * (8.0 - 3.0)) / 8.0, base])
+    assert np.array_equal(st["growth_rate"], want)
+    assert np.array_equal(st["spring_length"], 0.5 + want * dt)
+    o.close()
