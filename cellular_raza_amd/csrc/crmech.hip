@@ -22,6 +22,9 @@
 #include "crmech.h"
 #include "kernels_rod.hpp"
 #include "kernels_snapshot.hpp"
+#include "kernels_cycle.hpp"
+
+#include <hipcub/hipcub.hpp>
 
 namespace crm {
 
@@ -176,6 +179,18 @@ struct crm_ctx {
     double sense_t = 0.0;
     uint32_t* d_sense = nullptr;
     int sense_parity = -1;
+    // rods example cycle (crm_set_rod_cycle): capacity, per-slot and per-voxel scratch
+    bool cycle_on = false;
+    crm_rod_cycle cyc{};
+    uint64_t cap_agents = 0;        // rods allocated in every N-sized array
+    uint32_t* d_divflag = nullptr;  // [cap] 1: divides this step
+    uint32_t* d_drank = nullptr;    // [cap] rank of a dividing rod among its voxel's
+    uint64_t* d_dctr = nullptr;     // [cap] identifier counter of its daughter
+    uint32_t* d_ndiv = nullptr;     // [Vt + 1] daughters per voxel, then their exclusive scan
+    uint32_t* d_dbefore = nullptr;  // [Vt + 1]
+    uint64_t* d_vcount = nullptr;   // [Vt] Voxel::id_counter
+    void* d_scan_tmp = nullptr;
+    size_t scan_tmp_bytes = 0;
     // snapshot records (agent order) and species, allocated on the first save
     double* d_snap = nullptr;
     int32_t* d_snap_sp = nullptr;
@@ -204,11 +219,12 @@ thread_local std::string g_create_error;
 void free_device(crm_ctx* ctx) {
     void* ptrs[] = {ctx->d_work, ctx->d_tiles, ctx->d_start, ctx->d_count, ctx->d_bsum, ctx->d_key, ctx->d_rank,
                     ctx->d_perm, ctx->d_force, ctx->d_rec64, ctx->d_recq, ctx->d_noise, ctx->d_recrod, ctx->d_bnd,
-                    ctx->d_sense, ctx->d_snap, ctx->d_snap_sp};
+                    ctx->d_sense, ctx->d_snap, ctx->d_snap_sp, ctx->d_divflag, ctx->d_drank, ctx->d_dctr,
+                    ctx->d_ndiv, ctx->d_dbefore, ctx->d_vcount, ctx->d_scan_tmp};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& b : ctx->buf) {
-        void* q[] = {b.pos, b.vel, b.hdx, b.hdv, b.species, b.id, b.vox};
+        void* q[] = {b.pos, b.vel, b.hdx, b.hdv, b.species, b.id, b.vox, b.spring, b.birth, b.growth, b.ident, b.parent};
         for (void* p : q)
             if (p) (void)hipFree(p);
         b = StateBuf{};
@@ -228,6 +244,13 @@ void free_device(crm_ctx* ctx) {
     ctx->d_snap = nullptr;
     ctx->d_snap_sp = nullptr;
     ctx->snap_cap = 0;
+    ctx->d_divflag = ctx->d_drank = nullptr;
+    ctx->d_dctr = nullptr;
+    ctx->d_ndiv = ctx->d_dbefore = nullptr;
+    ctx->d_vcount = nullptr;
+    ctx->d_scan_tmp = nullptr;
+    ctx->scan_tmp_bytes = 0;
+    ctx->cap_agents = 0;
     ctx->noise_cap = 0;
     for (auto& g : ctx->graph)
         if (g) {
@@ -727,6 +750,125 @@ crm_status build_graph(crm_ctx* ctx, int parity) {
     return CRM_OK;
 }
 
+// Temporary storage of the daughters' exclusive scan (hipcub, Vt + 1 items).
+crm_status alloc_scan_tmp(crm_ctx* ctx) {
+    size_t bytes = 0;
+    CRM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, ctx->d_ndiv, ctx->d_dbefore, (int)(ctx->Vt + 1),
+                                             ctx->stream));
+    uint8_t* p = nullptr;
+    CRM_TRY(dalloc(ctx, p, std::max<size_t>(bytes, 1)));
+    ctx->d_scan_tmp = p;
+    ctx->scan_tmp_bytes = std::max<size_t>(bytes, 1);
+    return CRM_OK;
+}
+
+// Work blocks (rods_per_block rods, never across instances) for the current
+// per-instance counts; offsets and counts go to the instance parameters.
+crm_status rebuild_work(crm_ctx* ctx) {
+    std::vector<BlockWork> work;
+    uint64_t first = 0;
+    const uint64_t per = is_rod(ctx) ? ctx->rods_per_block : (uint64_t)kBlock;
+    for (uint32_t i = 0; i < ctx->n_inst; i++) {
+        ctx->hparams[i].agent_offset = first;
+        ctx->hparams[i].n_agents = ctx->counts[i];
+        for (uint64_t b = 0; b < ctx->counts[i]; b += per) {
+            BlockWork w;
+            w.inst = i;
+            w.first = first + b;
+            w.count = (uint32_t)std::min<uint64_t>(per, ctx->counts[i] - b);
+            work.push_back(w);
+        }
+        first += ctx->counts[i];
+    }
+    ctx->n_work = (uint32_t)work.size();
+    CRM_HIP(hipMemcpyAsync(ctx->d_work, work.data(), sizeof(BlockWork) * work.size(), hipMemcpyHostToDevice,
+                           ctx->stream));
+    CRM_HIP(hipMemcpyAsync(ctx->d_ip, ctx->hparams.data(), sizeof(InstParams) * ctx->n_inst, hipMemcpyHostToDevice,
+                           ctx->stream));
+    CRM_HIP(hipStreamSynchronize(ctx->stream));
+    return CRM_OK;
+}
+
+// One step with the rods example's cycle (kernels_cycle.hpp).  Synchronous: the
+// host reads the number of divisions to size the grown order.
+crm_status launch_cycle_step(crm_ctx* ctx, const double* noise_step) {
+    const StepArgs sa = step_args(ctx, noise_step);
+    const uint32_t D = ctx->D, Vt = (uint32_t)ctx->Vt;
+    hipLaunchKernelGGL(k_cycle_begin, dim3(1), dim3(1), 0, ctx->stream, ctx->d_ctrl);
+    CRM_TRY(launch_force(ctx));
+    CycleArgs Y;
+    Y.base = ctx->cyc.growth_base_rate;
+    Y.threshold = ctx->cyc.division_threshold;
+    Y.saturation = ctx->cyc.neighbor_saturation;
+    Y.divflag = ctx->d_divflag;
+    Y.drank = ctx->d_drank;
+    Y.dctr = ctx->d_dctr;
+    Y.ndiv = ctx->d_ndiv;
+    Y.dbefore = ctx->d_dbefore;
+    Y.vcount = ctx->d_vcount;
+    Y.n_inst = ctx->n_inst;
+    const RodArgs R = rod_args(ctx);
+    StateBuf B = ctx->buf[ctx->cur], B2 = ctx->buf[1 - ctx->cur];
+    {
+        Arrays A = arrays(ctx);
+        const dim3 g(ctx->n_work), b(kBlock);
+        if (D == 2) hipLaunchKernelGGL(k_rod_mech<2>, g, b, 0, ctx->stream, A, B, B2, R, sa, Y);
+        else hipLaunchKernelGGL(k_rod_mech<3>, g, b, 0, ctx->stream, A, B, B2, R, sa, Y);
+        CRM_HIP(hipGetLastError());
+    }
+    Control c;
+    CRM_TRY(read_control(ctx, c));
+    if (c.error_flags) return check_errors(ctx, "step");
+    const uint64_t n_old = ctx->N, n_new = n_old + c.n_div;
+    if (c.n_div == 0) {
+        ctx->cur = 1 - ctx->cur;  // the updated state is in the other buffer
+    } else {
+        if (n_new > ctx->cap_agents) {
+            ctx->err = "the cycle needs " + std::to_string(n_new) + " rods but the context holds max_agents = " +
+                       std::to_string(ctx->cap_agents);
+            ctx->halted = true;
+            return CRM_INVALID_ARGUMENT;
+        }
+        Arrays A = arrays(ctx);
+        hipLaunchKernelGGL(k_cycle_voxels, dim3((Vt + 1 + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, A,
+                           B2, Y);
+        CRM_HIP(hipGetLastError());
+        size_t bytes = ctx->scan_tmp_bytes;
+        CRM_HIP(hipcub::DeviceScan::ExclusiveSum(ctx->d_scan_tmp, bytes, ctx->d_ndiv, ctx->d_dbefore, (int)(Vt + 1),
+                                                 ctx->stream));
+        const dim3 g((uint32_t)((n_old + kBlock - 1) / kBlock)), b(kBlock);
+        if (D == 2)
+            hipLaunchKernelGGL(k_expand_rod<2>, g, b, 0, ctx->stream, A, B2, B, R, sa, Y, n_old, n_new,
+                               ctx->cyc.division_radius);
+        else
+            hipLaunchKernelGGL(k_expand_rod<3>, g, b, 0, ctx->stream, A, B2, B, R, sa, Y, n_old, n_new,
+                               ctx->cyc.division_radius);
+        CRM_HIP(hipGetLastError());
+        // daughters per instance from the scan at the instances' voxel boundaries
+        std::vector<uint32_t> at(ctx->n_inst + 1);
+        for (uint32_t i = 0; i <= ctx->n_inst; i++) {
+            const uint32_t v = i < ctx->n_inst ? ctx->hparams[i].voxel_offset : Vt;
+            CRM_HIP(hipMemcpyAsync(&at[i], ctx->d_dbefore + v, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+        }
+        CRM_HIP(hipStreamSynchronize(ctx->stream));
+        for (uint32_t i = 0; i < ctx->n_inst; i++) ctx->counts[i] += at[i + 1] - at[i];
+        ctx->N = n_new;
+        CRM_TRY(rebuild_work(ctx));
+        CRM_TRY(check_errors(ctx, "division"));
+    }
+    {
+        Arrays A = arrays(ctx);
+        const dim3 g(ctx->n_work), b(kBlock);
+        if (D == 2) hipLaunchKernelGGL(k_reflect_key_rod<2>, g, b, 0, ctx->stream, A, ctx->buf[ctx->cur], R);
+        else hipLaunchKernelGGL(k_reflect_key_rod<3>, g, b, 0, ctx->stream, A, ctx->buf[ctx->cur], R);
+        CRM_HIP(hipGetLastError());
+    }
+    CRM_TRY(launch_sort(ctx, false, &sa));
+    hipLaunchKernelGGL(k_advance_step, dim3(1), dim3(1), 0, ctx->stream, ctx->d_ctrl);
+    CRM_HIP(hipGetLastError());
+    return CRM_OK;
+}
+
 // History entries stored after ctx->step steps and their ring slots, oldest
 // first (the ring slot of push t is t % capacity).
 uint32_t history_slots(const crm_ctx* ctx, uint32_t slot[2]) {
@@ -1056,29 +1198,53 @@ crm_status crm_set_agents(crm_ctx* ctx, const uint64_t* counts, const double* po
 
     const uint64_t Vt = ctx->Vt;
     ctx->n_tiles = (uint32_t)((Vt + kScanTile - 1) / kScanTile);
-    CRM_TRY(dalloc(ctx, ctx->d_work, work.size()));
+    // room for the work blocks of `C` agents (a growing cycle rebuilds the table)
+    {
+        const uint64_t per = is_rod(ctx) ? ctx->rods_per_block : (uint64_t)kBlock;
+        const uint64_t Cw = (is_rod(ctx) && ctx->cycle_on) ? std::max<uint64_t>(N, ctx->cyc.max_agents) : N;
+        CRM_TRY(dalloc(ctx, ctx->d_work, std::max<size_t>(work.size(), Cw / per + ctx->n_inst + 1)));
+    }
     CRM_TRY(dalloc(ctx, ctx->d_start, Vt + 1));
     CRM_TRY(dalloc(ctx, ctx->d_count, 2 * Vt));  // arrivals, then stayers
     CRM_TRY(dalloc(ctx, ctx->d_bsum, ctx->n_tiles));
-    CRM_TRY(dalloc(ctx, ctx->d_key, 2 * N));
-    CRM_TRY(dalloc(ctx, ctx->d_rank, N));
-    CRM_TRY(dalloc(ctx, ctx->d_perm, N));
-    CRM_TRY(dalloc(ctx, ctx->d_force, (size_t)K * N));
+    // N-sized arrays hold `C` agents: the rods of a cycle grow up to max_agents
+    const uint64_t C = (is_rod(ctx) && ctx->cycle_on) ? std::max<uint64_t>(N, ctx->cyc.max_agents) : N;
+    ctx->cap_agents = C;
+    CRM_TRY(dalloc(ctx, ctx->d_key, 2 * C));
+    CRM_TRY(dalloc(ctx, ctx->d_rank, C));
+    CRM_TRY(dalloc(ctx, ctx->d_perm, C));
+    CRM_TRY(dalloc(ctx, ctx->d_force, (size_t)K * C));
     if (is_rod(ctx)) {
-        CRM_TRY(dalloc(ctx, ctx->d_recrod, (size_t)K * N));
-        CRM_TRY(dalloc(ctx, ctx->d_bnd, N));
+        CRM_TRY(dalloc(ctx, ctx->d_recrod, (size_t)K * C));
+        CRM_TRY(dalloc(ctx, ctx->d_bnd, C));
     }
     else if (ctx->cfg.precision == CRM_PRECISION_MIXED) CRM_TRY(dalloc(ctx, ctx->d_recq, N));
     else CRM_TRY(dalloc(ctx, ctx->d_rec64, N));
-    if (ctx->sense_on) CRM_TRY(dalloc(ctx, ctx->d_sense, (size_t)ctx->nv * N));
+    if (ctx->sense_on) CRM_TRY(dalloc(ctx, ctx->d_sense, (size_t)ctx->nv * C));
     for (auto& b : ctx->buf) {
-        CRM_TRY(dalloc(ctx, b.pos, (size_t)K * N));
-        CRM_TRY(dalloc(ctx, b.vel, (size_t)K * N));
-        CRM_TRY(dalloc(ctx, b.hdx, (size_t)2 * K * N));
-        CRM_TRY(dalloc(ctx, b.hdv, (size_t)2 * K * N));
-        CRM_TRY(dalloc(ctx, b.species, N));
-        CRM_TRY(dalloc(ctx, b.id, N));
-        CRM_TRY(dalloc(ctx, b.vox, N));
+        CRM_TRY(dalloc(ctx, b.pos, (size_t)K * C));
+        CRM_TRY(dalloc(ctx, b.vel, (size_t)K * C));
+        CRM_TRY(dalloc(ctx, b.hdx, (size_t)2 * K * C));
+        CRM_TRY(dalloc(ctx, b.hdv, (size_t)2 * K * C));
+        CRM_TRY(dalloc(ctx, b.species, C));
+        CRM_TRY(dalloc(ctx, b.id, C));
+        CRM_TRY(dalloc(ctx, b.vox, C));
+        if (is_rod(ctx)) {
+            CRM_TRY(dalloc(ctx, b.spring, C));
+            CRM_TRY(dalloc(ctx, b.birth, C));
+            CRM_TRY(dalloc(ctx, b.growth, C));
+            CRM_TRY(dalloc(ctx, b.ident, 3 * C));
+            CRM_TRY(dalloc(ctx, b.parent, 3 * C));
+        }
+    }
+    if (is_rod(ctx) && ctx->cycle_on) {
+        CRM_TRY(dalloc(ctx, ctx->d_divflag, C));
+        CRM_TRY(dalloc(ctx, ctx->d_drank, C));
+        CRM_TRY(dalloc(ctx, ctx->d_dctr, C));
+        CRM_TRY(dalloc(ctx, ctx->d_ndiv, Vt + 1));
+        CRM_TRY(dalloc(ctx, ctx->d_dbefore, Vt + 1));
+        CRM_TRY(dalloc(ctx, ctx->d_vcount, Vt));
+        CRM_TRY(alloc_scan_tmp(ctx));
     }
     CRM_HIP(hipMemcpy(ctx->d_work, work.data(), sizeof(BlockWork) * work.size(), hipMemcpyHostToDevice));
     CRM_TRY(upload_tiles(ctx, tiles));
@@ -1104,6 +1270,28 @@ crm_status crm_set_agents(crm_ctx* ctx, const uint64_t* counts, const double* po
         CRM_HIP(hipMemcpyAsync(S.vel, tv.data(), tv.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
         CRM_HIP(hipMemcpyAsync(S.species, sp.data(), N * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
         CRM_HIP(hipMemcpyAsync(S.id, id.data(), N * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+        if (is_rod(ctx)) {
+            // per-rod spring length from the instance, first push at step 0,
+            // growth at the cycle's base rate, CellIdentifier::Initial(index in
+            // the instance), no parent
+            std::vector<double> sl(N), gr(N, ctx->cycle_on ? ctx->cyc.growth_base_rate : 0.0);
+            std::vector<uint32_t> birth(N, 0);
+            std::vector<uint64_t> ident(3 * N, 0), parent(3 * N, 0);
+            uint64_t a = 0;
+            for (uint32_t i = 0; i < ctx->n_inst; i++)
+                for (uint64_t j = 0; j < counts[i]; j++, a++) {
+                    sl[a] = ctx->hparams[i].spring_length;
+                    ident[N + a] = j;
+                    parent[a] = 3;
+                }
+            CRM_HIP(hipMemcpyAsync(S.spring, sl.data(), N * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+            CRM_HIP(hipMemcpyAsync(S.growth, gr.data(), N * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+            CRM_HIP(hipMemcpyAsync(S.birth, birth.data(), N * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+            CRM_HIP(hipMemcpyAsync(S.ident, ident.data(), 3 * N * sizeof(uint64_t), hipMemcpyHostToDevice,
+                                   ctx->stream));
+            CRM_HIP(hipMemcpyAsync(S.parent, parent.data(), 3 * N * sizeof(uint64_t), hipMemcpyHostToDevice,
+                                   ctx->stream));
+        }
         CRM_HIP(hipStreamSynchronize(ctx->stream));
     }
     CRM_TRY(reset_control(ctx, 0));
@@ -1115,6 +1303,9 @@ crm_status crm_set_agents(crm_ctx* ctx, const uint64_t* counts, const double* po
     CRM_HIP(hipGetLastError());
     CRM_TRY(check_errors(ctx, "initial placement"));
     CRM_TRY(launch_sort(ctx, false, nullptr));
+    if (is_rod(ctx) && ctx->cycle_on)
+        hipLaunchKernelGGL(k_init_vcount, dim3((uint32_t)((Vt + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream,
+                           ctx->d_start, ctx->d_vcount, (uint32_t)Vt);
     CRM_HIP(hipStreamSynchronize(ctx->stream));
     ctx->have_agents = true;
     return CRM_OK;
@@ -1139,6 +1330,17 @@ crm_status crm_step(crm_ctx* ctx, uint32_t n_steps, const double* noise) {
             ctx->noise_cap = need;
         }
         CRM_HIP(hipMemcpyAsync(ctx->d_noise, noise, need * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    }
+    if (ctx->cycle_on) {  // the grown order is sized on the host every step
+        for (uint32_t k = 0; k < n_steps; k++) {
+            crm_status st = launch_cycle_step(ctx, use_noise_buf ? ctx->d_noise + per_step * k : nullptr);
+            if (st == CRM_OK) st = check_errors(ctx, "step");
+            if (st) return st;
+            ctx->step += 1;
+            ctx->sense_parity = -1;  // counts are by the pre-step order
+        }
+        CRM_HIP(hipStreamSynchronize(ctx->stream));
+        return CRM_OK;
     }
     const bool graphs = (ctx->cfg.flags & CRM_FLAG_GRAPH) && !timing_on(ctx) && !use_noise_buf;
     uint32_t s = 0;
@@ -1378,6 +1580,66 @@ crm_status crm_set_domain_force(crm_ctx* ctx, uint32_t inst, const crm_domain_fo
     CRM_HIP(hipSetDevice(ctx->device));
     CRM_HIP(hipStreamSynchronize(ctx->stream));
     CRM_HIP(hipMemcpy(ctx->d_ip + inst, &P, sizeof(InstParams), hipMemcpyHostToDevice));
+    return CRM_OK;
+}
+
+crm_status crm_set_rod_cycle(crm_ctx* ctx, const crm_rod_cycle* y) {
+    if (!ctx || !y) return CRM_INVALID_ARGUMENT;
+    if (!is_rod(ctx)) {
+        ctx->err = "the rods example's cycle needs rod agents";
+        return CRM_INVALID_ARGUMENT;
+    }
+    if (ctx->have_agents) {
+        ctx->err = "crm_set_rod_cycle must come before crm_set_agents (it sizes the context for max_agents rods)";
+        return CRM_INVALID_ARGUMENT;
+    }
+    const bool ok = std::isfinite(y->growth_base_rate) && std::isfinite(y->division_threshold) &&
+                    std::isfinite(y->division_radius) && std::isfinite(y->sensing_radius) &&
+                    y->neighbor_saturation > 0.0 && std::isfinite(y->neighbor_saturation) && y->max_agents > 0 &&
+                    y->max_agents < 0xffffffffull;
+    if (!ok) {
+        ctx->err = "invalid cycle parameters";
+        return CRM_INVALID_ARGUMENT;
+    }
+    CRM_TRY(crm_set_neighbor_sensing(ctx, 1, y->sensing_radius));
+    ctx->cyc = *y;
+    ctx->cycle_on = true;
+    return CRM_OK;
+}
+
+crm_status crm_set_rod_spring_lengths(crm_ctx* ctx, const double* sl) {
+    if (!ctx || !sl || !ctx->have_agents || !is_rod(ctx)) return CRM_INVALID_ARGUMENT;
+    CRM_HIP(hipSetDevice(ctx->device));
+    std::vector<uint32_t> id;
+    CRM_TRY(download_ids(ctx, id));
+    std::vector<double> v(ctx->N);
+    for (uint64_t s = 0; s < ctx->N; s++) v[s] = sl[id[s]];
+    CRM_HIP(hipMemcpyAsync(ctx->buf[ctx->cur].spring, v.data(), ctx->N * sizeof(double), hipMemcpyHostToDevice,
+                           ctx->stream));
+    CRM_HIP(hipStreamSynchronize(ctx->stream));
+    return CRM_OK;
+}
+
+crm_status crm_get_cycle_state(crm_ctx* ctx, double* spring_length, double* growth_rate, uint64_t* identifier,
+                               uint64_t* parent) {
+    if (!ctx || !ctx->have_agents || !is_rod(ctx)) return CRM_INVALID_ARGUMENT;
+    CRM_HIP(hipSetDevice(ctx->device));
+    std::vector<uint32_t> id;
+    CRM_TRY(download_ids(ctx, id));
+    const uint64_t N = ctx->N;
+    const StateBuf& S = ctx->buf[ctx->cur];
+    if (spring_length) CRM_TRY(download_by_id(ctx, S.spring, 1, spring_length, id));
+    if (growth_rate) CRM_TRY(download_by_id(ctx, S.growth, 1, growth_rate, id));
+    std::vector<uint64_t> tmp(3 * N);
+    uint64_t* outs[2] = {identifier, parent};
+    uint64_t* srcs[2] = {S.ident, S.parent};
+    for (int q = 0; q < 2; q++) {
+        if (!outs[q]) continue;
+        CRM_HIP(hipMemcpyAsync(tmp.data(), srcs[q], 3 * N * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+        CRM_HIP(hipStreamSynchronize(ctx->stream));
+        for (uint64_t s = 0; s < N; s++)
+            for (int j = 0; j < 3; j++) outs[q][(uint64_t)id[s] * 3 + j] = tmp[(uint64_t)j * N + s];
+    }
     return CRM_OK;
 }
 

@@ -31,6 +31,14 @@ struct StateBuf {
     int32_t* species;
     uint32_t* id;   // agent index
     uint32_t* vox;  // global voxel of the slot's agent
+    // rods only (null for points): RodMechanics::spring_length per rod, the step
+    // of the rod's first history push (0, or the step after its division), and
+    // the cycle state of the rods example
+    double* spring;   // [N]
+    uint32_t* birth;  // [N]
+    double* growth;   // [N]
+    uint64_t* ident;  // [3][N] CellIdentifier (kind, a, b)
+    uint64_t* parent; // [3][N]
 };
 
 struct Arrays {

@@ -432,8 +432,8 @@ __device__ __forceinline__ double rod_normal(const StepArgs& sa, uint32_t id, ui
 // bending and damping, accumulated in the reference's order.  pos(i, k) and
 // vel(i, k) read vertex i of the rod.
 template <int D, typename Pos, typename Vel>
-__device__ __forceinline__ void rod_row_increment(const InstParams& P, uint32_t nv, uint32_t v, Pos pos, Vel vel,
-                                                  const double* force, double* dx, double* dv) {
+__device__ __forceinline__ void rod_row_increment(const InstParams& P, double spring_length, uint32_t nv, uint32_t v,
+                                                  Pos pos, Vel vel, const double* force, double* dx, double* dv) {
     double total[D];
 #pragma unroll
     for (int k = 0; k < D; k++) total[k] = force[k];
@@ -450,7 +450,7 @@ __device__ __forceinline__ void rod_row_increment(const InstParams& P, uint32_t 
         const double nd = sqrt(n2);
         if (nd == 0.0) return false;
 #pragma unroll
-        for (int k = 0; k < D; k++) fi[k] = ((-(d[k] / nd)) * P.spring_tension) * (nd - P.spring_length);
+        for (int k = 0; k < D; k++) fi[k] = ((-(d[k] / nd)) * P.spring_tension) * (nd - spring_length);
         return true;
     };
     // springs i = v - 1 (v is its second vertex) then i = v (v is its first vertex)
@@ -662,6 +662,20 @@ __global__ __launch_bounds__(kBlock) void k_key_rod(Arrays A, StateBuf S, RodArg
     A.rank[s] = atomicAdd(&A.count[key], 1u);
 }
 
+// Per-rod fields that follow the rod through a sort or a re-layout (strides of
+// the source and destination buffers may differ).
+__device__ __forceinline__ void copy_rod_extras(const StateBuf& src, uint64_t o, const StateBuf& dst, uint64_t s,
+                                                uint64_t n_src, uint64_t n_dst) {
+    dst.spring[s] = src.spring[o];
+    dst.birth[s] = src.birth[o];
+    dst.growth[s] = src.growth[o];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        dst.ident[j * n_dst + s] = src.ident[j * n_src + o];
+        dst.parent[j * n_dst + s] = src.parent[j * n_src + o];
+    }
+}
+
 // Gather + integrate for rods: lane (s, v) computes vertex row v of the rod that
 // lands in sorted slot s (coming from slot o = perm[s]).
 template <int D, bool INTEGRATE>
@@ -691,7 +705,7 @@ __global__ __launch_bounds__(kBlock) void k_gather_rod(Arrays A, StateBuf src, S
                 vv[k] = vel((int)v, k);
                 f[k] = A.force[(uint64_t)(v * D + k) * N + o];
             }
-            rod_row_increment<D>(P, nv, v, pos, vel, f, dx, dv);
+            rod_row_increment<D>(P, src.spring[o], nv, v, pos, vel, f, dx, dv);
             rod_row_noise<D>(P, sa, id, t, nv, v, dxr);
             auto hist = [&](int i, int k, bool is_dv) {
                 const double* h = is_dv ? src.hdv : src.hdx;
@@ -711,7 +725,8 @@ __global__ __launch_bounds__(kBlock) void k_gather_rod(Arrays A, StateBuf src, S
                 }
             }
         } else {
-            const uint32_t n_hist = (uint32_t)min<uint64_t>(A.ctrl->step, sa.cap);
+            // every ring slot: rods born at different steps hold different counts
+            const uint32_t n_hist = sa.cap;
 #pragma unroll
             for (int k = 0; k < D; k++) {
                 nx[k] = src.pos[(uint64_t)(v * D + k) * N + o];
@@ -737,6 +752,7 @@ __global__ __launch_bounds__(kBlock) void k_gather_rod(Arrays A, StateBuf src, S
             dst.species[s] = src.species[o];
             dst.id[s] = id;
             dst.vox[s] = A.key[o];
+            copy_rod_extras(src, o, dst, s, N, N);
         }
     }
     __syncthreads();

@@ -64,7 +64,7 @@ struct Control {
     uint64_t step;
     uint64_t error_key;     // min over (step << 32 | agent index) of failing agents
     uint32_t error_flags;   // bit (1 << crm_status) per error kind seen
-    uint32_t pad;
+    uint32_t n_div;         // rods dividing in the current step (cycle)
     uint64_t out_of_grid;   // agents that kept their voxel because the new index had no voxel
 };
 

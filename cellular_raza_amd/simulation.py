@@ -99,6 +99,7 @@ class MechanicsBackend:
         # also read / update that JSON cache of earlier choices
         self.autotune = autotune
         self.tiling_policy: Optional[int] = None
+        self.cycle = None
 
     # -- helpers -----------------------------------------------------------
     def _message(self, h) -> str:
@@ -167,7 +168,36 @@ class MechanicsBackend:
             nz = np.ascontiguousarray(noise, dtype=np.float64)
             if nz.size != n_steps * self.n_agents * self.K:
                 raise ValueError("noise must hold n_steps * n_agents * n_vertices * dim values")
-        self._check(self._lib.crm_step(self._h, n_steps, _dp(nz)))
+        st = self._lib.crm_step(self._h, n_steps, _dp(nz))
+        if self.cycle is not None:  # divisions add rods
+            inf = _abi.Info()
+            if self._lib.crm_get_info(self._h, C.byref(inf)) == _abi.CRM_OK:
+                self.n_agents = int(inf.n_agents)
+        self._check(st)
+
+    def set_rod_cycle(self, cycle) -> None:
+        """The rods example's Cycle + NeighborSensing (crm_set_rod_cycle, a
+        RodGrowthCycle); call before set_agents, which sizes the context for
+        cycle.max_agents rods."""
+        self._check(self._lib.crm_set_rod_cycle(self._h, C.byref(cycle.to_struct())))
+        self.cycle = cycle
+
+    def set_rod_spring_lengths(self, spring_length) -> None:
+        sl = np.ascontiguousarray(spring_length, dtype=np.float64)
+        if sl.shape != (self.n_agents,):
+            raise ValueError("one spring length per rod")
+        self._check(self._lib.crm_set_rod_spring_lengths(self._h, _dp(sl)))
+
+    def cycle_state(self) -> Dict[str, np.ndarray]:
+        """Per rod by agent index: spring_length, growth_rate, identifier and parent
+        as (kind, a, b) rows (kind 0 Initial(a), 1 Division(a = voxel, b = counter),
+        3 no parent)."""
+        n = self.n_agents
+        sl, gr = np.empty(n), np.empty(n)
+        ident, parent = np.empty((n, 3), dtype=np.uint64), np.empty((n, 3), dtype=np.uint64)
+        self._check(self._lib.crm_get_cycle_state(self._h, _dp(sl), _dp(gr), ident.ctypes.data_as(_abi.U64P),
+                                                  parent.ctypes.data_as(_abi.U64P)))
+        return {"spring_length": sl, "growth_rate": gr, "identifier": ident, "parent": parent}
 
     def compute_forces(self) -> np.ndarray:
         out = np.empty((self.n_agents, self.K), dtype=np.float64)
