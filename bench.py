@@ -73,6 +73,42 @@ def force_bytes_per_agent(cbar, record_bytes, dim, stencil, force_bytes):
     return record_bytes + 8 * stencil + cbar * record_bytes + force_bytes
 
 
+def time_rods_example():
+    """The bacterial rods example end to end (5 rods growing and dividing for 25 h of
+    simulated time, dt 0.1 min): GPU wall clock for all 15000 steps, and for the
+    first 5000 beside the oracle's single-thread run of the same 5000 steps."""
+    from cellular_raza_amd import MechanicsBackend, workloads
+    from oracle.oracle import OracleSim
+
+    ex = workloads.bacterial_rods_example()
+    wl = ex.workload
+
+    def gpu():
+        g = MechanicsBackend(wl.instances, dt=wl.dt, mechanics_solver_order=2)
+        g.set_rod_cycle(ex.cycle)
+        g.set_agents(wl.pos.reshape(len(wl.pos), -1), wl.vel.reshape(len(wl.pos), -1))
+        g.set_rod_spring_lengths(ex.spring_length)
+        return g
+
+    g = gpu()
+    t0 = time.perf_counter()
+    g.step(5000)
+    t_5000 = time.perf_counter() - t0
+    n_5000 = g.n_agents
+    g.step(10000)
+    t_all = time.perf_counter() - t0
+    n_all = g.n_agents
+    g.close()
+    o = OracleSim(wl.instances, dt=wl.dt, mechanics_solver_order=2)
+    o.set_agents(wl.pos.reshape(len(wl.pos), -1), wl.vel.reshape(len(wl.pos), -1))
+    o.set_rod_cycle(ex.cycle)
+    o.set_spring_lengths(ex.spring_length)
+    t_cpu = o.step_timed(5000)
+    return {"steps": 15000, "gpu_s": t_all, "rods_at_end": n_all, "gpu_s_first_5000": t_5000,
+            "rods_at_5000": n_5000, "oracle_s_first_5000": t_cpu, "oracle_rods_at_5000": o.n_agents,
+            "oracle_threads": 1}
+
+
 def time_snapshot(wl, precision):
     """One save point after the history ring has filled: crm_get_snapshot (device gather
     + copy of the agent-order records) and crm_save_cells_json (gather + threaded JSON
@@ -228,6 +264,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--seq-steps", type=int, default=100, help="c4: steps of the one-after-another run")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--rods-example", action="store_true",
+                    help="also time the bacterial rods example (Cycle + NeighborSensing, 25 h = 15000 steps) on the "
+                         "GPU, and its first 5000 steps on the GPU and on the oracle")
     ap.add_argument("--snapshot", action="store_true",
                     help="also time one save point: the device gather into agent order with its copy to the host, "
                          "and the cells written in the reference's JSON layout (to a temporary folder)")
@@ -286,6 +325,9 @@ def main():
         extra["batched_over_sequential"] = value / seq_value
         extra["sequential_sample"] = f"{len(wl.instances)} contexts one after another, {args.seq_steps} steps each"
         print(f"bench: sequential {seq_value:.4e} agent-steps/s", file=sys.stderr, flush=True)
+    if args.rods_example:
+        extra["rods_example"] = time_rods_example()
+        print(f"bench: rods example {extra['rods_example']}", file=sys.stderr, flush=True)
     if args.snapshot:
         extra["snapshot"] = time_snapshot(wl, main_prec)
         print(f"bench: snapshot {extra['snapshot']}", file=sys.stderr, flush=True)

@@ -198,7 +198,7 @@ SIGNATURES = {
     "crm_set_domain_force": (C.c_int, [P, C.c_uint32, C.POINTER(DomainForce)]),
     "crm_set_rod_cycle": (C.c_int, [P, C.POINTER(RodCycle)]),
     "crm_set_rod_spring_lengths": (C.c_int, [P, DP]),
-    "crm_get_cycle_state": (C.c_int, [P, DP, DP, U64P, U64P]),
+    "crm_get_cycle_state": (C.c_int, [P, DP, DP, U64P, U64P, U32P]),
     "crm_save_cells_json": (C.c_int, [P, C.c_char_p, C.c_uint64, C.POINTER(JsonCells)]),
     "crm_get_timing": (C.c_int, [P, C.POINTER(Timing)]),
     "crm_reset_timing": (C.c_int, [P]),

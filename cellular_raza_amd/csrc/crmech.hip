@@ -194,6 +194,9 @@ struct crm_ctx {
     // snapshot records (agent order) and species, allocated on the first save
     double* d_snap = nullptr;
     int32_t* d_snap_sp = nullptr;
+    double* d_snap_rod = nullptr;     // [N][2] rods: spring length, growth rate
+    uint64_t* d_snap_id = nullptr;    // [N][6] rods: identifier, parent
+    uint32_t* d_snap_st = nullptr;    // [N] rods: history entries held
     size_t snap_cap = 0;
 
     uint64_t N = 0;
@@ -219,7 +222,7 @@ thread_local std::string g_create_error;
 void free_device(crm_ctx* ctx) {
     void* ptrs[] = {ctx->d_work, ctx->d_tiles, ctx->d_start, ctx->d_count, ctx->d_bsum, ctx->d_key, ctx->d_rank,
                     ctx->d_perm, ctx->d_force, ctx->d_rec64, ctx->d_recq, ctx->d_noise, ctx->d_recrod, ctx->d_bnd,
-                    ctx->d_sense, ctx->d_snap, ctx->d_snap_sp, ctx->d_divflag, ctx->d_drank, ctx->d_dctr,
+                    ctx->d_sense, ctx->d_snap, ctx->d_snap_sp, ctx->d_snap_rod, ctx->d_snap_id, ctx->d_snap_st, ctx->d_divflag, ctx->d_drank, ctx->d_dctr,
                     ctx->d_ndiv, ctx->d_dbefore, ctx->d_vcount, ctx->d_scan_tmp};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -243,6 +246,9 @@ void free_device(crm_ctx* ctx) {
     ctx->sense_parity = -1;
     ctx->d_snap = nullptr;
     ctx->d_snap_sp = nullptr;
+    ctx->d_snap_rod = nullptr;
+    ctx->d_snap_id = nullptr;
+    ctx->d_snap_st = nullptr;
     ctx->snap_cap = 0;
     ctx->d_divflag = ctx->d_drank = nullptr;
     ctx->d_dctr = nullptr;
@@ -890,15 +896,30 @@ crm_status gather_snapshot_device(crm_ctx* ctx, uint32_t& n_hist, size_t& len_ou
     n_hist = a.n_hist;
     const size_t len = (size_t)ctx->N * ctx->K * (2u + 2u * a.n_hist);
     if (len > ctx->snap_cap) {
-        if (ctx->d_snap) CRM_HIP(hipFree(ctx->d_snap));
+        void* old[] = {ctx->d_snap, ctx->d_snap_sp, ctx->d_snap_rod, ctx->d_snap_id, ctx->d_snap_st};
+        for (void* p : old)
+            if (p) CRM_HIP(hipFree(p));
         ctx->d_snap = nullptr;
+        ctx->d_snap_sp = nullptr;
+        ctx->d_snap_rod = nullptr;
+        ctx->d_snap_id = nullptr;
+        ctx->d_snap_st = nullptr;
         ctx->snap_cap = 0;
         CRM_TRY(dalloc(ctx, ctx->d_snap, len));
+        CRM_TRY(dalloc(ctx, ctx->d_snap_sp, ctx->N));
+        if (is_rod(ctx)) {
+            CRM_TRY(dalloc(ctx, ctx->d_snap_rod, 2 * ctx->N));
+            CRM_TRY(dalloc(ctx, ctx->d_snap_id, 6 * ctx->N));
+            CRM_TRY(dalloc(ctx, ctx->d_snap_st, ctx->N));
+        }
         ctx->snap_cap = len;
     }
-    if (!ctx->d_snap_sp) CRM_TRY(dalloc(ctx, ctx->d_snap_sp, ctx->N));
     a.out = ctx->d_snap;
     a.species = ctx->d_snap_sp;
+    a.rod = ctx->d_snap_rod;
+    a.ident = ctx->d_snap_id;
+    a.stored = ctx->d_snap_st;
+    a.step = ctx->step;
     hipLaunchKernelGGL(k_snapshot, dim3((uint32_t)((ctx->N + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream,
                        ctx->buf[ctx->cur], a);
     CRM_HIP(hipGetLastError());
@@ -965,21 +986,26 @@ void put_vec(std::string& o, const double* v, uint32_t nv, uint32_t D, bool matr
     o += "], " + std::to_string(nv) + ", " + std::to_string(D) + "]";
 }
 
-// A cell template split at its {pos} / {vel} placeholders.
+// A cell template split at its placeholders {pos}, {vel}, {spring_length},
+// {growth_rate} (the last two: the rod's values).
 struct Template {
     std::vector<std::string> lit;  // lit[0] hole[0] lit[1] hole[1] ... lit[n]
-    std::vector<int> hole;         // 0 pos, 1 vel
+    std::vector<int> hole;         // 0 pos, 1 vel, 2 spring length, 3 growth rate
 };
 
 Template split_template(const char* t) {
+    static const char* names[4] = {"{pos}", "{vel}", "{spring_length}", "{growth_rate}"};
     Template T;
     std::string cur;
     for (const char* p = t; *p;) {
-        if (std::strncmp(p, "{pos}", 5) == 0 || std::strncmp(p, "{vel}", 5) == 0) {
+        int h = -1;
+        for (int q = 0; q < 4 && h < 0; q++)
+            if (std::strncmp(p, names[q], std::strlen(names[q])) == 0) h = q;
+        if (h >= 0) {
             T.lit.push_back(cur);
             cur.clear();
-            T.hole.push_back(p[1] == 'p' ? 0 : 1);
-            p += 5;
+            T.hole.push_back(h);
+            p += std::strlen(names[h]);
         } else {
             cur += *p++;
         }
@@ -988,29 +1014,45 @@ Template split_template(const char* t) {
     return T;
 }
 
+// CellIdentifier as serde writes the externally tagged enum: {"Initial": a},
+// {"Division": [a, b]}; kind 3 (no parent) is null.
+std::string ident_json(const uint64_t* x) {
+    if (x[0] == 0) return "{\"Initial\": " + std::to_string(x[1]) + "}";
+    if (x[0] == 1) return "{\"Division\": [" + std::to_string(x[1]) + ", " + std::to_string(x[2]) + "]}";
+    if (x[0] == 2) return "{\"Inserted\": [" + std::to_string(x[1]) + ", " + std::to_string(x[2]) + "]}";
+    return "null";
+}
+
 // One data entry of the batch file: CombinedSaveFormat { identifier, element:
 // (CellBox { identifier, parent, cell }, AuxStorage { mechanics }) }
 // (storage/concepts.rs:176-190, cell.rs:128-137, aux_storage.rs:41-46).
+// rods: ids [6] (identifier, parent), rv [2] (spring length, growth rate) and the
+// history entries the rod holds (the newest `held` of the n_hist slots); points:
+// null, Initial(agent index), every entry
 void put_cell(std::string& o, uint64_t a, const double* r, const Template& T, uint32_t nv, uint32_t D,
-              uint32_t n_hist) {
+              uint32_t n_hist, const uint64_t* ids, const double* rv, uint32_t held) {
     const uint32_t K = nv * D;
     const bool matrix = nv > 1;
-    const std::string ident = "{\"Initial\": " + std::to_string(a) + "}";
+    const uint64_t initial[3] = {0, a, 0}, none[3] = {3, 0, 0};
+    const std::string ident = ident_json(ids ? ids : initial), parent = ident_json(ids ? ids + 3 : none);
     o += "    {\n      \"identifier\": " + ident + ",\n      \"element\": [\n        {\n          \"identifier\": " +
-         ident + ",\n          \"parent\": null,\n          \"cell\": ";
+         ident + ",\n          \"parent\": " + parent + ",\n          \"cell\": ";
     for (size_t i = 0; i < T.hole.size(); i++) {
         o += T.lit[i];
-        put_vec(o, r + (T.hole[i] == 0 ? 0u : K), nv, D, matrix);
+        const int h = T.hole[i];
+        if (h <= 1) put_vec(o, r + (h == 0 ? 0u : K), nv, D, matrix);
+        else put_num(o, rv ? rv[h - 2] : 0.0);
     }
     o += T.lit.back();
     o += "\n        },\n        {\n          \"mechanics\": {\n            \"positions\": [";
-    for (uint32_t i = 0; i < n_hist; i++) {
-        if (i) o += ", ";
+    const uint32_t first = ids ? n_hist - std::min(held, n_hist) : 0u;
+    for (uint32_t i = first; i < n_hist; i++) {
+        if (i > first) o += ", ";
         put_vec(o, r + 2u * K + i * K, nv, D, matrix);
     }
     o += "],\n            \"velocities\": [";
-    for (uint32_t i = 0; i < n_hist; i++) {
-        if (i) o += ", ";
+    for (uint32_t i = first; i < n_hist; i++) {
+        if (i > first) o += ", ";
         put_vec(o, r + 2u * K + (n_hist + i) * K, nv, D, matrix);
     }
     std::vector<double> zero(K, 0.0);
@@ -1621,7 +1663,7 @@ crm_status crm_set_rod_spring_lengths(crm_ctx* ctx, const double* sl) {
 }
 
 crm_status crm_get_cycle_state(crm_ctx* ctx, double* spring_length, double* growth_rate, uint64_t* identifier,
-                               uint64_t* parent) {
+                               uint64_t* parent, uint32_t* history_held) {
     if (!ctx || !ctx->have_agents || !is_rod(ctx)) return CRM_INVALID_ARGUMENT;
     CRM_HIP(hipSetDevice(ctx->device));
     std::vector<uint32_t> id;
@@ -1639,6 +1681,16 @@ crm_status crm_get_cycle_state(crm_ctx* ctx, double* spring_length, double* grow
         CRM_HIP(hipStreamSynchronize(ctx->stream));
         for (uint64_t s = 0; s < N; s++)
             for (int j = 0; j < 3; j++) outs[q][(uint64_t)id[s] * 3 + j] = tmp[(uint64_t)j * N + s];
+    }
+    if (history_held) {
+        std::vector<uint32_t> birth(N);
+        CRM_HIP(hipMemcpyAsync(birth.data(), S.birth, N * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+        CRM_HIP(hipStreamSynchronize(ctx->stream));
+        const uint64_t cap = ctx->cfg.solver_order;
+        for (uint64_t s = 0; s < N; s++) {
+            const uint64_t pushes = ctx->step > birth[s] ? ctx->step - birth[s] : 0;
+            history_held[id[s]] = (uint32_t)std::min(pushes, cap);
+        }
     }
     return CRM_OK;
 }
@@ -1729,6 +1781,21 @@ crm_status crm_save_cells_json(crm_ctx* ctx, const char* path, uint64_t iteratio
     uint32_t nh = 0;
     CRM_TRY(gather_snapshot(ctx, rec, sp, nh));
     const uint64_t N = ctx->N;
+    std::vector<double> rodv;
+    std::vector<uint64_t> ids;
+    std::vector<uint32_t> held;
+    if (is_rod(ctx)) {
+        rodv.resize(2 * N);
+        ids.resize(6 * N);
+        held.resize(N);
+        CRM_HIP(hipMemcpyAsync(rodv.data(), ctx->d_snap_rod, rodv.size() * sizeof(double), hipMemcpyDeviceToHost,
+                               ctx->stream));
+        CRM_HIP(hipMemcpyAsync(ids.data(), ctx->d_snap_id, ids.size() * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                               ctx->stream));
+        CRM_HIP(hipMemcpyAsync(held.data(), ctx->d_snap_st, N * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                               ctx->stream));
+        CRM_HIP(hipStreamSynchronize(ctx->stream));
+    }
     if (spec->n_templates > 1)
         for (uint64_t a = 0; a < N; a++)
             if (sp[a] < 0 || (uint32_t)sp[a] >= spec->n_templates) {
@@ -1767,7 +1834,9 @@ crm_status crm_save_cells_json(crm_ctx* ctx, const char* path, uint64_t iteratio
         o.reserve((a1 - a0) * (400 + 60 * rec_len));
         for (uint64_t a = a0; a < a1; a++) {
             if (a) o += ",\n";
-            put_cell(o, a, rec.data() + a * rec_len, T[spec->n_templates > 1 ? sp[a] : 0], nv, D, nh);
+            const bool rod = !ids.empty();
+            put_cell(o, a, rec.data() + a * rec_len, T[spec->n_templates > 1 ? sp[a] : 0], nv, D, nh,
+                     rod ? ids.data() + 6 * a : nullptr, rod ? rodv.data() + 2 * a : nullptr, rod ? held[a] : nh);
         }
     };
     std::vector<std::thread> pool;

@@ -14,6 +14,10 @@ namespace crm {
 struct SnapshotArgs {
     double* out;          // [N][rec]
     int32_t* species;     // [N] agent-order species (may be null)
+    double* rod;          // rods (may be null): [N][2] spring length, growth rate
+    uint64_t* ident;      // rods (may be null): [N][6] identifier (kind, a, b), parent (kind, a, b)
+    uint32_t* stored;     // rods (may be null): [N] history entries the rod holds (a daughter holds fewer)
+    uint64_t step;        // completed steps
     uint64_t N;
     uint32_t K, n_hist;   // values per vector, stored history entries (0..2)
     uint32_t slot[2];     // ring slots of the history entries, oldest first
@@ -38,6 +42,21 @@ __global__ __launch_bounds__(kBlock) void k_snapshot(StateBuf S, SnapshotArgs a)
         }
     }
     if (a.species) a.species[id] = S.species[s];
+    if (a.rod) {
+        a.rod[2ull * id] = S.spring[s];
+        a.rod[2ull * id + 1] = S.growth[s];
+    }
+    if (a.stored) {
+        const uint64_t pushes = a.step > S.birth[s] ? a.step - S.birth[s] : 0;
+        a.stored[id] = (uint32_t)min<uint64_t>(pushes, a.n_hist);
+    }
+    if (a.ident) {
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            a.ident[6ull * id + j] = S.ident[(uint64_t)j * N + s];
+            a.ident[6ull * id + 3 + j] = S.parent[(uint64_t)j * N + s];
+        }
+    }
 }
 
 }  // namespace crm

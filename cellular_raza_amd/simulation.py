@@ -195,9 +195,12 @@ class MechanicsBackend:
         n = self.n_agents
         sl, gr = np.empty(n), np.empty(n)
         ident, parent = np.empty((n, 3), dtype=np.uint64), np.empty((n, 3), dtype=np.uint64)
+        held = np.empty(n, dtype=np.uint32)
         self._check(self._lib.crm_get_cycle_state(self._h, _dp(sl), _dp(gr), ident.ctypes.data_as(_abi.U64P),
-                                                  parent.ctypes.data_as(_abi.U64P)))
-        return {"spring_length": sl, "growth_rate": gr, "identifier": ident, "parent": parent}
+                                                  parent.ctypes.data_as(_abi.U64P),
+                                                  held.ctypes.data_as(_abi.U32P)))
+        return {"spring_length": sl, "growth_rate": gr, "identifier": ident, "parent": parent,
+                "history_held": held}
 
     def compute_forces(self) -> np.ndarray:
         out = np.empty((self.n_agents, self.K), dtype=np.float64)
@@ -339,6 +342,9 @@ def run_simulation(
     species=None,
     mechanics_solver_order: int = 2,
     domain_force=None,
+    cycle=None,
+    spring_lengths=None,
+    cell_extra=None,
     **backend_kwargs,
 ) -> Dict[int, np.ndarray]:
     """Runs the [Mechanics, Interaction] aspects.  Without settings.storage it returns
@@ -353,7 +359,11 @@ def run_simulation(
         **backend_kwargs,
     )
     try:
+        if cycle is not None:  # the aspects Cycle + NeighborSensing of the rods example
+            be.set_rod_cycle(cycle)
         be.set_agents(positions, velocities, species)
+        if spring_lengths is not None:
+            be.set_rod_spring_lengths(spring_lengths)
         out: Dict[int, np.ndarray] = {}
         access = None
         if settings.storage is not None:
@@ -366,7 +376,7 @@ def run_simulation(
                 subdomains=st.StorageManager.open_or_create(builder.suffix(os.path.join(base, "subdomains")), 0),
             )
             n_sp = int(np.max(species)) + 1 if species is not None and len(species) else 1
-            templates = st.cell_templates(mechanics, interaction, n_species=n_sp)
+            templates = st.cell_templates(mechanics, interaction, n_species=n_sp, extra=cell_extra)
             sub = st.subdomain_element(domain)
         it = 0
         for target in settings.time.save_iterations:
@@ -404,13 +414,23 @@ def _store_cells(be: MechanicsBackend, cells, iteration: int, templates: Sequenc
             return [x.reshape(nv, dim).T.ravel().tolist(), nv, dim] if matrix else x.tolist()
 
         zero = vec(np.zeros(be.K))
+        cyc = be.cycle_state() if matrix else None
+        n_hist = snap["dx"].shape[1]
+        held = np.full(be.n_agents, n_hist)
+        if cyc is not None:  # rods born later hold fewer history entries
+            held = np.minimum(cyc["history_held"].astype(np.int64), n_hist)
         pairs = []
         for a in range(be.n_agents):
             t = templates[int(snap["species"][a]) if len(templates) > 1 else 0]
-            cellbox = {"identifier": {"Initial": a}, "parent": None,
-                       "cell": st.fill_template(t, shaped(snap["pos"][a]), shaped(snap["vel"][a]), matrix)}
-            aux = {"mechanics": {"positions": [vec(x) for x in snap["dx"][a]],
-                                 "velocities": [vec(x) for x in snap["dv"][a]],
+            ident = {"Initial": a} if cyc is None else st.identifier_json(cyc["identifier"][a])
+            parent = None if cyc is None else st.identifier_json(cyc["parent"][a])
+            sl = 0.0 if cyc is None else cyc["spring_length"][a]
+            gr = 0.0 if cyc is None else cyc["growth_rate"][a]
+            cellbox = {"identifier": ident, "parent": parent,
+                       "cell": st.fill_template(t, shaped(snap["pos"][a]), shaped(snap["vel"][a]), matrix, sl, gr)}
+            first = n_hist - int(held[a])
+            aux = {"mechanics": {"positions": [vec(x) for x in snap["dx"][a][first:]],
+                                 "velocities": [vec(x) for x in snap["dv"][a][first:]],
                                  "current_force": zero, "zero_force": zero}}
-            pairs.append((("Initial", a), [cellbox, aux]))
+            pairs.append((st._to_key(ident), [cellbox, aux]))
         store.store_batch_elements(iteration, pairs)

@@ -336,6 +336,9 @@ class StorageAccess:
 # Cell text from the building blocks' serde fields
 # --------------------------------------------------------------------------
 _POS, _VEL = "__crm_pos__", "__crm_vel__"
+# values of extra fields that the writer fills per rod
+SPRING_LENGTH, GROWTH_RATE = "__crm_spring_length__", "__crm_growth_rate__"
+_HOLES = {_POS: "{pos}", _VEL: "{vel}", SPRING_LENGTH: "{spring_length}", GROWTH_RATE: "{growth_rate}"}
 
 
 def mechanics_fields(m) -> Dict[str, Any]:
@@ -347,9 +350,9 @@ def mechanics_fields(m) -> Dict[str, Any]:
         return {"pos": _POS, "vel": _VEL, "mass": m.mass, "damping": m.damping, "kb_temperature": m.kb_temperature}
     if isinstance(m, bb.Brownian):
         return {"pos": _POS, "diffusion_constant": m.diffusion_constant, "kb_temperature": m.kb_temperature}
-    if isinstance(m, bb.RodMechanics):
+    if isinstance(m, bb.RodMechanics):  # spring_length per rod
         return {"pos": _POS, "vel": _VEL, "diffusion_constant": m.diffusion_constant,
-                "spring_tension": m.spring_tension, "rigidity": m.rigidity, "spring_length": m.spring_length,
+                "spring_tension": m.spring_tension, "rigidity": m.rigidity, "spring_length": SPRING_LENGTH,
                 "damping": m.damping}
     raise StorageError(f"no serde layout for mechanics {type(m).__name__}")
 
@@ -386,7 +389,8 @@ def cell_templates(mechanics, interaction, n_species: int = 1,
     """JSON text of the agent struct per species, {pos} / {vel} where the state goes.
     The default field names are those of the benchmarks' and examples' agents
     (`Cell { mechanics, interaction }`, cell_sorting.rs:96-102); `extra` appends
-    further fields (e.g. the rods example's growth_rate)."""
+    further fields; the values SPRING_LENGTH and GROWTH_RATE stand for the rod's own
+    (e.g. the rods example's `growth_rate: GROWTH_RATE, spring_length_threshold: 6e-6`)."""
     out = []
     for s in range(max(1, n_species)):
         cell = {mechanics_field: mechanics_fields(mechanics),
@@ -394,18 +398,35 @@ def cell_templates(mechanics, interaction, n_species: int = 1,
         if extra:
             cell.update(_jsonable(extra))
         text = json.dumps(cell, indent=2).replace("\n", "\n          ")
-        out.append(text.replace(f'"{_POS}"', "{pos}").replace(f'"{_VEL}"', "{vel}"))
+        for key, hole in _HOLES.items():
+            text = text.replace(f'"{key}"', hole)
+        out.append(text)
     return out
 
 
-def fill_template(template: str, pos: np.ndarray, vel: np.ndarray, matrix: bool) -> Any:
+def fill_template(template: str, pos: np.ndarray, vel: np.ndarray, matrix: bool, spring_length: float = 0.0,
+                  growth_rate: float = 0.0) -> Any:
     """The cell object of a template with its state filled in (the Memory option)."""
     def vec(v):
         if not matrix:
             return json.dumps([float(x) for x in v])
         nv, d = v.shape
         return json.dumps([[float(x) for x in v.T.ravel()], nv, d])
-    return json.loads(template.replace("{pos}", vec(pos)).replace("{vel}", vec(vel)))
+    text = template.replace("{pos}", vec(pos)).replace("{vel}", vec(vel))
+    text = text.replace("{spring_length}", json.dumps(float(spring_length)))
+    return json.loads(text.replace("{growth_rate}", json.dumps(float(growth_rate))))
+
+
+def identifier_json(row) -> Any:
+    """(kind, a, b) -> CellIdentifier as serde writes it; kind 3 -> None."""
+    kind, a, b = (int(x) for x in row)
+    if kind == 0:
+        return {"Initial": a}
+    if kind == 1:
+        return {"Division": [a, b]}
+    if kind == 2:
+        return {"Inserted": [a, b]}
+    return None
 
 
 def subdomain_element(domain: bb.CartesianCuboid) -> Dict[str, Any]:

@@ -121,6 +121,54 @@ def c3_rods_2d(n: int = 200_000, seed: int = 3, steps: int = 500, n_vertices: in
                     np.zeros_like(pos), None, [n], noise="philox" if diffusion_constant > 0 else "none")
 
 
+@dataclass
+class RodsExample:
+    """The bacterial rods example: workload, per-rod spring lengths and the cycle."""
+
+    workload: Workload
+    spring_length: np.ndarray
+    cycle: object
+
+
+def bacterial_rods_example(seed: int = 5, steps: int = 15000, n_rods: int = 5,
+                           max_agents: int = 1 << 16) -> RodsExample:
+    """cellular_raza-examples/bacterial_rods/src/main.rs:113-209 in SI units: 5 rods of
+    5 vertices, RodMechanics (spring tension 10 /min^2, rigidity 1 um/min^2, damping
+    1.5 /min, spring length 3 um), RodInteraction(Morse R 3 um, lambda 0.5 /um, strength
+    0.1 um^2/min^2, cutoff 5 um), growth 0.1 um/min slowed by neighbours within 2 R
+    ((8 - n) / 8), division at a spring length of 6 um; each rod's spring length
+    U[1.5, 2.5] um, first vertex U[dx, 2 dx] x U[dx/3, 2 dx/3] x U[dx, 2 dx] (dx = 15
+    um), later vertices 3 um apart at angles theta + U[-pi/8, pi/8] in the x-y plane;
+    domain [0, 200] x [0, 15] x [0, 45] um in 8 x 1 x 2 voxels, no gel force; dt 0.1
+    min, 25 h (15000 steps), AB3.  The example seeds ChaCha8; this draws the same
+    distributions from numpy's generator."""
+    from .building_blocks import RodGrowthCycle
+
+    um, minute = 1e-6, 1.0
+    nv, sl0 = 5, 3.0 * um
+    dxr = sl0 * nv
+    rng = np.random.default_rng(seed)
+    spring = rng.uniform(1.5, 2.5, n_rods) * um
+    pos = np.zeros((n_rods, nv, 3))
+    for a in range(n_rods):
+        pos[a, 0] = [rng.uniform(dxr, 2 * dxr), rng.uniform(dxr / 3, 2 * dxr / 3), rng.uniform(dxr, 2 * dxr)]
+        theta = rng.uniform(0.0, 2 * math.pi)
+        for i in range(1, nv):
+            phi = theta + rng.uniform(-math.pi / 8, math.pi / 8)
+            pos[a, i] = pos[a, i - 1] + sl0 * np.array([math.cos(phi), math.sin(phi), 0.0])
+    dom = CartesianCuboid.from_boundaries_and_n_voxels([0.0] * 3, [4 * 50.0 * um, dxr, 3 * dxr], [8, 1, 2])
+    pot = RodInteraction(MorsePotential(radius=3.0 * um, potential_stiffness=0.5 / um, cutoff=5.0 * um,
+                                        strength=0.1 * um ** 2 / minute ** 2))
+    mech = RodMechanics(n_vertices=nv, diffusion_constant=0.0, spring_tension=10.0 / minute ** 2,
+                        rigidity=1.0 * um / minute ** 2, spring_length=sl0, damping=1.5 / minute)
+    cycle = RodGrowthCycle(growth_base_rate=0.1 * um / minute, division_threshold=6.0 * um,
+                           division_radius=3.0 * um, sensing_radius=2.0 * 3.0 * um, neighbor_saturation=8.0,
+                           max_agents=max_agents)
+    wl = Workload("bacterial rods example (Cycle + NeighborSensing)", [InstanceSpec(dom, pot, mech)],
+                  0.1 * minute, steps, 2, pos, np.zeros_like(pos), None, [n_rods])
+    return RodsExample(wl, spring, cycle)
+
+
 def c4_ensemble_2d(n_instances: int = 512, seed: int = 4, steps: int = 1000, n_min: int = 500,
                    n_max: int = 20000) -> Workload:
     """Ensemble of 2-D Morse runs: N_i log-uniform in [n_min, n_max], strength U[0.05, 0.5],

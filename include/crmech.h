@@ -322,9 +322,11 @@ crm_status crm_set_rod_spring_lengths(crm_ctx* ctx, const double* spring_length)
 
 /* Per-rod cycle state by agent index (any pointer may be NULL): spring_length[N],
  * growth_rate[N], identifier[3 N] and parent[3 N] as (kind, a, b) with kind 0
- * Initial(a), 1 Division(a = voxel plain index, b = counter), 3 no parent. */
+ * Initial(a), 1 Division(a = voxel plain index, b = counter), 3 no parent, and
+ * history_held[N], the AuxStorageMechanics entries the rod holds (a daughter
+ * starts empty): its newest entries of crm_get_history's. */
 crm_status crm_get_cycle_state(crm_ctx* ctx, double* spring_length, double* growth_rate, uint64_t* identifier,
-                               uint64_t* parent);
+                               uint64_t* parent, uint32_t* history_held);
 
 /* Neighbour sensing fused into the force pass.  Replaces the NeighborSensing
  * hooks of the chili pair loop (concepts interaction.rs:64-79 trait,

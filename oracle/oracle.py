@@ -179,7 +179,9 @@ class OracleSim:
 
     def step_timed(self, n_steps):
         sec = C.c_double()
-        self._check(self._lib.orc_step_timed(self._h, n_steps, C.byref(sec)))
+        st = self._lib.orc_step_timed(self._h, n_steps, C.byref(sec))
+        self.n_agents = int(self._lib.orc_n_agents(self._h))
+        self._check(st)
         return sec.value
 
     def compute_forces(self):

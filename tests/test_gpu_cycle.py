@@ -135,3 +135,60 @@ def test_cycle_errors():
     with pytest.raises(SimulationError, match="InvalidArgument"):  # points
         p.set_rod_cycle(RodGrowthCycle(0.1, 1.0, 0.25, 0.5))
     p.close()
+
+
+def test_bacterial_rods_example_end_to_end(tmp_path):
+    """The rods example (main.rs:113-209) for its first 6.7 h (4000 steps) through
+    run_simulation with JSON storage, read back the way its plotter reads the cells
+    (plotter.py:22-42); every 20th save point's rod count and identifiers (exact)
+    against the oracle's run, positions within 1e-6 of the domain length through step
+    3000.  The crowded colony is chaotic: rounding differences of 1e-13 m at step 3500
+    reach 1e-9 m by step 4000, so later positions are not comparable while the
+    divisions (counts, identifiers) still are."""
+    import glob
+    import json
+    import os
+
+    from cellular_raza_amd import FixedStepsize, Settings, StorageBuilder, run_simulation
+    from cellular_raza_amd import storage as st
+
+    ex = workloads.bacterial_rods_example()
+    wl = ex.workload
+    spec = wl.instances[0]
+    steps = 4000
+    time = FixedStepsize.from_partial_save_steps(0.0, wl.dt, steps, 25)
+    builder = StorageBuilder().location(tmp_path).add_date(False)
+    access = run_simulation(spec.domain, spec.interaction, spec.mechanics, wl.pos.reshape(5, -1), Settings(
+        time=time, storage=builder), velocities=wl.vel.reshape(5, -1), mechanics_solver_order=2, cycle=ex.cycle,
+        spring_lengths=ex.spring_length, cell_extra={"growth_rate": st.GROWTH_RATE,
+                                                     "spring_length_threshold": 6e-6})
+    o = OracleSim(wl.instances, dt=wl.dt, mechanics_solver_order=2)
+    o.set_agents(wl.pos.reshape(5, -1), wl.vel.reshape(5, -1))
+    o.set_rod_cycle(ex.cycle)
+    o.set_spring_lengths(ex.spring_length)
+    done = 0
+    for it in range(0, steps + 1, 500):
+        if it > done:
+            o.step(it - done)
+            done = it
+        files = glob.glob(os.path.join(tmp_path, "cells", "json", f"{it:020d}", "*"))
+        cells = []
+        for f in files:
+            with open(f) as fh:
+                cells.extend(d["element"][0] for d in json.load(fh)["data"])
+        assert len(cells) == o.n_agents, it
+        st_o = o.cycle_state()
+        want = {st.identifier_json(r).__repr__(): a for a, r in enumerate(st_o["identifier"])}
+        po = o.get_state()[0].reshape(-1, 5, 3)
+        for c in cells:
+            a = want[c["identifier"].__repr__()]
+            assert c["cell"]["spring_length_threshold"] == 6e-6
+            if it > 3000:
+                continue
+            p = np.array(c["cell"]["mechanics"]["pos"][0], dtype=float).reshape(3, -1).T
+            # 1e-6 of the domain's length: FP64 summation-order rounding accumulated
+            # over thousands of steps of a crowded colony
+            assert np.allclose(p, po[a], rtol=0, atol=1e-6 * 200e-6), (it, a, np.abs(p - po[a]).max())
+            assert c["cell"]["growth_rate"] == st_o["growth_rate"][a]
+    assert o.n_agents > 40  # the colony grew (5 -> 44 rods in 6.7 h)
+    assert sorted(access.cells.get_all_iterations()) == list(range(0, steps + 1, 25))
