@@ -620,7 +620,7 @@ __device__ __forceinline__ uint32_t small_div(uint32_t a, float inv_d) {
 // whose region does not fit (local crowding) reads global memory instead; the
 // candidate order and the arithmetic are the same.
 template <int D, int POT, bool MIXED, bool SENSE>
-__global__ __launch_bounds__(kBlock, MIXED ? 8 : 4) void k_force_tiled(Arrays A, StateBuf S,
+__global__ __launch_bounds__(kBlock, MIXED ? 5 : 4) void k_force_tiled(Arrays A, StateBuf S,
                                                                      const ForceTile* __restrict__ tiles,
                                                                      uint32_t physical, uint32_t symmetric,
                                                                      uint32_t no_lds) {
