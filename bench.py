@@ -241,8 +241,8 @@ def workload_table():
         "c2": (lambda a: W.c2_sorting_3d(n=a.n or 1_000_000, seed=2, steps=a.steps), True,
                "3-D two-species cell sorting, Langevin, N=1,000,000, volume fraction 0.3",
                "k_force_tiled<3, 3, true, false>"),
-        "c1": (lambda a: W.c1_morse_2d(n=a.n or 1000, seed=1, steps=a.steps), True,
-               "2-D Morse NewtonDamped, N=1000 in 250x250, Euler", "k_force_tiled<2, 0, true, false>"),
+        "c1": (lambda a: W.c1_morse_2d(n=a.n or 1000, seed=1, steps=a.steps), False,
+               "2-D Morse NewtonDamped, N=1000 in 250x250, Euler", "k_force_tiled<2, 0, false, false>"),
         "c3": (lambda a: W.c3_rods_2d(n=a.n or 200_000, seed=3, steps=a.steps), True,
                "2-D bacterial rods, 200,000 rods x 8 vertices in 2000x2000, AB3", "k_force_rod<2, 0, true>"),
         "c4": (lambda a: W.c4_ensemble_2d(n_instances=a.n or 512, seed=4, steps=a.steps), True,
@@ -291,6 +291,14 @@ def main():
         return
 
     factory, has_mixed, desc, force_kernel = workload_table()[args.workload]
+    clamped = None
+    if args.workload == "c1" and args.warmup + args.steps > 65:
+        # 1000 discs of radius 6 in 250 x 250 overlap 1.8-fold: the overlapping pairs
+        # throw an agent out of the domain (BoundaryError) after 90 to 100 steps -- the
+        # system is chaotic, so the step depends on the rounding (FP64 GPU 90, oracle
+        # 101) -- so the run stays in the first 65 steps, in FP64
+        clamped = f"warmup {args.warmup} + steps {args.steps} clamped to 5 + 60"
+        args.warmup, args.steps = 5, 60
     wl = factory(args)
     N = wl.n_agents
     D = wl.dim
@@ -405,6 +413,8 @@ def main():
     for k, v in extra.items():
         if k not in line:
             line[k] = v
+    if clamped:
+        line["clamped"] = clamped
     if cpu is not None:
         line["gpu_over_cpu"] = value / cpu["value"]
     print(json.dumps(line), flush=True)
