@@ -189,6 +189,8 @@ struct crm_ctx {
     uint32_t* d_ndiv = nullptr;     // [Vt + 1] daughters per voxel, then their exclusive scan
     uint32_t* d_dbefore = nullptr;  // [Vt + 1]
     uint64_t* d_vcount = nullptr;   // [Vt] Voxel::id_counter
+    uint32_t* d_inst_div = nullptr; // [n_inst] dividing rods per instance
+    std::vector<BlockWork> work_host;  // the work table last uploaded
     void* d_scan_tmp = nullptr;
     size_t scan_tmp_bytes = 0;
     // snapshot records (agent order) and species, allocated on the first save
@@ -223,7 +225,7 @@ void free_device(crm_ctx* ctx) {
     void* ptrs[] = {ctx->d_work, ctx->d_tiles, ctx->d_start, ctx->d_count, ctx->d_bsum, ctx->d_key, ctx->d_rank,
                     ctx->d_perm, ctx->d_force, ctx->d_rec64, ctx->d_recq, ctx->d_noise, ctx->d_recrod, ctx->d_bnd,
                     ctx->d_sense, ctx->d_snap, ctx->d_snap_sp, ctx->d_snap_rod, ctx->d_snap_id, ctx->d_snap_st, ctx->d_divflag, ctx->d_drank, ctx->d_dctr,
-                    ctx->d_ndiv, ctx->d_dbefore, ctx->d_vcount, ctx->d_scan_tmp};
+                    ctx->d_ndiv, ctx->d_dbefore, ctx->d_vcount, ctx->d_scan_tmp, ctx->d_inst_div};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& b : ctx->buf) {
@@ -254,6 +256,7 @@ void free_device(crm_ctx* ctx) {
     ctx->d_dctr = nullptr;
     ctx->d_ndiv = ctx->d_dbefore = nullptr;
     ctx->d_vcount = nullptr;
+    ctx->d_inst_div = nullptr;
     ctx->d_scan_tmp = nullptr;
     ctx->scan_tmp_bytes = 0;
     ctx->cap_agents = 0;
@@ -597,6 +600,9 @@ crm_status check_errors(crm_ctx* ctx, const char* phase) {
     } else if (c.error_flags & (1u << CRM_INDEX_ERROR)) {
         kind = CRM_INDEX_ERROR;
         msg = "DecomposeError: could not find voxel index";
+    } else if (c.error_flags & (1u << CRM_DIVISION_ERROR)) {
+        kind = CRM_DIVISION_ERROR;
+        msg = "DivisionError: Could not calculate new segment lengths";
     } else {
         kind = CRM_DEVICE_ERROR;
         msg = "unknown device error";
@@ -769,9 +775,11 @@ crm_status alloc_scan_tmp(crm_ctx* ctx) {
 }
 
 // Work blocks (rods_per_block rods, never across instances) for the current
-// per-instance counts; offsets and counts go to the instance parameters.
+// per-instance counts; offsets and counts go to the instance parameters.  The
+// uploads are queued on the stream (the host copies live in the context).
 crm_status rebuild_work(crm_ctx* ctx) {
-    std::vector<BlockWork> work;
+    std::vector<BlockWork>& work = ctx->work_host;
+    work.clear();
     uint64_t first = 0;
     const uint64_t per = is_rod(ctx) ? ctx->rods_per_block : (uint64_t)kBlock;
     for (uint32_t i = 0; i < ctx->n_inst; i++) {
@@ -791,16 +799,16 @@ crm_status rebuild_work(crm_ctx* ctx) {
                            ctx->stream));
     CRM_HIP(hipMemcpyAsync(ctx->d_ip, ctx->hparams.data(), sizeof(InstParams) * ctx->n_inst, hipMemcpyHostToDevice,
                            ctx->stream));
-    CRM_HIP(hipStreamSynchronize(ctx->stream));
     return CRM_OK;
 }
 
-// One step with the rods example's cycle (kernels_cycle.hpp).  Synchronous: the
-// host reads the number of divisions to size the grown order.
+// One step with the rods example's cycle (kernels_cycle.hpp).  The host reads the
+// control block once per step (errors so far, divisions per instance) to size the
+// grown order; everything else stays queued on the stream.
 crm_status launch_cycle_step(crm_ctx* ctx, const double* noise_step) {
     const StepArgs sa = step_args(ctx, noise_step);
     const uint32_t D = ctx->D, Vt = (uint32_t)ctx->Vt;
-    hipLaunchKernelGGL(k_cycle_begin, dim3(1), dim3(1), 0, ctx->stream, ctx->d_ctrl);
+    hipLaunchKernelGGL(k_cycle_begin, dim3(1), dim3(64), 0, ctx->stream, ctx->d_ctrl, ctx->d_inst_div, ctx->n_inst);
     CRM_TRY(launch_force(ctx));
     CycleArgs Y;
     Y.base = ctx->cyc.growth_base_rate;
@@ -812,6 +820,7 @@ crm_status launch_cycle_step(crm_ctx* ctx, const double* noise_step) {
     Y.ndiv = ctx->d_ndiv;
     Y.dbefore = ctx->d_dbefore;
     Y.vcount = ctx->d_vcount;
+    Y.inst_div = ctx->d_inst_div;
     Y.n_inst = ctx->n_inst;
     const RodArgs R = rod_args(ctx);
     StateBuf B = ctx->buf[ctx->cur], B2 = ctx->buf[1 - ctx->cur];
@@ -823,7 +832,11 @@ crm_status launch_cycle_step(crm_ctx* ctx, const double* noise_step) {
         CRM_HIP(hipGetLastError());
     }
     Control c;
-    CRM_TRY(read_control(ctx, c));
+    std::vector<uint32_t> inst_div(ctx->n_inst);
+    CRM_HIP(hipMemcpyAsync(&c, ctx->d_ctrl, sizeof(Control), hipMemcpyDeviceToHost, ctx->stream));
+    CRM_HIP(hipMemcpyAsync(inst_div.data(), ctx->d_inst_div, ctx->n_inst * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    CRM_HIP(hipStreamSynchronize(ctx->stream));
     if (c.error_flags) return check_errors(ctx, "step");
     const uint64_t n_old = ctx->N, n_new = n_old + c.n_div;
     if (c.n_div == 0) {
@@ -850,17 +863,9 @@ crm_status launch_cycle_step(crm_ctx* ctx, const double* noise_step) {
             hipLaunchKernelGGL(k_expand_rod<3>, g, b, 0, ctx->stream, A, B2, B, R, sa, Y, n_old, n_new,
                                ctx->cyc.division_radius);
         CRM_HIP(hipGetLastError());
-        // daughters per instance from the scan at the instances' voxel boundaries
-        std::vector<uint32_t> at(ctx->n_inst + 1);
-        for (uint32_t i = 0; i <= ctx->n_inst; i++) {
-            const uint32_t v = i < ctx->n_inst ? ctx->hparams[i].voxel_offset : Vt;
-            CRM_HIP(hipMemcpyAsync(&at[i], ctx->d_dbefore + v, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
-        }
-        CRM_HIP(hipStreamSynchronize(ctx->stream));
-        for (uint32_t i = 0; i < ctx->n_inst; i++) ctx->counts[i] += at[i + 1] - at[i];
+        for (uint32_t i = 0; i < ctx->n_inst; i++) ctx->counts[i] += inst_div[i];
         ctx->N = n_new;
         CRM_TRY(rebuild_work(ctx));
-        CRM_TRY(check_errors(ctx, "division"));
     }
     {
         Arrays A = arrays(ctx);
@@ -1286,6 +1291,7 @@ crm_status crm_set_agents(crm_ctx* ctx, const uint64_t* counts, const double* po
         CRM_TRY(dalloc(ctx, ctx->d_ndiv, Vt + 1));
         CRM_TRY(dalloc(ctx, ctx->d_dbefore, Vt + 1));
         CRM_TRY(dalloc(ctx, ctx->d_vcount, Vt));
+        CRM_TRY(dalloc(ctx, ctx->d_inst_div, ctx->n_inst));
         CRM_TRY(alloc_scan_tmp(ctx));
     }
     CRM_HIP(hipMemcpy(ctx->d_work, work.data(), sizeof(BlockWork) * work.size(), hipMemcpyHostToDevice));
@@ -1375,14 +1381,14 @@ crm_status crm_step(crm_ctx* ctx, uint32_t n_steps, const double* noise) {
     }
     if (ctx->cycle_on) {  // the grown order is sized on the host every step
         for (uint32_t k = 0; k < n_steps; k++) {
+            // errors of a step surface at the next step's control read or below
             crm_status st = launch_cycle_step(ctx, use_noise_buf ? ctx->d_noise + per_step * k : nullptr);
-            if (st == CRM_OK) st = check_errors(ctx, "step");
             if (st) return st;
             ctx->step += 1;
             ctx->sense_parity = -1;  // counts are by the pre-step order
         }
         CRM_HIP(hipStreamSynchronize(ctx->stream));
-        return CRM_OK;
+        return check_errors(ctx, "step");
     }
     const bool graphs = (ctx->cfg.flags & CRM_FLAG_GRAPH) && !timing_on(ctx) && !use_noise_buf;
     uint32_t s = 0;

@@ -46,10 +46,14 @@ struct CycleArgs {
     uint32_t* ndiv;     // [Vt + 1]
     uint32_t* dbefore;  // [Vt + 1]
     uint64_t* vcount;   // [Vt]
+    uint32_t* inst_div; // [n_inst] dividing rods per instance this step
     uint32_t n_inst;
 };
 
-__global__ void k_cycle_begin(Control* c) { c->n_div = 0; }
+__global__ void k_cycle_begin(Control* c, uint32_t* inst_div, uint32_t n_inst) {
+    c->n_div = 0;
+    for (uint32_t i = threadIdx.x; i < n_inst; i += blockDim.x) inst_div[i] = 0;
+}
 
 __global__ void k_advance_step(Control* c) { c->step += 1; }
 
@@ -116,7 +120,10 @@ __global__ __launch_bounds__(kBlock) void k_rod_mech(Arrays A, StateBuf src, Sta
     dst.spring[s] = sl;
     const bool divide = sl > Y.threshold;
     Y.divflag[s] = divide ? 1u : 0u;
-    if (divide) atomicAdd(&A.ctrl->n_div, 1u);
+    if (divide) {
+        atomicAdd(&A.ctrl->n_div, 1u);
+        atomicAdd(&Y.inst_div[w.inst], 1u);
+    }
 }
 
 // update_cell_cycle_4's bookkeeping per voxel: mothers in list order get
