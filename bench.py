@@ -73,6 +73,37 @@ def force_bytes_per_agent(cbar, record_bytes, dim, stencil, force_bytes):
     return record_bytes + 8 * stencil + cbar * record_bytes + force_bytes
 
 
+def time_rods_colony(steps=100):
+    """The cycle pipeline at scale: 200,000 2-D rods of 8 vertices (the rods workload)
+    with per-rod spring lengths U[0.5, 0.6], growth 0.5 per unit time, division at
+    0.62 with radius 0.5 and sensing within 1.0: rod-steps/s over `steps` steps while
+    the colony divides, beside the plain step of the same rods (FP64 both)."""
+    from cellular_raza_amd import MechanicsBackend, RodGrowthCycle, workloads
+
+    wl = workloads.c3_rods_2d(n=200_000, seed=3)
+    n = wl.n_agents
+    rng = np.random.default_rng(7)
+    spring = rng.uniform(0.5, 0.6, n)
+    cyc = RodGrowthCycle(growth_base_rate=0.5, division_threshold=0.62, division_radius=0.5, sensing_radius=1.0,
+                         max_agents=3 * n)
+    out = {}
+    for mode in ("plain", "cycle"):
+        g = MechanicsBackend(wl.instances, dt=wl.dt, mechanics_solver_order=2)
+        if mode == "cycle":
+            g.set_rod_cycle(cyc)
+        g.set_agents(wl.pos.reshape(n, -1), wl.vel.reshape(n, -1))
+        g.set_rod_spring_lengths(spring)
+        g.step(5)
+        n0 = g.n_agents
+        t0 = time.perf_counter()
+        g.step(steps)
+        dt = time.perf_counter() - t0
+        out[mode] = {"rods_start": n0, "rods_end": g.n_agents, "s": dt,
+                     "rod_steps_per_s": steps * 0.5 * (n0 + g.n_agents) / dt}
+        g.close()
+    return {"steps": steps, **out}
+
+
 def time_rods_example():
     """The bacterial rods example end to end (5 rods growing and dividing for 25 h of
     simulated time, dt 0.1 min): GPU wall clock for all 15000 steps, and for the
@@ -267,6 +298,9 @@ def main():
     ap.add_argument("--rods-example", action="store_true",
                     help="also time the bacterial rods example (Cycle + NeighborSensing, 25 h = 15000 steps) on the "
                          "GPU, and its first 5000 steps on the GPU and on the oracle")
+    ap.add_argument("--rods-colony", action="store_true",
+                    help="also time the rods cycle at scale: the 200,000-rod 2-D workload growing and dividing "
+                         "(the rods example's cycle model), against the same rods without the cycle")
     ap.add_argument("--snapshot", action="store_true",
                     help="also time one save point: the device gather into agent order with its copy to the host, "
                          "and the cells written in the reference's JSON layout (to a temporary folder)")
@@ -333,6 +367,9 @@ def main():
         extra["batched_over_sequential"] = value / seq_value
         extra["sequential_sample"] = f"{len(wl.instances)} contexts one after another, {args.seq_steps} steps each"
         print(f"bench: sequential {seq_value:.4e} agent-steps/s", file=sys.stderr, flush=True)
+    if args.rods_colony:
+        extra["rods_colony"] = time_rods_colony()
+        print(f"bench: rods colony {extra['rods_colony']}", file=sys.stderr, flush=True)
     if args.rods_example:
         extra["rods_example"] = time_rods_example()
         print(f"bench: rods example {extra['rods_example']}", file=sys.stderr, flush=True)
