@@ -1733,7 +1733,10 @@ crm_status crm_set_neighbor_sensing(crm_ctx* ctx, uint32_t enable, double radius
 crm_status crm_get_neighbor_counts(crm_ctx* ctx, uint32_t* counts) {
     if (!ctx || !counts || !ctx->have_agents) return CRM_INVALID_ARGUMENT;
     if (!ctx->sense_on || ctx->sense_parity < 0) {
-        ctx->err = "no force evaluation with neighbour sensing since it was enabled or the agents were set";
+        ctx->err = ctx->cycle_on && ctx->step > 0
+                       ? "with the rods cycle the counts are consumed by the step (the growth rates of "
+                         "crm_get_cycle_state); crm_compute_forces evaluates them for the current state"
+                       : "no force evaluation with neighbour sensing since it was enabled or the agents were set";
         return CRM_INVALID_ARGUMENT;
     }
     CRM_HIP(hipSetDevice(ctx->device));
