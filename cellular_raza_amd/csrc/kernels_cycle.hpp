@@ -69,17 +69,27 @@ __global__ __launch_bounds__(kBlock) void k_init_vcount(const uint32_t* start, u
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_rod_mech(Arrays A, StateBuf src, StateBuf dst, RodArgs R, StepArgs sa,
                                                       CycleArgs Y) {
+    __shared__ double win[kRodWinK];
     const BlockWork w = A.work[blockIdx.x];
     uint64_t s;
     uint32_t v;
-    if (!rod_lane(w, R.nv, s, v)) return;
+    const bool active = rod_lane(w, R.nv, s, v);
     const InstParams& P = A.ip[w.inst];
     const uint64_t N = A.N, t = A.ctrl->step;
     const uint32_t nv = R.nv, K = nv * D;
-    const uint32_t id = src.id[s];
-    const Ring rg = ring_rod(t, src.birth[s], sa.cap, sa.newest_first);
     auto pos = [&](int i, int k) { return src.pos[(uint64_t)(i * D + k) * N + s]; };
     auto vel = [&](int i, int k) { return src.vel[(uint64_t)(i * D + k) * N + s]; };
+    if (active && v + 2 < nv) {  // bending window v of the rod, shared with the next two lanes
+        double F[D];
+        rod_bend_window<D>(P, pos, (int)v, F);
+#pragma unroll
+        for (int k = 0; k < D; k++) win[threadIdx.x * 3 + k] = F[k];
+    }
+    __syncthreads();
+    if (!active) return;
+    const uint32_t id = src.id[s];
+    const Ring rg = ring_rod(t, src.birth[s], sa.cap, sa.newest_first);
+    auto wn = [&](int i, int k) { return win[(threadIdx.x - v + (uint32_t)i) * 3 + k]; };
     double x[D], vv[D], f[D], dx[D], dv[D], dxr[D], nx[D], nvel[D];
 #pragma unroll
     for (int k = 0; k < D; k++) {
@@ -87,7 +97,7 @@ __global__ __launch_bounds__(kBlock) void k_rod_mech(Arrays A, StateBuf src, Sta
         vv[k] = vel((int)v, k);
         f[k] = A.force[(uint64_t)(v * D + k) * N + s];
     }
-    rod_row_increment<D>(P, src.spring[s], nv, v, pos, vel, f, dx, dv);
+    rod_row_increment<D>(P, src.spring[s], nv, v, pos, vel, wn, f, dx, dv);
     rod_row_noise<D>(P, sa, id, t, nv, v, dxr);
     auto hist = [&](int i, int k, bool is_dv) {
         const double* h = is_dv ? src.hdv : src.hdx;

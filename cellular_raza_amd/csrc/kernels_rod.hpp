@@ -428,12 +428,58 @@ __device__ __forceinline__ double rod_normal(const StepArgs& sa, uint32_t id, ui
     return (k & 1) ? z1 : z0;
 }
 
+// Bending window i of a rod (vertices i, i + 1, i + 2; bacterial_rods.rs:183-205):
+// the force F on the middle vertex; the outer two get -F / 2 each.  pos(i, k)
+// reads vertex i.  One lane per window computes it and shares it through LDS.
+template <int D, typename Pos>
+__device__ __forceinline__ void rod_bend_window(const InstParams& P, Pos pos, int i, double* F) {
+    double c1[D], c2[D];
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+        c1[k] = pos(i, k) - pos(i + 1, k);
+        c2[k] = pos(i + 1, k) - pos(i + 2, k);
+    }
+    double prod = 0.0, n1 = 0.0, n2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+        prod = prod + c1[k] * c2[k];
+        n1 = n1 + c1[k] * c1[k];
+        n2 = n2 + c2[k] * c2[k];
+    }
+    n1 = sqrt(n1);
+    n2 = sqrt(n2);
+    double ang = 0.0;
+    if (!(n1 == 0.0 || n2 == 0.0)) {  // nalgebra Matrix::angle
+        double cang = prod / (n1 * n2);
+        cang = cang < -1.0 ? -1.0 : (cang > 1.0 ? 1.0 : cang);
+        ang = acos(cang);
+    }
+    double fd[D], nf = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+        fd[k] = c1[k] - c2[k];
+        nf = nf + fd[k] * fd[k];
+    }
+    nf = sqrt(nf);
+    const double tn = tan(0.5 * ang);
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+        const double fdir = nf != 0.0 ? fd[k] / nf : fd[k];
+        F[k] = ((fdir * 2.0) * P.rigidity) * tn;
+    }
+}
+
+// LDS of the bending windows of a workgroup's rods: lane t holds window v of its
+// rod (t = lane of vertex v), read by the lanes of vertices v, v + 1, v + 2.
+constexpr int kRodWinK = kBlock * 3;
+
 // RodMechanics::calculate_increment row v (bacterial_rods.rs:157-213): springs,
 // bending and damping, accumulated in the reference's order.  pos(i, k) and
-// vel(i, k) read vertex i of the rod.
-template <int D, typename Pos, typename Vel>
+// vel(i, k) read vertex i of the rod; win(i, k) component k of bending window i.
+template <int D, typename Pos, typename Vel, typename Win>
 __device__ __forceinline__ void rod_row_increment(const InstParams& P, double spring_length, uint32_t nv, uint32_t v,
-                                                  Pos pos, Vel vel, const double* force, double* dx, double* dv) {
+                                                  Pos pos, Vel vel, Win win, const double* force, double* dx,
+                                                  double* dv) {
     double total[D];
 #pragma unroll
     for (int k = 0; k < D; k++) total[k] = force[k];
@@ -469,46 +515,14 @@ __device__ __forceinline__ void rod_row_increment(const InstParams& P, double sp
         }
     }
     // bending windows i = v - 2, v - 1, v (v is vertex i + 2, i + 1, i)
-    auto bend = [&](int i, double* F) {
-        double c1[D], c2[D];
-        dist(i, c1);
-        dist(i + 1, c2);
-        double prod = 0.0, n1 = 0.0, n2 = 0.0;
-#pragma unroll
-        for (int k = 0; k < D; k++) {
-            prod = prod + c1[k] * c2[k];
-            n1 = n1 + c1[k] * c1[k];
-            n2 = n2 + c2[k] * c2[k];
-        }
-        n1 = sqrt(n1);
-        n2 = sqrt(n2);
-        double ang = 0.0;
-        if (!(n1 == 0.0 || n2 == 0.0)) {  // nalgebra Matrix::angle
-            double cang = prod / (n1 * n2);
-            cang = cang < -1.0 ? -1.0 : (cang > 1.0 ? 1.0 : cang);
-            ang = acos(cang);
-        }
-        double fd[D], nf = 0.0;
-#pragma unroll
-        for (int k = 0; k < D; k++) {
-            fd[k] = c1[k] - c2[k];
-            nf = nf + fd[k] * fd[k];
-        }
-        nf = sqrt(nf);
-        const double tn = tan(0.5 * ang);
-#pragma unroll
-        for (int k = 0; k < D; k++) {
-            const double fdir = nf != 0.0 ? fd[k] / nf : fd[k];
-            F[k] = ((fdir * 2.0) * P.rigidity) * tn;
-        }
-    };
     for (int i = (int)v - 2; i <= (int)v; i++) {
         if (i < 0 || i + 2 >= (int)nv) continue;
-        double F[D];
-        bend(i, F);
         const int role = (int)v - i;  // 0: v = i, 1: v = i + 1, 2: v = i + 2
 #pragma unroll
-        for (int k = 0; k < D; k++) total[k] = total[k] + (role == 1 ? F[k] : (-F[k]) * 0.5);
+        for (int k = 0; k < D; k++) {
+            const double F = win(i, k);
+            total[k] = total[k] + (role == 1 ? F : (-F) * 0.5);
+        }
     }
 #pragma unroll
     for (int k = 0; k < D; k++) {
@@ -682,6 +696,7 @@ template <int D, bool INTEGRATE>
 __global__ __launch_bounds__(kBlock) void k_gather_rod(Arrays A, StateBuf src, StateBuf dst, RodArgs R,
                                                          StepArgs sa) {
     __shared__ double newpos[kRodLdsK];
+    __shared__ double win[INTEGRATE ? kRodWinK : 1];
     const BlockWork w = A.work[blockIdx.x];
     uint64_t s;
     uint32_t v;
@@ -690,6 +705,17 @@ __global__ __launch_bounds__(kBlock) void k_gather_rod(Arrays A, StateBuf src, S
     const uint64_t N = A.N;
     const uint32_t nv = R.nv, K = nv * D;
     double nx[D], nvel[D];
+    if constexpr (INTEGRATE) {  // bending window v of the rod, shared with the next two lanes
+        if (active && v + 2 < nv) {
+            const uint32_t o = A.perm[s];
+            auto pos = [&](int i, int k) { return src.pos[(uint64_t)(i * D + k) * N + o]; };
+            double F[D];
+            rod_bend_window<D>(P, pos, (int)v, F);
+#pragma unroll
+            for (int k = 0; k < D; k++) win[threadIdx.x * 3 + k] = F[k];
+        }
+        __syncthreads();
+    }
     if (active) {
         const uint32_t o = A.perm[s];
         const uint32_t id = src.id[o];
@@ -705,7 +731,8 @@ __global__ __launch_bounds__(kBlock) void k_gather_rod(Arrays A, StateBuf src, S
                 vv[k] = vel((int)v, k);
                 f[k] = A.force[(uint64_t)(v * D + k) * N + o];
             }
-            rod_row_increment<D>(P, src.spring[o], nv, v, pos, vel, f, dx, dv);
+            auto wn = [&](int i, int k) { return win[(threadIdx.x - v + (uint32_t)i) * 3 + k]; };
+            rod_row_increment<D>(P, src.spring[o], nv, v, pos, vel, wn, f, dx, dv);
             rod_row_noise<D>(P, sa, id, t, nv, v, dxr);
             auto hist = [&](int i, int k, bool is_dv) {
                 const double* h = is_dv ? src.hdv : src.hdx;
