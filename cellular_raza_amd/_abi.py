@@ -177,6 +177,7 @@ P = C.c_void_p
 DP = C.POINTER(C.c_double)
 U32P = C.POINTER(C.c_uint32)
 U64P = C.POINTER(C.c_uint64)
+U8P = C.POINTER(C.c_uint8)
 I32P = C.POINTER(C.c_int32)
 
 SIGNATURES = {
@@ -199,6 +200,7 @@ SIGNATURES = {
     "crm_set_rod_cycle": (C.c_int, [P, C.POINTER(RodCycle)]),
     "crm_set_rod_spring_lengths": (C.c_int, [P, DP]),
     "crm_get_cycle_state": (C.c_int, [P, DP, DP, U64P, U64P, U32P]),
+    "crm_remove_agents": (C.c_int, [P, U8P]),
     "crm_save_cells_json": (C.c_int, [P, C.c_char_p, C.c_uint64, C.POINTER(JsonCells)]),
     "crm_get_timing": (C.c_int, [P, C.POINTER(Timing)]),
     "crm_reset_timing": (C.c_int, [P]),

@@ -23,6 +23,7 @@
 #include "kernels_rod.hpp"
 #include "kernels_snapshot.hpp"
 #include "kernels_cycle.hpp"
+#include "kernels_remove.hpp"
 
 #include <hipcub/hipcub.hpp>
 
@@ -687,6 +688,20 @@ crm_status dalloc(crm_ctx* ctx, T*& p, size_t n) {
     CRM_HIP(hipMalloc((void**)&p, std::max<size_t>(n, 1) * sizeof(T)));
     return CRM_OK;
 }
+
+// Device buffers of one call, freed on every return path.
+struct CallScratch {
+    std::vector<void*> ptrs;
+    ~CallScratch() {
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+    template <typename T>
+    hipError_t get(T*& p, size_t n) {
+        const hipError_t e = hipMalloc((void**)&p, std::max<size_t>(n, 1) * sizeof(T));
+        if (e == hipSuccess) ptrs.push_back(p);
+        return e;
+    }
+};
 
 // Force tiles of every instance under tiling policy `pol`.
 void make_tiles(crm_ctx* ctx, uint32_t pol, std::vector<ForceTile>& tiles) {
@@ -1699,6 +1714,87 @@ crm_status crm_get_cycle_state(crm_ctx* ctx, double* spring_length, double* grow
         }
     }
     return CRM_OK;
+}
+
+crm_status crm_remove_agents(crm_ctx* ctx, const uint8_t* remove) {
+    if (!ctx || !remove) return CRM_INVALID_ARGUMENT;
+    if (!ctx->have_agents) {
+        ctx->err = "crm_set_agents must be called first";
+        return CRM_INVALID_ARGUMENT;
+    }
+    if (ctx->halted) return CRM_INVALID_ARGUMENT;
+    const uint64_t n_old = ctx->N;
+    uint64_t n_rm = 0;
+    for (uint64_t a = 0; a < n_old; a++) n_rm += remove[a] ? 1u : 0u;
+    if (n_rm == 0) return CRM_OK;
+    if (n_rm == n_old) {
+        ctx->err = "the removal would leave no agents";
+        return CRM_INVALID_ARGUMENT;
+    }
+    const uint64_t n_new = n_old - n_rm;
+    CRM_HIP(hipSetDevice(ctx->device));
+    CallScratch tmp;
+    RemoveArgs X{};
+    uint8_t* flag = nullptr;
+    CRM_HIP(tmp.get(flag, n_old));
+    CRM_HIP(tmp.get(X.keep_slot, n_old + 1));
+    CRM_HIP(tmp.get(X.keep_id, n_old + 1));
+    CRM_HIP(tmp.get(X.slot_before, n_old + 1));
+    CRM_HIP(tmp.get(X.id_before, n_old + 1));
+    CRM_HIP(tmp.get(X.inst_rm, ctx->n_inst));
+    X.flag = flag;
+    X.n_old = n_old;
+    X.n_inst = ctx->n_inst;
+    size_t bytes = 0;
+    CRM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, X.keep_slot, X.slot_before, (int)(n_old + 1),
+                                             ctx->stream));
+    uint8_t* scan_tmp = nullptr;
+    CRM_HIP(tmp.get(scan_tmp, bytes));
+    CRM_HIP(hipMemcpyAsync(flag, remove, n_old, hipMemcpyHostToDevice, ctx->stream));
+    CRM_HIP(hipMemsetAsync(X.inst_rm, 0, sizeof(uint32_t) * ctx->n_inst, ctx->stream));
+    const StateBuf src = ctx->buf[ctx->cur], dst = ctx->buf[1 - ctx->cur];
+    hipLaunchKernelGGL(k_remove_mark, dim3((uint32_t)((n_old + 1 + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       ctx->stream, arrays(ctx), src, X);
+    CRM_HIP(hipGetLastError());
+    CRM_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, bytes, X.keep_slot, X.slot_before, (int)(n_old + 1),
+                                             ctx->stream));
+    CRM_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, bytes, X.keep_id, X.id_before, (int)(n_old + 1),
+                                             ctx->stream));
+    hipLaunchKernelGGL(k_remove_compact, dim3((uint32_t)((n_old + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       ctx->stream, src, dst, X, n_new, ctx->K, ctx->cfg.solver_order, (uint32_t)(is_rod(ctx) ? 1 : 0));
+    CRM_HIP(hipGetLastError());
+    std::vector<uint32_t> inst_rm(ctx->n_inst);
+    CRM_HIP(hipMemcpyAsync(inst_rm.data(), X.inst_rm, sizeof(uint32_t) * ctx->n_inst, hipMemcpyDeviceToHost,
+                           ctx->stream));
+    CRM_HIP(hipStreamSynchronize(ctx->stream));
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < ctx->n_inst; i++) {
+        if (inst_rm[i] > ctx->counts[i]) total = UINT64_MAX;
+        if (total != UINT64_MAX) total += inst_rm[i];
+    }
+    if (total != n_rm) {
+        ctx->err = "agent removal: the removed agents do not match the instances' voxels";
+        ctx->halted = true;
+        return CRM_DEVICE_ERROR;
+    }
+    for (uint32_t i = 0; i < ctx->n_inst; i++) ctx->counts[i] -= inst_rm[i];
+    ctx->N = n_new;
+    ctx->cur = 1 - ctx->cur;
+    CRM_TRY(rebuild_work(ctx));
+    ctx->sense_parity = -1;
+    for (auto& g : ctx->graph)  // captured for the old agent count
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+    // the voxel lists of the survivors: a set-up sort with every agent in its voxel
+    CRM_HIP(hipMemsetAsync(ctx->d_count, 0, sizeof(uint32_t) * 2 * ctx->Vt, ctx->stream));
+    hipLaunchKernelGGL(k_remove_key, dim3((uint32_t)((n_new + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream,
+                       arrays(ctx), ctx->buf[ctx->cur]);
+    CRM_HIP(hipGetLastError());
+    CRM_TRY(launch_sort(ctx, false, nullptr));
+    CRM_HIP(hipStreamSynchronize(ctx->stream));
+    return check_errors(ctx, "agent removal");
 }
 
 crm_status crm_set_neighbor_sensing(crm_ctx* ctx, uint32_t enable, double radius) {

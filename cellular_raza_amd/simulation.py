@@ -202,6 +202,18 @@ class MechanicsBackend:
         return {"spring_length": sl, "growth_rate": gr, "identifier": ident, "parent": parent,
                 "history_held": held}
 
+    def remove_agents(self, remove) -> None:
+        """CycleEvent::Remove for the agents with remove[a] true (a boolean per
+        agent index; crm_remove_agents): they leave their voxels, the others keep
+        their order, state, history and identifiers, and agent indices are
+        compacted in order (update_cycle.rs:41-55)."""
+        rm = np.ascontiguousarray(remove, dtype=np.uint8)
+        if rm.shape != (self.n_agents,):
+            raise ValueError("one flag per agent")
+        st = self._lib.crm_remove_agents(self._h, rm.ctypes.data_as(_abi.U8P))
+        self._check(st)
+        self.n_agents -= int(np.count_nonzero(rm))
+
     def compute_forces(self) -> np.ndarray:
         out = np.empty((self.n_agents, self.K), dtype=np.float64)
         self._check(self._lib.crm_compute_forces(self._h, _dp(out)))

@@ -328,6 +328,20 @@ crm_status crm_set_rod_spring_lengths(crm_ctx* ctx, const double* spring_length)
 crm_status crm_get_cycle_state(crm_ctx* ctx, double* spring_length, double* growth_rate, uint64_t* identifier,
                                uint64_t* parent, uint32_t* history_held);
 
+/* Removes the agents with remove[a] != 0 (remove[n_agents], by agent index):
+ * CycleEvent::Remove as update_cell_cycle_4 applies it (chili/update_cycle.rs:
+ * 41-55: the flagged cells leave their voxel's list, Vec::retain keeps the order of
+ * the others, and neither identifiers nor the voxels' id counters change).  Call
+ * it between steps for the agents whose cycle returned Remove in the step just
+ * taken (or whose update_conditional_phased_death returned true, update_cycle.rs:
+ * 113-119): the reference removes them after that step's mechanics update and
+ * before its sort, which leaves the same voxel lists.  The survivors keep their
+ * state, history and cycle state; agent indices are compacted in order (agent a
+ * becomes the number of survivors before a), which also renumbers the Philox noise
+ * streams and the rows of every by-agent array.  Removing every agent is
+ * CRM_INVALID_ARGUMENT.  Points and rods, with or without the rods cycle. */
+crm_status crm_remove_agents(crm_ctx* ctx, const uint8_t* remove);
+
 /* Neighbour sensing fused into the force pass.  Replaces the NeighborSensing
  * hooks of the chili pair loop (concepts interaction.rs:64-79 trait,
  * update_mechanics.rs:125-139 neighbor_sensing_single, called per pair at

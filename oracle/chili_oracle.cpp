@@ -597,6 +597,7 @@ struct SimBase {
     virtual int set_rod_cycle(const crm_rod_cycle& y) = 0;
     virtual int set_spring_lengths(const double* sl) = 0;
     virtual void cycle_out(double* spring_length, double* growth_rate, uint64_t* ident, uint64_t* parent) = 0;
+    virtual int remove_agents(const uint8_t* flags) = 0;
     bool cycle_on = false;
     crm_rod_cycle cyc{};
 };
@@ -1192,6 +1193,35 @@ struct Sim : SimBase {
         return CRM_OK;
     }
 
+    // CycleEvent::Remove as update_cell_cycle_4 applies it (update_cycle.rs:41-55):
+    // the flagged cells leave their voxel's Vec, retain keeps the order of the
+    // others; identifiers and id counters stay.  Agent indices are compacted in
+    // order (agent a becomes the number of survivors before a).
+    int remove_agents(const uint8_t* flags) override {
+        std::vector<uint64_t> before(n_agents);
+        uint64_t kept = 0;
+        for (uint64_t a = 0; a < n_agents; a++) {
+            before[a] = kept;
+            kept += flags[a] ? 0 : 1;
+        }
+        if (kept == n_agents) return CRM_OK;
+        if (kept == 0) {
+            err = "the removal would leave no agents";
+            return CRM_INVALID_ARGUMENT;
+        }
+        for (auto& I : inst)
+            for (auto& cells : I.voxels) {
+                std::vector<Cell<KM>> keep;
+                keep.reserve(cells.size());
+                for (const Cell<KM>& c : cells)
+                    if (!flags[c.id]) keep.push_back(c);
+                for (Cell<KM>& c : keep) c.id = before[c.id];
+                cells.swap(keep);
+            }
+        n_agents = kept;
+        return CRM_OK;
+    }
+
     void cycle_out(double* spring_length, double* growth_rate, uint64_t* ident, uint64_t* parent) override {
         for_each_cell([&](Cell<KM>& c) {
             if (spring_length) spring_length[c.id] = c.spring_length;
@@ -1584,6 +1614,11 @@ int orc_set_rod_cycle(void* h, const crm_rod_cycle* y) {
 int orc_set_spring_lengths(void* h, const double* sl) {
     if (!h || !sl) return CRM_INVALID_ARGUMENT;
     return static_cast<SimBase*>(h)->set_spring_lengths(sl);
+}
+
+int orc_remove_agents(void* h, const uint8_t* flags) {
+    if (!h || !flags) return CRM_INVALID_ARGUMENT;
+    return static_cast<SimBase*>(h)->remove_agents(flags);
 }
 
 int orc_cycle_out(void* h, double* spring_length, double* growth_rate, uint64_t* ident, uint64_t* parent) {

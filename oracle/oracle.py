@@ -59,6 +59,7 @@ def load():
         "orc_set_rod_cycle": (C.c_int, [P, C.POINTER(_abi.RodCycle)]),
         "orc_set_spring_lengths": (C.c_int, [P, _abi.DP]),
         "orc_cycle_out": (C.c_int, [P, _abi.DP, _abi.DP, _abi.U64P, _abi.U64P]),
+        "orc_remove_agents": (C.c_int, [P, _abi.U8P]),
         "orc_n_voxels_total": (C.c_uint64, [P]),
         "orc_n_agents": (C.c_uint64, [P]),
         "orc_last_error": (C.c_size_t, [P, C.c_char_p, C.c_size_t]),
@@ -168,6 +169,11 @@ class OracleSim:
     def set_spring_lengths(self, spring_length):
         sl = np.ascontiguousarray(spring_length, dtype=np.float64)
         self._check(self._lib.orc_set_spring_lengths(self._h, _dp(sl)))
+
+    def remove_agents(self, remove):
+        rm = np.ascontiguousarray(remove, dtype=np.uint8)
+        self._check(self._lib.orc_remove_agents(self._h, rm.ctypes.data_as(_abi.U8P)))
+        self.n_agents = int(self._lib.orc_n_agents(self._h))
 
     def cycle_state(self):
         n = self.n_agents

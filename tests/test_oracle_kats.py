@@ -628,3 +628,83 @@ def test_rod_growth_reacts_to_neighbours():
     assert np.array_equal(st["growth_rate"], want)
     assert np.array_equal(st["spring_length"], 0.5 + want * dt)
     o.close()
+
+
+# ---------------------------------------------------------------------------
+# CycleEvent::Remove (update_cycle.rs:41-55)
+# ---------------------------------------------------------------------------
+def _sorting_sim(n, seed, order=2):
+    rng = np.random.default_rng(seed)
+    L = 30.0
+    dom = CartesianCuboid.from_boundaries_and_interaction_range([0.0] * 3, [L] * 3, 3.0)
+    pot = MorsePotential(radius=1.0, potential_stiffness=1.5, cutoff=3.0, strength=0.5)
+    spec = [InstanceSpec(dom, pot, NewtonDamped(damping_constant=1.0, mass=1.0))]
+    pos = rng.uniform(1e-9, L, size=(n, 3))
+    o = O.OracleSim(spec, dt=0.01, mechanics_solver_order=order)
+    o.set_agents(pos, rng.normal(scale=0.1, size=(n, 3)))
+    return o, spec, rng
+
+
+def test_remove_keeps_voxel_order_state_and_history():
+    """The removed agents leave their voxel's list and the others keep their order
+    (Vec::retain), state and history; agent indices are compacted in order.  The
+    expected lists are the old ones filtered and renumbered in numpy."""
+    o, _, rng = _sorting_sim(600, 21)
+    o.step(7)
+    vox0, start0, order0 = o.get_voxels()
+    p0, v0 = o.get_state()
+    dx0, dv0 = o.get_history()[:2]
+    rm = rng.random(600) < 0.3
+    new_index = np.cumsum(~rm) - 1
+    o.remove_agents(rm)
+    assert o.n_agents == int((~rm).sum())
+    vox1, start1, order1 = o.get_voxels()
+    want_order = new_index[order0[~rm[order0]]]
+    assert np.array_equal(order1, want_order)
+    counts0 = np.diff(start0.astype(np.int64))
+    removed_per_voxel = np.bincount(vox0[rm], minlength=len(counts0))
+    assert np.array_equal(np.diff(start1.astype(np.int64)), counts0 - removed_per_voxel)
+    assert np.array_equal(vox1, vox0[~rm])
+    p1, v1 = o.get_state()
+    assert np.array_equal(p1, p0[~rm]) and np.array_equal(v1, v0[~rm])
+    dx1, dv1 = o.get_history()[:2]
+    assert np.array_equal(dx1, dx0[:, ~rm]) and np.array_equal(dv1, dv0[:, ~rm])
+    o.remove_agents(np.zeros(o.n_agents, dtype=bool))  # no-op
+    assert o.n_agents == int((~rm).sum())
+    with pytest.raises(O.OracleError):
+        o.remove_agents(np.ones(o.n_agents, dtype=bool))
+    o.close()
+
+
+def test_remove_isolated_agent_leaves_the_others_unchanged():
+    """An agent beyond every cutoff exerts no force: removing it (AB2, with history)
+    leaves the other agents' trajectories bitwise equal to the run that keeps it,
+    while removing an interacting agent changes them."""
+    rng = np.random.default_rng(22)
+    pos = rng.uniform(1e-9, 30.0, size=(300, 3))
+    pos[0] = [0.5, 0.5, 0.5]  # a corner, more than 3 cutoffs from everyone
+    pos = pos[(np.linalg.norm(pos - pos[0], axis=1) > 9.0) | (np.arange(300) == 0)]
+    vel = np.zeros_like(pos)
+    dom = CartesianCuboid.from_boundaries_and_interaction_range([0.0] * 3, [30.0] * 3, 3.0)
+    pot = MorsePotential(radius=1.0, potential_stiffness=1.5, cutoff=3.0, strength=0.5)
+    spec = [InstanceSpec(dom, pot, NewtonDamped(damping_constant=1.0, mass=1.0))]
+
+    def run(index):
+        o = O.OracleSim(spec, dt=0.01, mechanics_solver_order=2)
+        o.set_agents(pos, vel)
+        o.step(5)
+        if index is not None:
+            rm = np.zeros(len(pos), dtype=bool)
+            rm[index] = True
+            o.remove_agents(rm)
+        o.step(10)
+        p = o.get_state()[0]
+        o.close()
+        return p
+
+    base = run(None)
+    assert np.array_equal(run(0), base[1:])
+    d = np.linalg.norm(pos[:, None] - pos[None], axis=2) + np.eye(len(pos)) * 1e9
+    j = int(np.argmin(d.min(axis=1)))
+    assert d.min() < 3.0
+    assert not np.array_equal(run(j), np.delete(base, j, axis=0))
