@@ -1300,16 +1300,18 @@ __global__ __launch_bounds__(kBlock) void k_gather_integrate_point(Arrays A, Sta
         v[k] = src.vel[k * N + o];
         f[k] = A.force[k * N + o];
     }
-    // the ring before this step (min(t, cap) entries), in registers
+    // the ring before this step (min(t, cap) entries), in registers, except the
+    // slot this step overwrites: point_step reads the new increment in its place
     const uint32_t n_old = (uint32_t)min<uint64_t>(t, rg.cap);
     double hx[2][D], hv[2][D];
 #pragma unroll
     for (int h = 0; h < 2; h++) {
+        const bool live = (uint32_t)h < n_old && (uint32_t)h != rg.wslot;  // uniform
 #pragma unroll
         for (int k = 0; k < D; k++) {
             const uint64_t e = ((uint64_t)h * D + k) * N + o;
-            hx[h][k] = (uint32_t)h < n_old ? src.hdx[e] : 0.0;
-            hv[h][k] = (uint32_t)h < n_old ? src.hdv[e] : 0.0;
+            hx[h][k] = live ? src.hdx[e] : 0.0;
+            hv[h][k] = live ? src.hdv[e] : 0.0;
         }
     }
     auto hist = [&](int i, int k, bool is_dv) {

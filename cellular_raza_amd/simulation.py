@@ -357,13 +357,19 @@ def run_simulation(
     cycle=None,
     spring_lengths=None,
     cell_extra=None,
+    remove=None,
     **backend_kwargs,
 ) -> Dict[int, np.ndarray]:
     """Runs the [Mechanics, Interaction] aspects.  Without settings.storage it returns
     {iteration: positions} at the save iterations (iteration 0 is the initial state,
     run_sim.rs:363-367); with a StorageBuilder it stores the cells and the subdomain
     at those iterations as run_simulation! does (run_sim.rs:375-378,397-398,430-445)
-    and returns the StorageAccess."""
+    and returns the StorageAccess.
+
+    remove(backend, iteration) -> mask or None, called after every step, stands for
+    the agents' Cycle returning CycleEvent::Remove in that step: the agents of a
+    returned boolean mask (by agent index) are removed (MechanicsBackend.remove_agents,
+    update_cycle.rs:41-55) before the next step or save point."""
     be = MechanicsBackend(
         [InstanceSpec(domain, interaction, mechanics, domain_force)],
         dt=settings.time.dt,
@@ -393,9 +399,15 @@ def run_simulation(
         it = 0
         for target in settings.time.save_iterations:
             target = min(target, settings.time.maximum_iterations)
-            if target > it:
+            if target > it and remove is None:
                 be.step(target - it)
                 it = target
+            while it < target:  # one step at a time, each followed by the removals
+                be.step(1)
+                it += 1
+                mask = remove(be, it)
+                if mask is not None and np.any(mask):
+                    be.remove_agents(mask)
             if access is None:
                 out[it] = be.get_state()[0]
                 continue

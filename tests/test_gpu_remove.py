@@ -224,3 +224,29 @@ def test_remove_errors():
     assert g.n_agents == 200
     g.step(2)
     g.close()
+
+
+def test_run_simulation_with_removals():
+    """run_simulation(remove=...): the first agent is removed after every third step;
+    the saved positions against the oracle doing the same between single steps."""
+    from cellular_raza_amd import FixedStepsize, Settings, run_simulation
+
+    wl = workloads.c1_morse_2d(n=400, seed=39, steps=10)
+    spec = wl.instances[0]
+    time = FixedStepsize.from_partial_save_steps(0.0, wl.dt, 12, 4)
+
+    def rm(be, it):
+        return (np.arange(be.n_agents) == 0) if it % 3 == 0 else None
+
+    out = run_simulation(spec.domain, spec.interaction, spec.mechanics, wl.pos, Settings(time=time),
+                         velocities=wl.vel, mechanics_solver_order=wl.solver_order, remove=rm, precision="fp64")
+    o = OracleSim(wl.instances, dt=wl.dt, mechanics_solver_order=wl.solver_order)
+    o.set_agents(wl.pos, wl.vel)
+    assert np.array_equal(out[0], wl.pos)
+    for it in range(1, 13):
+        o.step(1)
+        if it % 3 == 0:
+            o.remove_agents(np.arange(o.n_agents) == 0)
+        if it % 4 == 0:
+            assert out[it].shape[0] == o.n_agents == 400 - it // 3
+            assert rel_err(out[it], o.get_state()[0]) <= TRAJ_TOL, it
