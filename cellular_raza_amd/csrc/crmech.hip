@@ -275,6 +275,7 @@ Arrays arrays(crm_ctx* ctx) {
     A.work = ctx->d_work;
     A.ctrl = ctx->d_ctrl;
     A.start = ctx->d_start;
+    A.tile_off = ctx->d_bsum;
     A.count = ctx->d_count;
     A.stay = ctx->d_count + ctx->Vt;
     // two key buffers: the current step's (read by the sort) and the next step's
@@ -516,8 +517,6 @@ crm_status launch_sort(crm_ctx* ctx, bool advance, const StepArgs* sa) {
                        ctx->d_bsum, Vt);
     hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->d_bsum, ctx->n_tiles, ctx->d_start,
                        Vt, ctx->d_ctrl, (uint32_t)(advance ? 1 : 0));
-    hipLaunchKernelGGL(k_scan_add, dim3((Vt + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, ctx->d_start,
-                       ctx->d_bsum, Vt);
     CRM_TRY(t_end(ctx, CRM_KERNEL_SCAN));
     CRM_TRY(t_begin(ctx));
     hipLaunchKernelGGL(k_scatter, dim3((uint32_t)((ctx->N + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream, A);

@@ -5,7 +5,7 @@
 //   k_key_point        next position (solvers.rs:23-347 + apply_boundary,
 //                      cartesian_cuboid_n.rs:594-631) -> next voxel index
 //                      (cartesian_cuboid_n.rs:469-491) and its histogram
-//   k_scan_*           exclusive scan of the voxel histogram            } integer counting sort,
+//   k_scan_local/tiles exclusive scan of the voxel histogram            } integer counting sort,
 //   k_scatter          slot -> sorted position                          } stable by previous slot
 //   k_fixup            per-voxel order by previous slot, histogram reset } (sort_cells_in_voxels,
 //   k_gather_integrate_point  the full mechanics update, written into   }  update_mechanics.rs:418-513)
@@ -45,7 +45,8 @@ struct Arrays {
     const InstParams* ip;
     const BlockWork* work;
     Control* ctrl;
-    uint32_t* start;  // [Vt + 1]
+    uint32_t* start;  // [Vt + 1] (offsets inside the scan tile between k_scan_local and k_fixup)
+    uint32_t* tile_off;  // [n_tiles] offset of each scan tile (k_scan_tiles)
     uint32_t* count;  // [Vt] arrivals per next voxel (atomic ranks)
     uint32_t* stay;   // [Vt] agents binned ahead that stay in their voxel (zero otherwise)
     uint32_t* key;    // [N] next voxel
@@ -1121,26 +1122,25 @@ __global__ __launch_bounds__(kBlock) void k_scan_tiles(uint32_t* __restrict__ bs
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_scan_add(uint32_t* __restrict__ start, const uint32_t* __restrict__ bsum,
-                                                      uint32_t Vt) {
-    const uint64_t q = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (q < Vt) start[q] += bsum[q / kScanTile];
-}
-
+// Between k_scan_local and k_fixup, start[v] holds v's offset inside its scan
+// tile and tile_off[v / kScanTile] (scanned by k_scan_tiles) the tile's offset.
 __global__ __launch_bounds__(kBlock) void k_scatter(Arrays A) {
     const uint64_t s = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     if (s >= A.N) return;
     const uint32_t k = A.key[s], r = A.rank[s];
     const uint32_t pos = (r & kMoved) ? A.stay[k] + (r & ~kMoved) : r;
-    A.perm[A.start[k] + pos] = (uint32_t)s;
+    A.perm[A.start[k] + A.tile_off[k / kScanTile] + pos] = (uint32_t)s;
 }
 
-// Orders each voxel's slots by previous slot (atomic arrival order is not
-// deterministic) and clears the histogram for the next step.
+// Writes the final voxel starts (tile-local offset + tile offset; the end from
+// the voxel's size, so no thread reads a start another one writes), orders each
+// voxel's slots by previous slot (atomic arrival order is not deterministic) and
+// clears the histogram for the next step.
 __global__ __launch_bounds__(kBlock) void k_fixup(Arrays A) {
     const uint64_t v = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     if (v >= A.Vt) return;
-    const uint32_t b = A.start[v], e = A.start[v + 1];
+    const uint32_t b = A.start[v] + A.tile_off[v / kScanTile], e = b + A.count[v] + A.stay[v];
+    A.start[v] = b;
     A.count[v] = 0u;
     for (uint32_t i = b + 1; i < e; i++) {
         uint32_t x = A.perm[i];

@@ -8,7 +8,7 @@
 //                            pair weights (update_mechanics.rs:178-220)
 //   k_key_rod                next vertex positions, per-vertex reflection
 //                            (bacterial_rods.rs:489-533), centroid voxel + rank
-//   k_scan_* / k_scatter / k_fixup   as for point agents (slots are rods)
+//   k_scan_local/tiles, k_scatter, k_fixup   as for point agents (slots are rods)
 //   k_gather_integrate_rod   RodMechanics::calculate_increment (bacterial_rods.rs:157-240)
 //                            + solver, written into the rod's new slot
 // State arrays are the point arrays with K = nv * dim components per slot; the
