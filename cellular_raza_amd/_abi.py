@@ -189,6 +189,7 @@ SIGNATURES = {
     "crm_get_state": (C.c_int, [P, DP, DP]),
     "crm_get_history": (C.c_int, [P, DP, DP, U32P]),
     "crm_get_voxels": (C.c_int, [P, U32P, U32P]),
+    "crm_get_voxel_order": (C.c_int, [P, U32P]),
     "crm_get_info": (C.c_int, [P, C.POINTER(Info)]),
     "crm_autotune": (C.c_int, [P, C.c_char_p, C.POINTER(C.c_uint32)]),
     "crm_get_tiling": (C.c_int, [P, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),

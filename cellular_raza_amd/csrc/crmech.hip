@@ -446,6 +446,14 @@ crm_status launch_key_d(crm_ctx* ctx, StateBuf S, const StepArgs& sa) {
     return CRM_INVALID_ARGUMENT;
 }
 
+// Ranks of the keys of the current order (stayers in order, arrivals atomic).
+crm_status launch_rank(crm_ctx* ctx) {
+    hipLaunchKernelGGL(k_rank_ahead, dim3((uint32_t)((ctx->Vt + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream,
+                       arrays(ctx));
+    CRM_HIP(hipGetLastError());
+    return CRM_OK;
+}
+
 crm_status launch_key(crm_ctx* ctx, const StepArgs& sa) {
     CRM_TRY(t_begin(ctx));
     StateBuf S = ctx->buf[ctx->cur];
@@ -461,6 +469,7 @@ crm_status launch_key(crm_ctx* ctx, const StepArgs& sa) {
         st = ctx->D == 2 ? launch_key_d<2>(ctx, S, sa) : launch_key_d<3>(ctx, S, sa);
     }
     if (st) return st;
+    CRM_TRY(launch_rank(ctx));
     return t_end(ctx, CRM_KERNEL_INTEGRATE);
 }
 
@@ -877,6 +886,9 @@ crm_status launch_cycle_step(crm_ctx* ctx, const double* noise_step) {
             hipLaunchKernelGGL(k_expand_rod<3>, g, b, 0, ctx->stream, A, B2, B, R, sa, Y, n_old, n_new,
                                ctx->cyc.division_radius);
         CRM_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_grow_start, dim3((Vt + 1 + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream,
+                           ctx->d_start, ctx->d_dbefore, Vt);
+        CRM_HIP(hipGetLastError());
         for (uint32_t i = 0; i < ctx->n_inst; i++) ctx->counts[i] += inst_div[i];
         ctx->N = n_new;
         CRM_TRY(rebuild_work(ctx));
@@ -888,6 +900,7 @@ crm_status launch_cycle_step(crm_ctx* ctx, const double* noise_step) {
         else hipLaunchKernelGGL(k_reflect_key_rod<3>, g, b, 0, ctx->stream, A, ctx->buf[ctx->cur], R);
         CRM_HIP(hipGetLastError());
     }
+    CRM_TRY(launch_rank(ctx));
     CRM_TRY(launch_sort(ctx, false, &sa));
     hipLaunchKernelGGL(k_advance_step, dim3(1), dim3(1), 0, ctx->stream, ctx->d_ctrl);
     CRM_HIP(hipGetLastError());
@@ -1488,6 +1501,15 @@ crm_status crm_get_voxels(crm_ctx* ctx, uint32_t* voxel_of_agent, uint32_t* voxe
                                ctx->stream));
         CRM_HIP(hipStreamSynchronize(ctx->stream));
     }
+    return CRM_OK;
+}
+
+crm_status crm_get_voxel_order(crm_ctx* ctx, uint32_t* order) {
+    if (!ctx || !order || !ctx->have_agents) return CRM_INVALID_ARGUMENT;
+    CRM_HIP(hipSetDevice(ctx->device));
+    CRM_HIP(hipMemcpyAsync(order, ctx->buf[ctx->cur].id, ctx->N * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    CRM_HIP(hipStreamSynchronize(ctx->stream));
     return CRM_OK;
 }
 

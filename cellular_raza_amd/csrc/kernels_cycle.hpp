@@ -13,8 +13,9 @@
 //   k_expand_rod        re-layout into the grown order: every voxel's list
 //                       followed by its daughters (the reference's
 //                       cells.extend(new_cells)); RodMechanics::divide for mothers
-//   k_reflect_key_rod   per-vertex reflection, centroid voxel and rank
-//   (the counting sort of the plain step, then k_advance_step)
+//   k_grow_start        voxel starts of the grown order (division steps)
+//   k_reflect_key_rod   per-vertex reflection, centroid voxel
+//   (k_rank_ahead and the counting sort of the plain step, then k_advance_step)
 // Each rod keeps the step of its first history push (`birth`): a daughter starts
 // with an empty AuxStorage, its ring fills from the step after its division.
 #pragma once
@@ -271,7 +272,7 @@ __global__ __launch_bounds__(kBlock) void k_expand_rod(Arrays A, StateBuf src, S
 }
 
 // apply_boundary (per vertex, bacterial_rods.rs:489-533) in place, then the
-// centroid voxel and arrival rank of every rod, as k_key_rod does for the plain step.
+// centroid voxel of every rod, as k_key_rod does for the plain step.
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_reflect_key_rod(Arrays A, StateBuf S, RodArgs R) {
     __shared__ double newpos[kRodLdsK];
@@ -324,8 +325,13 @@ __global__ __launch_bounds__(kBlock) void k_reflect_key_rod(Arrays A, StateBuf S
         key = S.vox[s];
         atomicAdd((unsigned long long*)&A.ctrl->out_of_grid, 1ull);
     }
-    A.key[s] = key;
-    A.rank[s] = atomicAdd(&A.count[key], 1u);
+    A.key[s] = key;  // ranked by k_rank_ahead
+}
+
+// Voxel starts of the grown order: each voxel's list is followed by its daughters.
+__global__ __launch_bounds__(kBlock) void k_grow_start(uint32_t* start, const uint32_t* dbefore, uint32_t Vt) {
+    const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
+    if (g <= Vt) start[g] += dbefore[g];
 }
 
 }  // namespace crm

@@ -4,10 +4,12 @@
 //   k_force_tiled      pair loop over the 3^D voxel stencil (update_mechanics.rs:153-238)
 //   k_key_point        next position (solvers.rs:23-347 + apply_boundary,
 //                      cartesian_cuboid_n.rs:594-631) -> next voxel index
-//                      (cartesian_cuboid_n.rs:469-491) and its histogram
-//   k_scan_local/tiles exclusive scan of the voxel histogram            } integer counting sort,
-//   k_scatter          slot -> sorted position                          } stable by previous slot
-//   k_fixup            per-voxel order by previous slot, histogram reset } (sort_cells_in_voxels,
+//                      (cartesian_cuboid_n.rs:469-491); NewtonDamped and
+//                      Langevin get it from the previous gather instead
+//   k_rank_ahead       ranks: stayers in order, arrivals atomic         } integer counting sort:
+//   k_scan_local/tiles exclusive scan of the voxel histogram            } stayers keep their order,
+//   k_scatter          slot -> sorted position                          } movers appended by
+//   k_fixup            arrivals by previous slot, histogram reset       } previous slot (sort_cells_in_voxels,
 //   k_gather_integrate_point  the full mechanics update, written into   }  update_mechanics.rs:418-513)
 //                      the agent's new sorted slot, and the force records
 //
@@ -989,7 +991,7 @@ __device__ __forceinline__ bool point_step(const InstParams& P, const StepArgs& 
 }
 
 // Next voxel of every agent (get_index_of, cartesian_cuboid_n.rs:469-491) from
-// the position it will have after this step, and its arrival rank there.  A
+// the position it will have after this step (k_rank_ahead ranks it).  A
 // position whose index has no voxel keeps the old one (update_mechanics.rs:430-453).
 template <int D, int MECH>
 __global__ __launch_bounds__(kBlock) void k_key_point(Arrays A, StateBuf S, StepArgs sa) {
@@ -1025,8 +1027,7 @@ __global__ __launch_bounds__(kBlock) void k_key_point(Arrays A, StateBuf S, Step
         key = S.vox[s];
         atomicAdd((unsigned long long*)&A.ctrl->out_of_grid, 1ull);
     }
-    A.key[s] = key;
-    A.rank[s] = atomicAdd(&A.count[key], 1u);
+    A.key[s] = key;  // ranked by k_rank_ahead
 }
 
 // Initial binning (Domain::decompose, concepts-derive/src/domain.rs:357-380 with
@@ -1134,18 +1135,26 @@ __global__ __launch_bounds__(kBlock) void k_scatter(Arrays A) {
 
 // Writes the final voxel starts (tile-local offset + tile offset; the end from
 // the voxel's size, so no thread reads a start another one writes), orders each
-// voxel's slots by previous slot (atomic arrival order is not deterministic) and
-// clears the histogram for the next step.
+// voxel's arrivals by previous slot (atomic arrival order is not deterministic)
+// and clears the histogram for the next step.  The resulting list is the
+// reference's: the cells that stay keep their order and the movers are appended
+// in order of old voxel, then old position (sort_cells_in_voxels_step_1/2,
+// update_mechanics.rs:418-513).  Set-up sorts rank every agent atomically with
+// stay = 0 and get the whole voxel in order of previous slot.
 __global__ __launch_bounds__(kBlock) void k_fixup(Arrays A) {
     const uint64_t v = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     if (v >= A.Vt) return;
-    const uint32_t b = A.start[v] + A.tile_off[v / kScanTile], e = b + A.count[v] + A.stay[v];
+    const uint32_t stay = A.stay[v];
+    const uint32_t b = A.start[v] + A.tile_off[v / kScanTile], e = b + A.count[v] + stay;
     A.start[v] = b;
     A.count[v] = 0u;
-    for (uint32_t i = b + 1; i < e; i++) {
+    // the stayers hold [b, b + stay) in their old order; the arrivals follow, put in
+    // order of previous slot, i.e. of (old voxel, old position)
+    const uint32_t m = b + stay;
+    for (uint32_t i = m + 1; i < e; i++) {
         uint32_t x = A.perm[i];
         uint32_t j = i;
-        while (j > b) {
+        while (j > m) {
             uint32_t y = A.perm[j - 1];
             if (y <= x) break;
             A.perm[j] = y;
@@ -1214,13 +1223,12 @@ __device__ __forceinline__ void bin_ahead(const Arrays& A, const InstParams& P, 
     A.key_next[s] = key;
 }
 
-// Arrival ranks of the keys binned ahead, with an atomic only for agents that
-// change voxel.  Thread per voxel v of the current order: the agents of v that
-// stay get consecutive ranks in slot order and their number goes to stay[v];
-// the others take an atomic rank among the arrivals of their next voxel
-// (count), flagged with kMoved, which the scatter places after that voxel's
-// stayers.  k_fixup orders every voxel by previous slot afterwards, so only the
-// uniqueness of the ranks matters.
+// Ranks of the next voxels (binned ahead by the gather, or by a key kernel after
+// the forces), with an atomic only for agents that change voxel.  Thread per
+// voxel v of the current order: the agents of v that stay get consecutive ranks
+// in slot order and their number goes to stay[v]; the others take an atomic rank
+// among the arrivals of their next voxel (count), flagged with kMoved, which the
+// scatter places after that voxel's stayers; k_fixup puts the arrivals in order.
 __global__ __launch_bounds__(kBlock) void k_rank_ahead(Arrays A) {
     const uint64_t v = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     if (v >= A.Vt) return;

@@ -672,8 +672,7 @@ __global__ __launch_bounds__(kBlock) void k_key_rod(Arrays A, StateBuf S, RodArg
         key = S.vox[s];
         atomicAdd((unsigned long long*)&A.ctrl->out_of_grid, 1ull);
     }
-    A.key[s] = key;
-    A.rank[s] = atomicAdd(&A.count[key], 1u);
+    A.key[s] = key;  // ranked by k_rank_ahead
 }
 
 // Per-rod fields that follow the rod through a sort or a re-layout (strides of

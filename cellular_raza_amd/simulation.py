@@ -246,6 +246,13 @@ class MechanicsBackend:
         )
         return vox, start
 
+    def voxel_order(self) -> np.ndarray:
+        """Agent indices in voxel-list order (crm_get_voxel_order): with the starts
+        of get_voxels, the cell list of every voxel."""
+        out = np.empty(self.n_agents, dtype=np.uint32)
+        self._check(self._lib.crm_get_voxel_order(self._h, out.ctypes.data_as(_abi.U32P)))
+        return out
+
     def tiling(self, inst: int = 0) -> Dict[str, int]:
         """Force tiling of instance `inst`: voxel rows and columns per tile, tile count."""
         r, c, n = C.c_uint32(), C.c_uint32(), C.c_uint32()

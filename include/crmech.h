@@ -240,6 +240,14 @@ crm_status crm_get_history(crm_ctx* ctx, double* dx_hist, double* dv_hist, uint3
  * (n_voxels_total + 1 entries); either pointer may be NULL. */
 crm_status crm_get_voxels(crm_ctx* ctx, uint32_t* voxel_of_agent, uint32_t* voxel_start);
 
+/* The voxels' cell lists: order[n_agents] holds agent indices in sorted order, so
+ * order[voxel_start[v] .. voxel_start[v + 1]) lists voxel v's agents in the order
+ * of the reference's Voxel::cells (datastructures.rs:349-396): after each step the
+ * agents that stayed keep their order and the arrivals follow by old voxel, then
+ * old position (sort_cells_in_voxels, update_mechanics.rs:418-513); daughters of
+ * the rods cycle follow their voxel's list (update_cycle.rs:57-66). */
+crm_status crm_get_voxel_order(crm_ctx* ctx, uint32_t* order);
+
 crm_status crm_get_info(crm_ctx* ctx, crm_info* info);
 
 /* Tiling autotuner (no counterpart in the reference, whose CPU backend has no

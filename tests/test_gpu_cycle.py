@@ -49,6 +49,10 @@ def pair(spec, cycle, pos, vel=None, spring=None, dt=0.1, order=2):
 
 def assert_same(g, o, tag, tol=1e-7):
     assert g.n_agents == o.n_agents, tag
+    # the voxels' lists (the order that numbers the mothers and appends daughters)
+    vo, so, order = o.get_voxels()
+    assert np.array_equal(g.get_voxels()[1], so), tag
+    assert np.array_equal(g.voxel_order(), order.astype(np.uint32)), tag
     sg, so = g.cycle_state(), o.cycle_state()
     assert np.array_equal(sg["identifier"], so["identifier"]), tag
     assert np.array_equal(sg["parent"], so["parent"]), tag

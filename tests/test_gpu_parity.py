@@ -193,7 +193,9 @@ def test_voxels_and_neighbour_sets_bit_exact():
         pg = g.get_state()[0]
         # binning of the GPU positions by the reference formula, computed independently
         assert np.array_equal(vg.astype(np.int64), voxel_reference(dom, pg))
-        # same agent sets per voxel as the oracle (order inside a voxel may differ)
+        # same agent lists per voxel as the oracle, in the reference's order: the
+        # agents that stayed keep their order, the arrivals follow by old voxel
+        assert np.array_equal(g.voxel_order(), order.astype(np.uint32))
         assert np.array_equal(vg, vo)
         assert np.array_equal(sg, so)
         assert sg[-1] == wl.n_agents and np.all(np.diff(sg.astype(np.int64)) >= 0)

@@ -48,9 +48,10 @@ def pair(wl, **kw):
 def assert_same(g, o, tag, tol=TRAJ_TOL):
     assert g.n_agents == o.n_agents, tag
     vg, sg = g.get_voxels()
-    vo, so, _ = o.get_voxels()
+    vo, so, order = o.get_voxels()
     assert np.array_equal(vg, vo), tag
     assert np.array_equal(sg, so), tag
+    assert np.array_equal(g.voxel_order(), order.astype(np.uint32)), tag
     pg, vlg = g.get_state()
     po, vlo = o.get_state()
     assert rel_err(pg, po) <= tol, (tag, rel_err(pg, po))

@@ -145,9 +145,11 @@ def test_rod_voxels_centroid_binning_bit_exact():
         g.step(2)
         o.step(2)
         vg, sg = g.get_voxels()
-        vo, so, _ = o.get_voxels()
+        vo, so, order = o.get_voxels()
         assert np.array_equal(vg, vo)
         assert np.array_equal(sg, so)
+        # the voxels' lists in the reference's order (stayers, then arrivals)
+        assert np.array_equal(g.voxel_order(), order.astype(np.uint32))
         # independent check: centroid voxel by trunc((c - min) / dx)
         c = g.get_state()[0].reshape(len(pos), -1, 2).sum(axis=1) / pos.shape[1]
         idx = [np.trunc((c[:, k] - dom.min[k]) / dx[k]).astype(np.int64) for k in range(2)]
@@ -242,5 +244,6 @@ def test_c3_scaled_invariants():
     o.step(5)
     assert rel_err(g.get_state()[0], o.get_state()[0]) <= TRAJ_TOL
     vg, sg = g.get_voxels()
-    vo, so, _ = o.get_voxels()
+    vo, so, order = o.get_voxels()
     assert np.array_equal(vg, vo) and np.array_equal(sg, so)
+    assert np.array_equal(g.voxel_order(), order.astype(np.uint32))

@@ -708,3 +708,33 @@ def test_remove_isolated_agent_leaves_the_others_unchanged():
     j = int(np.argmin(d.min(axis=1)))
     assert d.min() < 3.0
     assert not np.array_equal(run(j), np.delete(base, j, axis=0))
+
+
+def test_rebinning_keeps_stayers_and_appends_movers():
+    """sort_cells_in_voxels_step_1/2 (update_mechanics.rs:418-513) with one
+    subdomain: a voxel's new list is its stayers in their old order followed by the
+    arrivals in order of old voxel, then old position -- computed here from the
+    lists before and the voxels after each step."""
+    rng = np.random.default_rng(24)
+    L = 30.0
+    dom = CartesianCuboid.from_boundaries_and_interaction_range([0.0] * 3, [L] * 3, 3.0)
+    pot = MorsePotential(radius=1.0, potential_stiffness=1.5, cutoff=3.0, strength=0.5)
+    spec = [InstanceSpec(dom, pot, NewtonDamped(damping_constant=0.1, mass=1.0))]
+    o = O.OracleSim(spec, dt=0.02, mechanics_solver_order=1)
+    n = 3000
+    o.set_agents(rng.uniform(1.0, L - 1.0, size=(n, 3)), rng.normal(scale=8.0, size=(n, 3)))
+    moved = 0
+    for _ in range(4):
+        vox0, start0, order0 = o.get_voxels()
+        o.step(1)
+        vox1, start1, order1 = o.get_voxels()
+        pos_in_old = np.empty(n, dtype=np.int64)
+        pos_in_old[order0] = np.arange(n)
+        for w in np.flatnonzero(np.diff(start1.astype(np.int64))):
+            old = order0[start0[w]:start0[w + 1]]
+            stay = [a for a in old if vox1[a] == w]
+            arrivals = sorted((a for a in np.flatnonzero((vox1 == w) & (vox0 != w))), key=lambda a: pos_in_old[a])
+            assert list(order1[start1[w]:start1[w + 1]]) == stay + arrivals, w
+            moved += len(arrivals)
+    assert moved > 100
+    o.close()
