@@ -29,7 +29,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-PROFILE = os.path.join(ROOT, "profiles", "c2_cell_sorting_1M.md")
+# committed rocprofv3 summaries of the workloads that have one
+PROFILES = {"c2": "profiles/c2_cell_sorting_1M.md", "c3": "profiles/c3_rods_2d_200k.md"}
 
 
 def candidates_per_agent(wl, start):
@@ -55,12 +56,14 @@ def candidates_per_agent(wl, start):
     return tot / num
 
 
-def profile_figure(table, kernel):
-    """A per-kernel figure from the committed rocprofv3 summary: `hbm_bytes_per_dispatch`
-    (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known size) or
-    `valu_issue_utilisation`; None when absent."""
+def profile_figure(table, kernel, workload):
+    """A per-kernel figure from the workload's committed rocprofv3 summary:
+    `hbm_bytes_per_dispatch` (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of
+    known size) or `valu_issue_utilisation`; None when absent."""
+    if workload not in PROFILES:
+        return None
     try:
-        text = open(PROFILE).read()
+        text = open(os.path.join(ROOT, PROFILES[workload])).read()
         block = text.split("```json", 1)[1].split("```", 1)[0]
         return json.loads(block)[table].get(kernel)
     except (OSError, IndexError, KeyError, ValueError):
@@ -436,11 +439,10 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": profile_figure("hbm_bytes_per_dispatch", force_kernel) if args.workload == "c2" else None,
-            "valu_issue_utilisation": profile_figure("valu_issue_utilisation", force_kernel)
-            if args.workload == "c2" else None,
-            "traffic_source": "profiles/c2_cell_sorting_1M.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated; "
-                              "SQ_INSTS_VALU for the VALU issue utilisation)" if args.workload == "c2" else None,
+            "traffic": profile_figure("hbm_bytes_per_dispatch", force_kernel, args.workload),
+            "valu_issue_utilisation": profile_figure("valu_issue_utilisation", force_kernel, args.workload),
+            "traffic_source": PROFILES[args.workload] + " (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated; "
+                              "SQ_INSTS_VALU for the VALU issue utilisation)" if args.workload in PROFILES else None,
             "kernel": force_kernel,
             "bytes_per_agent": bf,
             "candidates_per_agent": cbar,

@@ -515,6 +515,8 @@ crm_status launch_gather_setup_d(crm_ctx* ctx, StateBuf src, StateBuf dst, const
     return CRM_INVALID_ARGUMENT;
 }
 
+bool keys_ahead(const crm_ctx* ctx);
+
 // scan + scatter + fixup + gather: the counting sort that rebuilds the voxel
 // lists from d_key / d_rank / d_count and moves the state to the other buffer.
 crm_status launch_sort(crm_ctx* ctx, bool advance, const StepArgs* sa) {
@@ -542,12 +544,23 @@ crm_status launch_sort(crm_ctx* ctx, bool advance, const StepArgs* sa) {
         const RodArgs R = rod_args(ctx);
         const StepArgs sa0 = sa ? *sa : step_args(ctx, nullptr);
         const dim3 g(ctx->n_work), b(kBlock);
+        const bool ahead = keys_ahead(ctx);
         if (ctx->D == 2) {
-            if (advance) hipLaunchKernelGGL((k_gather_rod<2, true>), g, b, 0, ctx->stream, A, src, dst, R, sa0);
-            else hipLaunchKernelGGL((k_gather_rod<2, false>), g, b, 0, ctx->stream, A, src, dst, R, sa0);
+            if (ahead) {
+                if (advance) hipLaunchKernelGGL((k_gather_rod<2, true, true>), g, b, 0, ctx->stream, A, src, dst, R, sa0);
+                else hipLaunchKernelGGL((k_gather_rod<2, false, true>), g, b, 0, ctx->stream, A, src, dst, R, sa0);
+            } else {
+                if (advance) hipLaunchKernelGGL((k_gather_rod<2, true, false>), g, b, 0, ctx->stream, A, src, dst, R, sa0);
+                else hipLaunchKernelGGL((k_gather_rod<2, false, false>), g, b, 0, ctx->stream, A, src, dst, R, sa0);
+            }
         } else {
-            if (advance) hipLaunchKernelGGL((k_gather_rod<3, true>), g, b, 0, ctx->stream, A, src, dst, R, sa0);
-            else hipLaunchKernelGGL((k_gather_rod<3, false>), g, b, 0, ctx->stream, A, src, dst, R, sa0);
+            if (ahead) {
+                if (advance) hipLaunchKernelGGL((k_gather_rod<3, true, true>), g, b, 0, ctx->stream, A, src, dst, R, sa0);
+                else hipLaunchKernelGGL((k_gather_rod<3, false, true>), g, b, 0, ctx->stream, A, src, dst, R, sa0);
+            } else {
+                if (advance) hipLaunchKernelGGL((k_gather_rod<3, true, false>), g, b, 0, ctx->stream, A, src, dst, R, sa0);
+                else hipLaunchKernelGGL((k_gather_rod<3, false, false>), g, b, 0, ctx->stream, A, src, dst, R, sa0);
+            }
         }
     } else if (advance) {
         if (ctx->D == 2)
@@ -569,10 +582,14 @@ crm_status launch_sort(crm_ctx* ctx, bool advance, const StepArgs* sa) {
     return CRM_OK;
 }
 
-// NewtonDamped and Langevin point agents are binned one step ahead by the
-// gather (their position update needs neither force nor noise); the others get
-// a key kernel after the forces.
-bool keys_ahead(const crm_ctx* ctx) { return !is_rod(ctx) && ctx->mech_kind != CRM_MECHANICS_BROWNIAN; }
+// NewtonDamped and Langevin point agents, and rods outside the cycle, are binned
+// one step ahead by the gather (their position update needs no force, and no
+// noise but Philox's of the next step, so not with a caller's noise buffer); the
+// others get a key kernel after the forces.
+bool keys_ahead(const crm_ctx* ctx) {
+    if (is_rod(ctx)) return !ctx->cycle_on && ctx->cfg.noise_mode != CRM_NOISE_BUFFER;
+    return ctx->mech_kind != CRM_MECHANICS_BROWNIAN;
+}
 
 crm_status launch_step(crm_ctx* ctx, const double* noise_step) {
     const StepArgs sa = step_args(ctx, noise_step);

@@ -689,13 +689,38 @@ __device__ __forceinline__ void copy_rod_extras(const StateBuf& src, uint64_t o,
     }
 }
 
+// Position of vertex row v after step t1 from its state at t1 (x, vel and the
+// ring through hist), reflected (the position update of k_key_rod), into out[D].
+template <int D, typename Hist>
+__device__ __forceinline__ void rod_ahead(const InstParams& P, const StepArgs& sa, uint64_t t1, uint32_t id,
+                                          uint32_t nv, uint32_t v, const double* x, const double* vel, Hist hist,
+                                          double* out) {
+    const Ring rg = ring_at(t1, sa.cap, sa.newest_first);
+    double dx[D], dv[D], dxr[D], nx[D], unused[D];
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+        dx[k] = vel[k];
+        dv[k] = 0.0;
+    }
+    rod_row_noise<D>(P, sa, id, t1, nv, v, dxr);
+    rod_row_solve<D, false>(sa, rg, x, vel, dx, dv, dxr, hist, nx, unused);
+    rod_row_reflect<D, false>(P, nx, unused);
+#pragma unroll
+    for (int k = 0; k < D; k++) out[k] = nx[k];
+}
+
 // Gather + integrate for rods: lane (s, v) computes vertex row v of the rod that
-// lands in sorted slot s (coming from slot o = perm[s]).
-template <int D, bool INTEGRATE>
+// lands in sorted slot s (coming from slot o = perm[s]).  With AHEAD it also bins
+// the rod for the next step, as k_key_rod would after that step's forces: the
+// position update needs neither the force nor noise other than the next step's
+// (Philox, or none), so the new vertex positions, velocities and ring give the
+// next centroid; the step's boundary error is raised here then.
+template <int D, bool INTEGRATE, bool AHEAD>
 __global__ __launch_bounds__(kBlock) void k_gather_rod(Arrays A, StateBuf src, StateBuf dst, RodArgs R,
                                                          StepArgs sa) {
     __shared__ double newpos[kRodLdsK];
     __shared__ double win[INTEGRATE ? kRodWinK : 1];
+    __shared__ double aheadpos[AHEAD ? kRodLdsK : 1];
     const BlockWork w = A.work[blockIdx.x];
     uint64_t s;
     uint32_t v;
@@ -738,7 +763,17 @@ __global__ __launch_bounds__(kBlock) void k_gather_rod(Arrays A, StateBuf src, S
                 return h[((uint64_t)rg.it[i] * K + v * D + k) * N + o];
             };
             rod_row_solve<D, true>(sa, rg, x, vv, dx, dv, dxr, hist, nx, nvel);
-            rod_row_reflect<D, true>(P, nx, nvel);
+            const bool out = rod_row_reflect<D, true>(P, nx, nvel);
+            if (AHEAD && out) raise_error(A.ctrl, CRM_BOUNDARY_ERROR, t, id);
+            if constexpr (AHEAD) {
+                // the ring after this step: slot rg.wslot holds dx, the others src's
+                const Ring rg1 = ring_at(t + 1, sa.cap, sa.newest_first);
+                auto hist1 = [&](int i, int k, bool) {
+                    const uint32_t sl = rg1.it[i];
+                    return sl == rg.wslot ? dx[k] : src.hdx[((uint64_t)sl * K + v * D + k) * N + o];
+                };
+                rod_ahead<D>(P, sa, t + 1, id, nv, v, nx, nvel, hist1, aheadpos + threadIdx.x * 3);
+            }
             if (rg.cap > 0) {
                 const uint32_t n_hist = (uint32_t)min<uint64_t>(t + 1, rg.cap);
                 for (uint32_t h = 0; h < n_hist; h++) {
@@ -765,6 +800,14 @@ __global__ __launch_bounds__(kBlock) void k_gather_rod(Arrays A, StateBuf src, S
                     dst.hdx[e + s] = src.hdx[e + o];
                     dst.hdv[e + s] = src.hdv[e + o];
                 }
+            }
+            if constexpr (AHEAD) {
+                const uint64_t t1 = A.ctrl->step;
+                const Ring rg1 = ring_at(t1, sa.cap, sa.newest_first);
+                auto hist1 = [&](int i, int k, bool) {
+                    return src.hdx[((uint64_t)rg1.it[i] * K + v * D + k) * N + o];
+                };
+                rod_ahead<D>(P, sa, t1, id, nv, v, nx, nvel, hist1, aheadpos + threadIdx.x * 3);
             }
         }
 #pragma unroll
@@ -815,6 +858,27 @@ __global__ __launch_bounds__(kBlock) void k_gather_rod(Arrays A, StateBuf src, S
         // radius rounded up by a relative 1e-12 so that it covers every vertex
         const double radius = degenerate ? __builtin_inf() : sqrt(r2) * (1.0 + 1e-12);
         R.bnd[s] = make_double4(c[0], c[1], D == 3 ? c[2] : 0.0, radius);
+        if constexpr (AHEAD) {
+            // next voxel: the centroid after the next step (row_sum / nrows), as k_key_rod
+            double c1[D];
+#pragma unroll
+            for (int k = 0; k < D; k++) c1[k] = 0.0;
+            for (uint32_t q = 0; q < nv; q++) {
+#pragma unroll
+                for (int k = 0; k < D; k++) c1[k] = c1[k] + aheadpos[(threadIdx.x + q) * 3 + k];
+            }
+            uint32_t idx[3] = {0, 0, 0};
+            bool ok = true;
+#pragma unroll
+            for (int k = 0; k < D; k++) {
+                const int ax = k + 3 - D;
+                ok = ok && axis_index(c1[k] / (double)nv, P.min[ax], P.dx[ax], P.n[ax], idx[ax]);
+            }
+            uint32_t key = dst.vox[s];
+            if (ok) key = P.voxel_offset + (idx[0] * P.n[1] + idx[1]) * P.n[2] + idx[2];
+            else atomicAdd((unsigned long long*)&A.ctrl->out_of_grid, 1ull);
+            A.key_next[s] = key;
+        }
     }
 }
 
