@@ -329,13 +329,15 @@ def main():
 
     factory, has_mixed, desc, force_kernel = workload_table()[args.workload]
     clamped = None
-    if args.workload == "c1" and args.warmup + args.steps > 65:
+    if args.workload == "c1" and args.warmup + args.steps > 35:
         # 1000 discs of radius 6 in 250 x 250 overlap 1.8-fold: the overlapping pairs
-        # throw an agent out of the domain (BoundaryError) after 90 to 100 steps -- the
-        # system is chaotic, so the step depends on the rounding (FP64 GPU 90, oracle
-        # 101) -- so the run stays in the first 65 steps, in FP64
-        clamped = f"warmup {args.warmup} + steps {args.steps} clamped to 5 + 60"
-        args.warmup, args.steps = 5, 60
+        # throw an agent out of the domain (BoundaryError) within 40 to 100 steps --
+        # the system is chaotic, so the step depends on the summation order of the
+        # forces (FP64 GPU step 40, oracle step 100; every GPU step agrees with the
+        # oracle restarted from the GPU state) -- so the run stays in the first 35
+        # steps, in FP64
+        clamped = f"warmup {args.warmup} + steps {args.steps} clamped to 5 + 30"
+        args.warmup, args.steps = 5, 30
     wl = factory(args)
     N = wl.n_agents
     D = wl.dim
