@@ -269,6 +269,10 @@ void free_device(crm_ctx* ctx) {
         }
 }
 
+// d_count holds the arrivals per voxel, then (from this offset, 256-byte aligned)
+// the stayers per voxel.
+uint64_t stay_offset(const crm_ctx* ctx) { return (ctx->Vt + 63) & ~(uint64_t)63; }
+
 Arrays arrays(crm_ctx* ctx) {
     Arrays A;
     A.ip = ctx->d_ip;
@@ -277,7 +281,7 @@ Arrays arrays(crm_ctx* ctx) {
     A.start = ctx->d_start;
     A.tile_off = ctx->d_bsum;
     A.count = ctx->d_count;
-    A.stay = ctx->d_count + ctx->Vt;
+    A.stay = ctx->d_count + stay_offset(ctx);
     // two key buffers: the current step's (read by the sort) and the next step's
     // (binned ahead by the gather), by state-buffer parity
     A.key = ctx->d_key + (size_t)ctx->cur * ctx->N;
@@ -524,7 +528,7 @@ crm_status launch_sort(crm_ctx* ctx, bool advance, const StepArgs* sa) {
     const uint32_t Vt = (uint32_t)ctx->Vt;
     CRM_TRY(t_begin(ctx));
     hipLaunchKernelGGL(k_scan_local, dim3(ctx->n_tiles), dim3(kBlock), 0, ctx->stream, ctx->d_count,
-                       ctx->d_count + Vt, ctx->d_start,
+                       ctx->d_count + stay_offset(ctx), ctx->d_start,
                        ctx->d_bsum, Vt);
     hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->d_bsum, ctx->n_tiles, ctx->d_start,
                        Vt, ctx->d_ctrl, (uint32_t)(advance ? 1 : 0));
@@ -1296,7 +1300,7 @@ crm_status crm_set_agents(crm_ctx* ctx, const uint64_t* counts, const double* po
         CRM_TRY(dalloc(ctx, ctx->d_work, std::max<size_t>(work.size(), Cw / per + ctx->n_inst + 1)));
     }
     CRM_TRY(dalloc(ctx, ctx->d_start, Vt + 1));
-    CRM_TRY(dalloc(ctx, ctx->d_count, 2 * Vt));  // arrivals, then stayers
+    CRM_TRY(dalloc(ctx, ctx->d_count, stay_offset(ctx) + Vt));  // arrivals, then stayers
     CRM_TRY(dalloc(ctx, ctx->d_bsum, ctx->n_tiles));
     // N-sized arrays hold `C` agents: the rods of a cycle grow up to max_agents
     const uint64_t C = (is_rod(ctx) && ctx->cycle_on) ? std::max<uint64_t>(N, ctx->cyc.max_agents) : N;
@@ -1340,7 +1344,7 @@ crm_status crm_set_agents(crm_ctx* ctx, const uint64_t* counts, const double* po
     }
     CRM_HIP(hipMemcpy(ctx->d_work, work.data(), sizeof(BlockWork) * work.size(), hipMemcpyHostToDevice));
     CRM_TRY(upload_tiles(ctx, tiles));
-    CRM_HIP(hipMemsetAsync(ctx->d_count, 0, sizeof(uint32_t) * std::max<uint64_t>(2 * Vt, 1), ctx->stream));
+    CRM_HIP(hipMemsetAsync(ctx->d_count, 0, sizeof(uint32_t) * (stay_offset(ctx) + Vt), ctx->stream));
 
     // upload in agent-index order (slot = agent index before the first sort)
     ctx->cur = 0;
@@ -1826,7 +1830,7 @@ crm_status crm_remove_agents(crm_ctx* ctx, const uint8_t* remove) {
             g = nullptr;
         }
     // the voxel lists of the survivors: a set-up sort with every agent in its voxel
-    CRM_HIP(hipMemsetAsync(ctx->d_count, 0, sizeof(uint32_t) * 2 * ctx->Vt, ctx->stream));
+    CRM_HIP(hipMemsetAsync(ctx->d_count, 0, sizeof(uint32_t) * (stay_offset(ctx) + ctx->Vt), ctx->stream));
     hipLaunchKernelGGL(k_remove_key, dim3((uint32_t)((n_new + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream,
                        arrays(ctx), ctx->buf[ctx->cur]);
     CRM_HIP(hipGetLastError());

@@ -1078,26 +1078,55 @@ constexpr int kScanItems = 8;
 constexpr int kScanTile = kBlock * kScanItems;
 
 
+// Each thread scans kScanItems consecutive voxels; whole groups move as two
+// 16-byte loads per array and two 16-byte stores (count, stay and start are
+// 256-byte aligned and a group starts at a multiple of 8 voxels).
 __global__ __launch_bounds__(kBlock) void k_scan_local(const uint32_t* __restrict__ count,
                                                         const uint32_t* __restrict__ stay,
                                                         uint32_t* __restrict__ start, uint32_t* __restrict__ bsum,
                                                         uint32_t Vt) {
+    static_assert(kScanItems == 8, "two uint4 per group");
     const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+    const bool whole = base + kScanItems <= Vt;
     uint32_t vals[kScanItems];
+    if (whole) {
+        const uint4* c4 = reinterpret_cast<const uint4*>(count + base);
+        const uint4* s4 = reinterpret_cast<const uint4*>(stay + base);
+        const uint4 c0 = c4[0], c1 = c4[1], s0 = s4[0], s1 = s4[1];
+        vals[0] = c0.x + s0.x;
+        vals[1] = c0.y + s0.y;
+        vals[2] = c0.z + s0.z;
+        vals[3] = c0.w + s0.w;
+        vals[4] = c1.x + s1.x;
+        vals[5] = c1.y + s1.y;
+        vals[6] = c1.z + s1.z;
+        vals[7] = c1.w + s1.w;
+    } else {
+#pragma unroll
+        for (int i = 0; i < kScanItems; i++) {
+            const uint64_t q = base + i;
+            vals[i] = q < Vt ? count[q] + stay[q] : 0u;
+        }
+    }
     uint32_t sum = 0;
 #pragma unroll
-    for (int i = 0; i < kScanItems; i++) {
-        uint64_t q = base + i;
-        vals[i] = q < Vt ? count[q] + stay[q] : 0u;
-        sum += vals[i];
-    }
+    for (int i = 0; i < kScanItems; i++) sum += vals[i];
     uint32_t total;
     uint32_t run = block_exclusive_scan(sum, total);
+    uint32_t out[kScanItems];
 #pragma unroll
     for (int i = 0; i < kScanItems; i++) {
-        uint64_t q = base + i;
-        if (q < Vt) start[q] = run;
+        out[i] = run;
         run += vals[i];
+    }
+    if (whole) {
+        uint4* o4 = reinterpret_cast<uint4*>(start + base);
+        o4[0] = make_uint4(out[0], out[1], out[2], out[3]);
+        o4[1] = make_uint4(out[4], out[5], out[6], out[7]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < kScanItems; i++)
+            if (base + i < Vt) start[base + i] = out[i];
     }
     if (threadIdx.x == 0) bsum[blockIdx.x] = total;
 }
