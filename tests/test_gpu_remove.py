@@ -251,3 +251,34 @@ def test_run_simulation_with_removals():
         if it % 4 == 0:
             assert out[it].shape[0] == o.n_agents == 400 - it // 3
             assert rel_err(out[it], o.get_state()[0]) <= TRAJ_TOL, it
+
+
+def test_remove_at_full_size_invariants():
+    """1 M agents (the 3-D sorting workload, mixed precision): 10 % removed between
+    steps.  The survivors' rows are unchanged by the removal; after more steps the
+    voxel lists are a permutation of the agents, each list holds the agents whose
+    positions bin to that voxel by trunc((p - min) / dx), and the starts add up."""
+    wl = workloads.c2_sorting_3d(n=1_000_000, seed=40, steps=10)
+    g = MechanicsBackend(wl.instances, dt=wl.dt, mechanics_solver_order=wl.solver_order, precision="mixed",
+                         noise="philox", seed=2)
+    g.set_agents(wl.pos, wl.vel, wl.species)
+    g.step(4)
+    rm = np.random.default_rng(41).random(g.n_agents) < 0.1
+    p_before = g.get_state()[0]
+    g.remove_agents(rm)
+    assert g.n_agents == int((~rm).sum())
+    assert np.array_equal(g.get_state()[0], p_before[~rm])
+    g.step(4)
+    vox, start = g.get_voxels()
+    order = g.voxel_order()
+    assert start[0] == 0 and start[-1] == g.n_agents and np.all(np.diff(start.astype(np.int64)) >= 0)
+    assert np.array_equal(np.sort(order), np.arange(g.n_agents, dtype=np.uint32))
+    counts = np.diff(start.astype(np.int64))
+    assert np.array_equal(vox[order], np.repeat(np.arange(len(counts)), counts))
+    dom = wl.instances[0].domain
+    n, dx, pos = dom.get_n_voxels(), dom.get_dx(), g.get_state()[0]
+    plain = np.zeros(g.n_agents, dtype=np.int64)
+    for k in range(3):
+        plain = plain * n[k] + np.trunc((pos[:, k] - dom.min[k]) / dx[k]).astype(np.int64)
+    assert np.array_equal(vox.astype(np.int64), plain)
+    g.close()
