@@ -332,8 +332,10 @@ __device__ __forceinline__ void gaussian_pair(uint64_t seed, uint64_t agent, uin
     double u2 = (double)(d >> 11) * inv53;
     double rad = sqrt(-2.0 * log(u1));
     double th = 6.283185307179586 * u2;
-    z0 = rad * cos(th);
-    z1 = rad * sin(th);
+    double sn, cs;
+    sincos(th, &sn, &cs);  // one argument reduction for both
+    z0 = rad * cs;
+    z1 = rad * sn;
 }
 
 // ---------------------------------------------------------------------------
