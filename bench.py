@@ -107,6 +107,30 @@ def time_rods_colony(steps=100):
     return {"steps": steps, **out}
 
 
+def time_removal(wl, precision, frac=0.1, repeats=3):
+    """CycleEvent::Remove at scale: `frac` of the agents of the workload removed
+    between steps (crm_remove_agents: flags to the device, compaction, set-up sort),
+    `repeats` times in a row; wall clock per removal and removed agents per second."""
+    from cellular_raza_amd import MechanicsBackend
+
+    g = MechanicsBackend(wl.instances, dt=wl.dt, mechanics_solver_order=wl.solver_order, noise=wl.noise, seed=1,
+                         precision=precision)
+    g.set_agents(wl.pos, wl.vel, wl.species, wl.counts)
+    g.step(4)
+    rng = np.random.default_rng(11)
+    secs, removed = [], []
+    for _ in range(repeats):
+        rm = rng.random(g.n_agents) < frac
+        t0 = time.perf_counter()
+        g.remove_agents(rm)
+        secs.append(time.perf_counter() - t0)
+        removed.append(int(rm.sum()))
+        g.step(2)
+    g.close()
+    return {"agents_before": wl.n_agents, "removed": removed, "s": secs,
+            "removed_per_s": sum(removed) / sum(secs), "ms_per_removal": 1e3 * sum(secs) / len(secs)}
+
+
 def time_rods_example():
     """The bacterial rods example end to end (5 rods growing and dividing for 25 h of
     simulated time, dt 0.1 min): GPU wall clock for all 15000 steps, and for the
@@ -307,6 +331,9 @@ def main():
     ap.add_argument("--snapshot", action="store_true",
                     help="also time one save point: the device gather into agent order with its copy to the host, "
                          "and the cells written in the reference's JSON layout (to a temporary folder)")
+    ap.add_argument("--removal", action="store_true",
+                    help="also time agent removal (CycleEvent::Remove): 10 %% of the workload's agents removed "
+                         "between steps, three times")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy (reference for the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -378,6 +405,9 @@ def main():
     if args.rods_example:
         extra["rods_example"] = time_rods_example()
         print(f"bench: rods example {extra['rods_example']}", file=sys.stderr, flush=True)
+    if args.removal:
+        extra["removal"] = time_removal(wl, main_prec)
+        print(f"bench: removal {extra['removal']}", file=sys.stderr, flush=True)
     if args.snapshot:
         extra["snapshot"] = time_snapshot(wl, main_prec)
         print(f"bench: snapshot {extra['snapshot']}", file=sys.stderr, flush=True)
