@@ -282,3 +282,21 @@ def test_remove_at_full_size_invariants():
         plain = plain * n[k] + np.trunc((pos[:, k] - dom.min[k]) / dx[k]).astype(np.int64)
     assert np.array_equal(vox.astype(np.int64), plain)
     g.close()
+
+
+def test_remove_every_agent_of_one_instance():
+    """An instance of an ensemble emptied by a removal keeps stepping with no agents
+    while the others go on (work blocks and voxel lists without it)."""
+    wl = workloads.c4_ensemble_2d(n_instances=4, seed=42, steps=10, n_min=60, n_max=300)
+    g, o = pair(wl)
+    g.step(2)
+    o.step(2)
+    vox = g.get_voxels()[0]
+    nv0 = int(np.prod(wl.instances[0].domain.get_n_voxels()))
+    nv1 = int(np.prod(wl.instances[1].domain.get_n_voxels()))
+    rm = (vox >= nv0) & (vox < nv0 + nv1)  # every agent of instance 1
+    assert rm.sum() == int(wl.counts[1])
+    remove_both(g, o, rm)
+    g.step(6)
+    o.step(6)
+    assert_same(g, o, "empty instance")
