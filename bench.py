@@ -30,7 +30,8 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 # committed rocprofv3 summaries of the workloads that have one
-PROFILES = {"c2": "profiles/c2_cell_sorting_1M.md", "c3": "profiles/c3_rods_2d_200k.md"}
+PROFILES = {"c2": "profiles/c2_cell_sorting_1M.md", "c3": "profiles/c3_rods_2d_200k.md",
+            "c5": "profiles/c5_mie_3d_4M.md"}
 
 
 def candidates_per_agent(wl, start):
