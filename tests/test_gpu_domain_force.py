@@ -84,13 +84,10 @@ def test_points_spring_with_interaction(precision):
     o.close()
 
 
-@pytest.mark.parametrize("kind", ["spring", "gel"])
-@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("kind,dim", [("spring", 2), ("spring", 3), ("gel", 3)])
 def test_rods_domain_force(kind, dim):
     from test_gpu_rods import rods_case
 
-    if kind == "gel" and dim == 2:
-        pytest.skip("the gel force is defined for 3-D rods")
     spec, pos, vel, sp = rods_case(n=600, L=24.0 if dim == 3 else 50.0, nv=6, dim=dim, seed=32)
     L = spec[0].domain.max[0]
     force = (DomainSpring(strength=[0.03] * dim, center=[0.5 * L] * dim) if kind == "spring" else
