@@ -191,6 +191,8 @@ struct crm_ctx {
     uint32_t* d_dbefore = nullptr;  // [Vt + 1]
     uint64_t* d_vcount = nullptr;   // [Vt] Voxel::id_counter
     uint32_t* d_inst_div = nullptr; // [n_inst] dividing rods per instance
+    uint8_t* d_rm = nullptr;        // agent removal scratch (grow-only), rm_cap bytes
+    size_t rm_cap = 0;
     std::vector<BlockWork> work_host;  // the work table last uploaded
     void* d_scan_tmp = nullptr;
     size_t scan_tmp_bytes = 0;
@@ -226,7 +228,7 @@ void free_device(crm_ctx* ctx) {
     void* ptrs[] = {ctx->d_work, ctx->d_tiles, ctx->d_start, ctx->d_count, ctx->d_bsum, ctx->d_key, ctx->d_rank,
                     ctx->d_perm, ctx->d_force, ctx->d_rec64, ctx->d_recq, ctx->d_noise, ctx->d_recrod, ctx->d_bnd,
                     ctx->d_sense, ctx->d_snap, ctx->d_snap_sp, ctx->d_snap_rod, ctx->d_snap_id, ctx->d_snap_st, ctx->d_divflag, ctx->d_drank, ctx->d_dctr,
-                    ctx->d_ndiv, ctx->d_dbefore, ctx->d_vcount, ctx->d_scan_tmp, ctx->d_inst_div};
+                    ctx->d_ndiv, ctx->d_dbefore, ctx->d_vcount, ctx->d_scan_tmp, ctx->d_inst_div, ctx->d_rm};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& b : ctx->buf) {
@@ -258,6 +260,8 @@ void free_device(crm_ctx* ctx) {
     ctx->d_ndiv = ctx->d_dbefore = nullptr;
     ctx->d_vcount = nullptr;
     ctx->d_inst_div = nullptr;
+    ctx->d_rm = nullptr;
+    ctx->rm_cap = 0;
     ctx->d_scan_tmp = nullptr;
     ctx->scan_tmp_bytes = 0;
     ctx->cap_agents = 0;
@@ -717,20 +721,6 @@ crm_status dalloc(crm_ctx* ctx, T*& p, size_t n) {
     CRM_HIP(hipMalloc((void**)&p, std::max<size_t>(n, 1) * sizeof(T)));
     return CRM_OK;
 }
-
-// Device buffers of one call, freed on every return path.
-struct CallScratch {
-    std::vector<void*> ptrs;
-    ~CallScratch() {
-        for (void* p : ptrs) (void)hipFree(p);
-    }
-    template <typename T>
-    hipError_t get(T*& p, size_t n) {
-        const hipError_t e = hipMalloc((void**)&p, std::max<size_t>(n, 1) * sizeof(T));
-        if (e == hipSuccess) ptrs.push_back(p);
-        return e;
-    }
-};
 
 // Force tiles of every instance under tiling policy `pol`.
 void make_tiles(crm_ctx* ctx, uint32_t pol, std::vector<ForceTile>& tiles) {
@@ -1775,23 +1765,34 @@ crm_status crm_remove_agents(crm_ctx* ctx, const uint8_t* remove) {
     }
     const uint64_t n_new = n_old - n_rm;
     CRM_HIP(hipSetDevice(ctx->device));
-    CallScratch tmp;
+    // scratch, carved from one grow-only buffer at 256-byte aligned offsets: flags
+    // [n_old] bytes, four [n_old + 1] uint32 arrays, [n_inst] uint32, the scan's storage
+    const size_t n1 = n_old + 1;
+    size_t bytes = 0;
+    CRM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (uint32_t*)nullptr, (uint32_t*)nullptr, (int)n1,
+                                             ctx->stream));
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t o_keep_slot = al(n_old), o_keep_id = o_keep_slot + al(4 * n1), o_slot_before = o_keep_id + al(4 * n1),
+                 o_id_before = o_slot_before + al(4 * n1), o_inst = o_id_before + al(4 * n1),
+                 o_scan = o_inst + al(4 * (size_t)ctx->n_inst), need = o_scan + al(std::max<size_t>(bytes, 1));
+    if (need > ctx->rm_cap) {
+        if (ctx->d_rm) CRM_HIP(hipFree(ctx->d_rm));
+        ctx->d_rm = nullptr;
+        ctx->rm_cap = 0;
+        CRM_TRY(dalloc(ctx, ctx->d_rm, need));
+        ctx->rm_cap = need;
+    }
     RemoveArgs X{};
-    uint8_t* flag = nullptr;
-    CRM_HIP(tmp.get(flag, n_old));
-    CRM_HIP(tmp.get(X.keep_slot, n_old + 1));
-    CRM_HIP(tmp.get(X.keep_id, n_old + 1));
-    CRM_HIP(tmp.get(X.slot_before, n_old + 1));
-    CRM_HIP(tmp.get(X.id_before, n_old + 1));
-    CRM_HIP(tmp.get(X.inst_rm, ctx->n_inst));
+    uint8_t* flag = ctx->d_rm;
+    X.keep_slot = reinterpret_cast<uint32_t*>(ctx->d_rm + o_keep_slot);
+    X.keep_id = reinterpret_cast<uint32_t*>(ctx->d_rm + o_keep_id);
+    X.slot_before = reinterpret_cast<uint32_t*>(ctx->d_rm + o_slot_before);
+    X.id_before = reinterpret_cast<uint32_t*>(ctx->d_rm + o_id_before);
+    X.inst_rm = reinterpret_cast<uint32_t*>(ctx->d_rm + o_inst);
+    void* scan_tmp = ctx->d_rm + o_scan;
     X.flag = flag;
     X.n_old = n_old;
     X.n_inst = ctx->n_inst;
-    size_t bytes = 0;
-    CRM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, X.keep_slot, X.slot_before, (int)(n_old + 1),
-                                             ctx->stream));
-    uint8_t* scan_tmp = nullptr;
-    CRM_HIP(tmp.get(scan_tmp, bytes));
     CRM_HIP(hipMemcpyAsync(flag, remove, n_old, hipMemcpyHostToDevice, ctx->stream));
     CRM_HIP(hipMemsetAsync(X.inst_rm, 0, sizeof(uint32_t) * ctx->n_inst, ctx->stream));
     const StateBuf src = ctx->buf[ctx->cur], dst = ctx->buf[1 - ctx->cur];
