@@ -347,7 +347,9 @@ crm_status crm_get_cycle_state(crm_ctx* ctx, double* spring_length, double* grow
  * state, history and cycle state; agent indices are compacted in order (agent a
  * becomes the number of survivors before a), which also renumbers the Philox noise
  * streams and the rows of every by-agent array.  Removing every agent is
- * CRM_INVALID_ARGUMENT.  Points and rods, with or without the rods cycle. */
+ * CRM_INVALID_ARGUMENT.  Points and rods, with or without the rods cycle.  The
+ * context keeps the call's scratch memory (device allocation on the first call
+ * and when the agent count exceeds every earlier call's). */
 crm_status crm_remove_agents(crm_ctx* ctx, const uint8_t* remove);
 
 /* Neighbour sensing fused into the force pass.  Replaces the NeighborSensing
