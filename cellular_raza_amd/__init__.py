@@ -11,6 +11,7 @@ from .building_blocks import (  # noqa: F401
     Brownian3D,
     CartesianCuboid,
     CellSpecificInteraction,
+    CellSpecificInteractionLangevin,
     Langevin,
     Langevin2D,
     Langevin3D,

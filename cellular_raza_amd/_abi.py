@@ -38,6 +38,7 @@ POTENTIAL_BOUND_LENNARD_JONES = 2
 POTENTIAL_MORSE_SPECIES = 3
 POTENTIAL_CELL_SORTING = 4
 POTENTIAL_NONE = 5
+POTENTIAL_CELL_SORTING_LANGEVIN = 6
 
 # crm_mechanics_kind
 MECHANICS_NEWTON_DAMPED = 0
@@ -143,6 +144,8 @@ class Config(C.Structure):
         ("dt", C.c_double),
         ("device", C.c_int32),
         ("flags", C.c_uint32),
+        ("n_subdomains", C.c_uint32),
+        ("reserved", C.c_uint32),
     ]
 
 
@@ -201,6 +204,7 @@ SIGNATURES = {
     "crm_set_rod_cycle": (C.c_int, [P, C.POINTER(RodCycle)]),
     "crm_set_rod_spring_lengths": (C.c_int, [P, DP]),
     "crm_get_cycle_state": (C.c_int, [P, DP, DP, U64P, U64P, U32P]),
+    "crm_get_identifiers": (C.c_int, [P, U64P, U64P, U32P]),
     "crm_remove_agents": (C.c_int, [P, U8P]),
     "crm_save_cells_json": (C.c_int, [P, C.c_char_p, C.c_uint64, C.POINTER(JsonCells)]),
     "crm_get_timing": (C.c_int, [P, C.POINTER(Timing)]),

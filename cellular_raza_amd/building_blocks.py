@@ -202,6 +202,48 @@ class CellSpecificInteraction:
 
 
 @dataclass
+class CellSpecificInteractionLangevin:
+    """CellSpecificInteraction of the Langevin cell-sorting example
+    (cellular_raza-examples/cell_sorting/src/main-langevin.rs:35-92).
+
+    R = 2 cell_radius, r = |own - ext|, dir = (own - ext) / r (0 at r = 0):
+    force = repulsion (R - r) dir for r <= R, -a (r - R)(cutoff - r) / cutoff
+    exp(-(r / R - 1)^2) dir for R < r <= cutoff, else 0, where a is the species
+    pair's attraction; the interaction returns (-force, force).  Species 0 is
+    RedCell and 1 BlueCell (the example's enum order), so the attraction matrix is
+    [[attraction_rr, attraction_rb], [attraction_rb, attraction_bb]];
+    `attraction_matrix` (row = own species, up to 4 x 4) replaces it when given.
+    """
+
+    cell_radius: float
+    attraction_rr: float
+    attraction_bb: float
+    attraction_rb: float
+    repulsion: float
+    cutoff: float
+    attraction_matrix: Sequence[Sequence[float]] = None
+
+    def matrix(self):
+        if self.attraction_matrix is not None:
+            return [[float(x) for x in row] for row in self.attraction_matrix]
+        return [[self.attraction_rr, self.attraction_rb], [self.attraction_rb, self.attraction_bb]]
+
+    def fill(self, p: _abi.Potential) -> None:
+        p.kind = _abi.POTENTIAL_CELL_SORTING_LANGEVIN
+        p.radius = self.cell_radius
+        p.strength = self.repulsion
+        p.cutoff = self.cutoff
+        m = self.matrix()
+        n = len(m)
+        if n < 1 or n > _abi.CRM_MAX_SPECIES or any(len(row) != n for row in m):
+            raise ValueError("the attraction matrix must be square, 1x1 .. 4x4")
+        p.n_species = n
+        for a in range(n):
+            for b in range(n):
+                p.species_matrix[a * _abi.CRM_MAX_SPECIES + b] = float(m[a][b])
+
+
+@dataclass
 class NoInteraction:
     """NoInteraction (interaction.rs:12-35): the Mechanics-only aspect set."""
 

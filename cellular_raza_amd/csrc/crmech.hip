@@ -96,6 +96,38 @@ static crm_status host_grid(const crm_domain& d, uint32_t D, HostGrid& g, std::s
     return CRM_OK;
 }
 
+// get_decomp_res (cartesian_cuboid_n.rs:19-59): V = avg * n + (avg - 1) * m with
+// n + m = k chunks, searched the reference's way (avg from ceil(V / k) upwards,
+// trading full chunks for short ones).  False when no split is found.
+static bool decomp_res(uint64_t V, uint64_t k, uint64_t& n_out, uint64_t& avg_out) {
+    int64_t avg = (int64_t)std::ceil((double)V / (double)k);
+    int64_t n = (int64_t)k, m = 0;
+    for (uint64_t it = 0; it < k; it++) {
+        const int64_t rest = (int64_t)V - avg * n - (avg - 1) * m;
+        if (rest == 0) {
+            n_out = (uint64_t)n;
+            avg_out = (uint64_t)avg;
+            return true;
+        }
+        if (rest > 0) {
+            if (n == (int64_t)k) {
+                avg += 1;
+                m = 0;
+            }
+        } else {
+            n -= 1;
+            m += 1;
+        }
+    }
+    return false;
+}
+
+// Interactions whose parameters depend on the agents' species.
+static bool species_kind(uint32_t kind) {
+    return kind == CRM_POTENTIAL_MORSE_SPECIES || kind == CRM_POTENTIAL_CELL_SORTING ||
+           kind == CRM_POTENTIAL_CELL_SORTING_LANGEVIN;
+}
+
 static int exact_int_exponent(double e) {
     if (e >= 0.0 && e <= 64.0 && std::floor(e) == e) return (int)e;
     return -1;
@@ -206,6 +238,9 @@ struct crm_ctx {
 
     uint64_t N = 0;
     std::vector<uint64_t> counts;
+    // point agents: CellIdentifier and parent by agent index, (kind, a, b) each (rods
+    // keep theirs on the device); renumbered with the agent indices on removal
+    std::vector<uint64_t> pt_ident, pt_parent;
     bool have_agents = false;
     bool halted = false;
     uint64_t step = 0;
@@ -372,6 +407,8 @@ crm_status launch_force_d(crm_ctx* ctx, StateBuf S) {
         case CRM_POTENTIAL_MORSE_SPECIES: return launch_force_dp<D, CRM_POTENTIAL_MORSE_SPECIES>(ctx, S);
         case CRM_POTENTIAL_CELL_SORTING: return launch_force_dp<D, CRM_POTENTIAL_CELL_SORTING>(ctx, S);
         case CRM_POTENTIAL_NONE: return launch_force_dp<D, CRM_POTENTIAL_NONE>(ctx, S);
+        case CRM_POTENTIAL_CELL_SORTING_LANGEVIN:
+            return launch_force_dp<D, CRM_POTENTIAL_CELL_SORTING_LANGEVIN>(ctx, S);
     }
     return CRM_INVALID_ARGUMENT;
 }
@@ -1069,15 +1106,13 @@ std::string ident_json(const uint64_t* x) {
 // One data entry of the batch file: CombinedSaveFormat { identifier, element:
 // (CellBox { identifier, parent, cell }, AuxStorage { mechanics }) }
 // (storage/concepts.rs:176-190, cell.rs:128-137, aux_storage.rs:41-46).
-// rods: ids [6] (identifier, parent), rv [2] (spring length, growth rate) and the
-// history entries the rod holds (the newest `held` of the n_hist slots); points:
-// null, Initial(agent index), every entry
-void put_cell(std::string& o, uint64_t a, const double* r, const Template& T, uint32_t nv, uint32_t D,
-              uint32_t n_hist, const uint64_t* ids, const double* rv, uint32_t held) {
+// ids [6] (identifier, parent); rods: rv [2] (spring length, growth rate); the
+// history entries the agent holds are the newest `held` of the n_hist slots.
+void put_cell(std::string& o, const double* r, const Template& T, uint32_t nv, uint32_t D, uint32_t n_hist,
+              const uint64_t* ids, const double* rv, uint32_t held) {
     const uint32_t K = nv * D;
     const bool matrix = nv > 1;
-    const uint64_t initial[3] = {0, a, 0}, none[3] = {3, 0, 0};
-    const std::string ident = ident_json(ids ? ids : initial), parent = ident_json(ids ? ids + 3 : none);
+    const std::string ident = ident_json(ids), parent = ident_json(ids + 3);
     o += "    {\n      \"identifier\": " + ident + ",\n      \"element\": [\n        {\n          \"identifier\": " +
          ident + ",\n          \"parent\": " + parent + ",\n          \"cell\": ";
     for (size_t i = 0; i < T.hole.size(); i++) {
@@ -1088,7 +1123,7 @@ void put_cell(std::string& o, uint64_t a, const double* r, const Template& T, ui
     }
     o += T.lit.back();
     o += "\n        },\n        {\n          \"mechanics\": {\n            \"positions\": [";
-    const uint32_t first = ids ? n_hist - std::min(held, n_hist) : 0u;
+    const uint32_t first = n_hist - std::min(held, n_hist);
     for (uint32_t i = first; i < n_hist; i++) {
         if (i > first) o += ", ";
         put_vec(o, r + 2u * K + i * K, nv, D, matrix);
@@ -1144,21 +1179,24 @@ crm_status crm_create(const crm_config* cfg, const crm_instance* instances, uint
         cfg->history_order > CRM_HISTORY_NEWEST_FIRST || cfg->noise_mode > CRM_NOISE_PHILOX)
         return fail(CRM_INVALID_ARGUMENT, "unknown precision / weighting / history order / noise mode");
     if (!(cfg->dt > 0.0)) return fail(CRM_INVALID_ARGUMENT, "dt must be positive");
+    if (cfg->n_subdomains > 65535) return fail(CRM_INVALID_ARGUMENT, "n_subdomains must be below 65536");
     ctx->D = cfg->dim;
     ctx->n_inst = n_instances;
     ctx->inst.assign(instances, instances + n_instances);
     ctx->pot_kind = instances[0].potential.kind;
     ctx->mech_kind = instances[0].mechanics.kind;
-    if (ctx->pot_kind > CRM_POTENTIAL_NONE) return fail(CRM_INVALID_ARGUMENT, "unknown potential kind");
+    if (ctx->pot_kind > CRM_POTENTIAL_CELL_SORTING_LANGEVIN) return fail(CRM_INVALID_ARGUMENT, "unknown potential kind");
     if (ctx->mech_kind > CRM_MECHANICS_ROD) return fail(CRM_INVALID_ARGUMENT, "unknown mechanics kind");
     ctx->nv = 1;
     if (ctx->mech_kind == CRM_MECHANICS_ROD) {
         ctx->nv = instances[0].mechanics.n_vertices;
         if (ctx->nv < 2 || ctx->nv > CRM_MAX_ROD_VERTICES)
             return fail(CRM_INVALID_ARGUMENT, "rods need 2..16 vertices");
-        if (ctx->pot_kind == CRM_POTENTIAL_CELL_SORTING)
+        if (ctx->pot_kind == CRM_POTENTIAL_CELL_SORTING || ctx->pot_kind == CRM_POTENTIAL_CELL_SORTING_LANGEVIN)
             return fail(CRM_INVALID_ARGUMENT, "rods take a point potential (Morse, Mie, BoundLennardJones)");
         ctx->rods_per_block = kBlock / ctx->nv;
+        if (cfg->n_subdomains > 1 && cfg->weighting == CRM_WEIGHT_REFERENCE)
+            return fail(CRM_INVALID_ARGUMENT, "chili subdomain weighting (n_subdomains > 1) is for point agents");
     }
     ctx->K = ctx->nv * ctx->D;
     uint64_t voff = 0;
@@ -1167,8 +1205,7 @@ crm_status crm_create(const crm_config* cfg, const crm_instance* instances, uint
         if (I.potential.kind != ctx->pot_kind || I.mechanics.kind != ctx->mech_kind ||
             (ctx->mech_kind == CRM_MECHANICS_ROD && I.mechanics.n_vertices != ctx->nv))
             return fail(CRM_INVALID_ARGUMENT, "all instances must share potential and mechanics kinds");
-        if ((I.potential.kind == CRM_POTENTIAL_MORSE_SPECIES || I.potential.kind == CRM_POTENTIAL_CELL_SORTING) &&
-            (I.potential.n_species == 0 || I.potential.n_species > CRM_MAX_SPECIES))
+        if (species_kind(I.potential.kind) && (I.potential.n_species == 0 || I.potential.n_species > CRM_MAX_SPECIES))
             return fail(CRM_INVALID_ARGUMENT, "n_species must be 1..4");
         HostGrid g;
         crm_status st = host_grid(I.domain, ctx->D, g, ctx->err);
@@ -1187,6 +1224,19 @@ crm_status crm_create(const crm_config* cfg, const crm_instance* instances, uint
         }
         P.voxel_offset = (uint32_t)voff;
         P.n_voxels = (uint32_t)g.V;
+        if (cfg->n_subdomains > 1 && cfg->weighting == CRM_WEIGHT_REFERENCE) {
+            // create_subdomains (cartesian_cuboid_n.rs:493-584): lexicographic chunks
+            uint64_t nf = 0, avg = 0;
+            if (g.V < cfg->n_subdomains || !decomp_res(g.V, cfg->n_subdomains, nf, avg))
+                return fail(CRM_SETUP_ERROR, "DecomposeError: Could not find a suiting decomposition of " +
+                                                 std::to_string(g.V) + " voxels into " +
+                                                 std::to_string(cfg->n_subdomains) + " subdomains");
+            P.sub_on = 1;
+            P.sub_avg = (uint32_t)avg;
+            P.sub_avg1 = (uint32_t)std::max<uint64_t>(avg - 1, 1);
+            P.sub_switch = (uint32_t)(nf * avg);
+            P.sub_n = (uint32_t)nf;
+        }
         fill_pot<double>(P.pd, I.potential);
         fill_pot<float>(P.pf, I.potential);
         const crm_mechanics& M = I.mechanics;
@@ -1203,7 +1253,7 @@ crm_status crm_create(const crm_config* cfg, const crm_instance* instances, uint
         P.brownian_coeff = std::sqrt(2.0 * M.diffusion_constant);
         ctx->hparams.push_back(P);
         voff += g.V;
-        if (I.potential.kind == CRM_POTENTIAL_MORSE_SPECIES) {
+        if (I.potential.kind == CRM_POTENTIAL_MORSE_SPECIES || I.potential.kind == CRM_POTENTIAL_CELL_SORTING_LANGEVIN) {
             for (uint32_t a = 0; a < I.potential.n_species; a++)
                 for (uint32_t b = 0; b < I.potential.n_species; b++)
                     if (I.potential.species_matrix[a * CRM_MAX_SPECIES + b] !=
@@ -1244,7 +1294,7 @@ crm_status crm_set_agents(crm_ctx* ctx, const uint64_t* counts, const double* po
         return CRM_INVALID_ARGUMENT;
     }
     const uint32_t nsp = ctx->inst[0].potential.n_species;
-    if (species && (ctx->pot_kind == CRM_POTENTIAL_MORSE_SPECIES || ctx->pot_kind == CRM_POTENTIAL_CELL_SORTING))
+    if (species && species_kind(ctx->pot_kind))
         for (uint64_t a = 0; a < N; a++)
             if (species[a] < 0 || (uint32_t)species[a] >= nsp) {
                 ctx->err = "species out of range (agent " + std::to_string(a) + ")";
@@ -1379,6 +1429,18 @@ crm_status crm_set_agents(crm_ctx* ctx, const uint64_t* counts, const double* po
                                    ctx->stream));
         }
         CRM_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    ctx->pt_ident.clear();
+    ctx->pt_parent.clear();
+    if (!is_rod(ctx)) {  // CellIdentifier::Initial(index among the instance's cells), no parent
+        ctx->pt_ident.assign(3 * N, 0);
+        ctx->pt_parent.assign(3 * N, 0);
+        uint64_t a = 0;
+        for (uint32_t i = 0; i < ctx->n_inst; i++)
+            for (uint64_t j = 0; j < counts[i]; j++, a++) {
+                ctx->pt_ident[3 * a + 1] = j;
+                ctx->pt_parent[3 * a] = 3;
+            }
     }
     CRM_TRY(reset_control(ctx, 0));
     Arrays A = arrays(ctx);
@@ -1748,6 +1810,19 @@ crm_status crm_get_cycle_state(crm_ctx* ctx, double* spring_length, double* grow
     return CRM_OK;
 }
 
+crm_status crm_get_identifiers(crm_ctx* ctx, uint64_t* identifier, uint64_t* parent, uint32_t* history_held) {
+    if (!ctx || !ctx->have_agents) return CRM_INVALID_ARGUMENT;
+    if (is_rod(ctx)) return crm_get_cycle_state(ctx, nullptr, nullptr, identifier, parent, history_held);
+    const uint64_t N = ctx->N;
+    if (identifier) std::copy(ctx->pt_ident.begin(), ctx->pt_ident.end(), identifier);
+    if (parent) std::copy(ctx->pt_parent.begin(), ctx->pt_parent.end(), parent);
+    if (history_held) {
+        const uint32_t held = (uint32_t)std::min<uint64_t>(ctx->step, ctx->cfg.solver_order);
+        std::fill(history_held, history_held + N, held);
+    }
+    return CRM_OK;
+}
+
 crm_status crm_remove_agents(crm_ctx* ctx, const uint8_t* remove) {
     if (!ctx || !remove) return CRM_INVALID_ARGUMENT;
     if (!ctx->have_agents) {
@@ -1821,6 +1896,19 @@ crm_status crm_remove_agents(crm_ctx* ctx, const uint8_t* remove) {
         return CRM_DEVICE_ERROR;
     }
     for (uint32_t i = 0; i < ctx->n_inst; i++) ctx->counts[i] -= inst_rm[i];
+    if (!is_rod(ctx)) {  // identifiers follow the agent indices, in order
+        uint64_t m = 0;
+        for (uint64_t a = 0; a < n_old; a++) {
+            if (remove[a]) continue;
+            for (int q = 0; q < 3; q++) {
+                ctx->pt_ident[3 * m + q] = ctx->pt_ident[3 * a + q];
+                ctx->pt_parent[3 * m + q] = ctx->pt_parent[3 * a + q];
+            }
+            m++;
+        }
+        ctx->pt_ident.resize(3 * n_new);
+        ctx->pt_parent.resize(3 * n_new);
+    }
     ctx->N = n_new;
     ctx->cur = 1 - ctx->cur;
     CRM_TRY(rebuild_work(ctx));
@@ -1932,7 +2020,15 @@ crm_status crm_save_cells_json(crm_ctx* ctx, const char* path, uint64_t iteratio
     std::vector<double> rodv;
     std::vector<uint64_t> ids;
     std::vector<uint32_t> held;
-    if (is_rod(ctx)) {
+    if (!is_rod(ctx)) {  // identifiers from the host copy; every agent holds the whole ring
+        ids.resize(6 * N);
+        held.assign(N, nh);
+        for (uint64_t a = 0; a < N; a++)
+            for (int q = 0; q < 3; q++) {
+                ids[6 * a + q] = ctx->pt_ident[3 * a + q];
+                ids[6 * a + 3 + q] = ctx->pt_parent[3 * a + q];
+            }
+    } else {
         rodv.resize(2 * N);
         ids.resize(6 * N);
         held.resize(N);
@@ -1982,9 +2078,8 @@ crm_status crm_save_cells_json(crm_ctx* ctx, const char* path, uint64_t iteratio
         o.reserve((a1 - a0) * (400 + 60 * rec_len));
         for (uint64_t a = a0; a < a1; a++) {
             if (a) o += ",\n";
-            const bool rod = !ids.empty();
-            put_cell(o, a, rec.data() + a * rec_len, T[spec->n_templates > 1 ? sp[a] : 0], nv, D, nh,
-                     rod ? ids.data() + 6 * a : nullptr, rod ? rodv.data() + 2 * a : nullptr, rod ? held[a] : nh);
+            put_cell(o, rec.data() + a * rec_len, T[spec->n_templates > 1 ? sp[a] : 0], nv, D, nh,
+                     ids.data() + 6 * a, rodv.empty() ? nullptr : rodv.data() + 2 * a, held[a]);
         }
     };
     std::vector<std::thread> pool;

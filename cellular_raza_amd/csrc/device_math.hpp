@@ -223,6 +223,55 @@ __device__ __forceinline__ void cell_sorting(const T* z, bool same, const PotPar
     }
 }
 
+// CellSpecificInteraction of examples/cell_sorting/src/main-langevin.rs:44-92.
+// R = R_own + R_ext, r = |z|, dir = z / r (zero at r == 0):
+//   r <= R:       force = (repulsion * (R - r)) * dir
+//   r <= cutoff:  force = ((((-a) * (r - R)) * (cutoff - r)) / cutoff) * exp(-1 * (r / R - 1)^2) * dir
+//   otherwise     0
+// with a = attraction of the (own, ext) species pair; it returns (-force, force),
+// so the force on own is -force.  Continuous at R and at the cutoff.  The float
+// path takes r and 1/r from one v_rsq_f32 and the exponential from v_exp_f32.
+template <typename T, int D>
+__device__ __forceinline__ void sorting_langevin(const T* z, T attraction, const PotParams<T>& P, int cut, T* f) {
+    T d2 = T(0), r, inv;
+    if constexpr (std::is_same<T, float>::value) {
+        const FastDist q = fast_dist<D>(z);
+        d2 = q.d2;
+        r = q.dist;
+        inv = q.inv;
+    } else {
+#pragma unroll
+        for (int k = 0; k < D; k++) d2 = d2 + z[k] * z[k];
+        r = dsqrt<T>(d2);
+        inv = T(0);
+    }
+    if (d2 == T(0)) {  // dir = 0
+#pragma unroll
+        for (int k = 0; k < D; k++) f[k] = T(0);
+        return;
+    }
+    const T R = P.two_radius;
+    T s;
+    if (r <= R) {
+        s = P.strength * (R - r);
+    } else if (cut < 0 ? (r <= P.cutoff) : (cut == 1)) {
+        const T sm1 = r / R - T(1);
+        T e;
+        if constexpr (std::is_same<T, float>::value) e = __expf(-(sm1 * sm1));
+        else e = dexp<T>(T(-1) * (sm1 * sm1));
+        s = ((((-attraction) * (r - R)) * (P.cutoff - r)) / P.cutoff) * e;
+    } else {
+#pragma unroll
+        for (int k = 0; k < D; k++) f[k] = T(0);
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+        if constexpr (std::is_same<T, float>::value) f[k] = -(s * (z[k] * inv));
+        else f[k] = -(s * (z[k] / r));
+    }
+}
+
 // Radius at which the potential is discontinuous (the cutoff test).
 template <int POT, typename T>
 __device__ __forceinline__ T cut_radius(const PotParams<T>& P) {
@@ -245,7 +294,8 @@ __device__ __forceinline__ int cut_decision(const PotParams<double>& P, double r
 
 // One Interaction::calculate_force_between evaluation.  `strength` is the
 // pair's interaction strength (Morse: the potential's strength, or that times
-// the species-matrix entry, computed by the caller), `same` tells the
+// the species-matrix entry; the Langevin sorting interaction: the species pair's
+// attraction; computed by the caller), `same` tells the
 // cell-sorting interaction whether both agents are of one species.
 template <int POT, typename T, int D>
 __device__ __forceinline__ bool point_force(const T* z, bool same, T strength, const PotParams<T>& P, int cut,
@@ -261,6 +311,9 @@ __device__ __forceinline__ bool point_force(const T* z, bool same, T strength, c
     } else if constexpr (POT == CRM_POTENTIAL_NONE) {
 #pragma unroll
         for (int k = 0; k < D; k++) f[k] = T(0);
+        return false;
+    } else if constexpr (POT == CRM_POTENTIAL_CELL_SORTING_LANGEVIN) {
+        sorting_langevin<T, D>(z, strength, P, cut, f);
         return false;
     } else {
         cell_sorting<T, D>(z, same, P, cut, f);

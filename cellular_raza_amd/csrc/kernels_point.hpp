@@ -206,6 +206,8 @@ struct LdsSource {
     uint32_t np, n_vox;
     const uint32_t* start;
     uint32_t voff, n1, n2;
+    const uint16_t* sub;  // chili subdomain of each staged record's voxel (when the instance has several)
+    __device__ __forceinline__ uint32_t sub_of(uint32_t j, const W&, const InstParams&) const { return sub[j]; }
     // staged voxel index of (row rb, column c, plane pa)
     __device__ __forceinline__ uint32_t vox(uint32_t rb, uint32_t c, uint32_t pa) const {
         return (rb * G.NC + (c - G.cl)) * np + pa;
@@ -268,6 +270,9 @@ struct GlobalSource {
             return rec[j];
     }
     __device__ __forceinline__ uint32_t slot(uint32_t j, const W&) const { return j; }
+    __device__ __forceinline__ uint32_t sub_of(uint32_t, const W& w, const InstParams& P) const {
+        return subdomain_of(P, w.gvox() - w.voff);
+    }
 };
 
 // Strength row of the species matrix in four scalars (a register array indexed
@@ -365,8 +370,13 @@ __device__ __forceinline__ void agent_force(const Arrays& A, const StateBuf& S, 
         }
     }
     const bool wider = MIXED ? mark_f > thr_f : mark_d > thr_d;
+    // chili with several subdomains: pairs across a subdomain border get both halves
+    // (update_mechanics.rs:285-382)
+    const bool subs = P.sub_on != 0;
+    const uint32_t my_sub = subs ? subdomain_of(P, S.vox[s] - P.voxel_offset) : 0u;
 
-    // species-matrix row (own species) and column (own species as ext)
+    // species-matrix row (own species) and column (own species as ext): Morse
+    // strengths, or the Langevin sorting interaction's attractions
     Row4<T> srow, scol;
     {
         T r4[CRM_MAX_SPECIES], c4[CRM_MAX_SPECIES];
@@ -375,6 +385,9 @@ __device__ __forceinline__ void agent_force(const Arrays& A, const StateBuf& S, 
             if constexpr (POT == CRM_POTENTIAL_MORSE_SPECIES) {
                 r4[k] = PP.strength * PP.species_matrix[sp_i * CRM_MAX_SPECIES + k];
                 c4[k] = PP.strength * PP.species_matrix[k * CRM_MAX_SPECIES + sp_i];
+            } else if constexpr (POT == CRM_POTENTIAL_CELL_SORTING_LANGEVIN) {  // attraction
+                r4[k] = PP.species_matrix[sp_i * CRM_MAX_SPECIES + k];
+                c4[k] = PP.species_matrix[k * CRM_MAX_SPECIES + sp_i];
             } else {
                 r4[k] = c4[k] = PP.strength;
             }
@@ -472,7 +485,7 @@ __device__ __forceinline__ void agent_force(const Arrays& A, const StateBuf& S, 
                 if (!fmark) return;
             }
         }
-        const bool full = physical || (j >= ov.b && j < ov.e);
+        const bool full = physical || (j >= ov.b && j < ov.e) || (subs && src.sub_of(j, w, P) != my_sub);
         int sp_j;
         int cut = -1;
         if constexpr (MIXED) {
@@ -637,6 +650,7 @@ __global__ __launch_bounds__(kBlock, MIXED ? 5 : 4) void k_force_tiled(Arrays A,
     __shared__ uint32_t own_gs[kOwnVoxMax];          // global slot of own voxel u's first agent
     __shared__ uint8_t ord[kBlock];                  // own agents in lane order
     __shared__ uint32_t bins[kSortBins];
+    __shared__ uint16_t lsub[CAP];                   // chili subdomain of each staged record (sub_on only)
     const Rec* __restrict__ rec = rec_array(A, std::integral_constant<bool, MIXED>());
     const ForceTile T = tiles[xcd_block(blockIdx.x, gridDim.x)];
     const InstParams& P = A.ip[T.inst];
@@ -699,6 +713,11 @@ __global__ __launch_bounds__(kBlock, MIXED ? 5 : 4) void k_force_tiled(Arrays A,
                         if constexpr (MIXED) lds[run + k] = to_region<D>(r, (int)pa, (int)rb, G);
                         else lds[run + k] = r;
                     }
+                    if (P.sub_on && cnt[q]) {
+                        const int a = (int)T.a0 + (int)pa - (int)PA_OWN, b = (int)T.b0 - 1 + (int)rb;
+                        const uint16_t o = (uint16_t)subdomain_of(P, ((uint32_t)a * n1 + (uint32_t)b) * n2 + c);
+                        for (uint32_t k = 0; k < cnt[q]; k++) lsub[run + k] = o;
+                    }
                     // own voxel u = r * W + (c - c0): its count and first global slot
                     if (pa == PA_OWN && rb >= 1 && rb <= R && c >= T.c0 && c < T.c1) {
                         const uint32_t u = (rb - 1) * W + (c - T.c0);
@@ -722,7 +741,7 @@ __global__ __launch_bounds__(kBlock, MIXED ? 5 : 4) void k_force_tiled(Arrays A,
             }
             __syncthreads();
             const uint32_t n_own = own_pre[n_ownv];
-            LdsSource<D, MIXED> src{lds, vst, G, NP, n_vox, A.start, voff, n1, n2};
+            LdsSource<D, MIXED> src{lds, vst, G, NP, n_vox, A.start, voff, n1, n2, lsub};
             const float inv_w = 1.0f / (float)W;
             // own agent t: own voxel u (largest u with own_pre[u] <= t), LDS index,
             // global slot, and the three ranges of its stream

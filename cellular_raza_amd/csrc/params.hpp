@@ -6,6 +6,7 @@
 // plain = (i0 * n1 + i1) * n2 + i2, and a 2-D point (x, y) sits on axes 1, 2.
 #pragma once
 
+#include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include "crmech.h"
@@ -48,7 +49,17 @@ struct InstParams {
     uint32_t dpad;
     double dk[3], dc[3];    // spring: strength and centre per axis, 3-D layout
     double gel, fric, fric_d;  // rods gel: pressure, surface friction, friction distance
+    // chili subdomains (create_subdomains with get_decomp_res(V, k) = (n, m, avg)):
+    // plain index p belongs to chunk p / avg below sub_switch = n * avg, else to
+    // (p - sub_switch) / sub_avg1 + n with sub_avg1 = max(avg - 1, 1).  sub_on = 0:
+    // one subdomain.
+    uint32_t sub_on, sub_avg, sub_avg1, sub_switch, sub_n, sub_pad;
 };
+
+// Subdomain (chili thread) owning the voxel with plain index p of an instance.
+__host__ __device__ __forceinline__ uint32_t subdomain_of(const InstParams& P, uint32_t p) {
+    return p < P.sub_switch ? p / P.sub_avg : (p - P.sub_switch) / P.sub_avg1 + P.sub_n;
+}
 
 // Work unit of the agent-parallel kernels: a block of consecutive slots inside
 // one instance, so per-instance parameters are wave-uniform.

@@ -59,7 +59,11 @@ class MechanicsBackend:
         timing: bool = False,
         lds: bool = True,
         autotune=False,
+        n_subdomains: int = 1,
     ):
+        """n_subdomains: chili's thread count (Settings::n_threads) for the reference
+        weighting -- pairs across the borders of its lexicographic voxel chunks get the
+        full force (point agents)."""
         self._lib = load()
         if not instances:
             raise ValueError("at least one instance")
@@ -79,6 +83,7 @@ class MechanicsBackend:
         cfg.device = device
         cfg.flags = ((_abi.FLAG_GRAPH if graph else 0) | (_abi.FLAG_TIMING if timing else 0)
                      | (0 if lds else _abi.FLAG_NO_LDS))
+        cfg.n_subdomains = int(n_subdomains)
         arr = (_abi.Instance * len(instances))()
         for i, spec in enumerate(instances):
             if spec.domain.dim != dim:
@@ -201,6 +206,18 @@ class MechanicsBackend:
                                                   held.ctypes.data_as(_abi.U32P)))
         return {"spring_length": sl, "growth_rate": gr, "identifier": ident, "parent": parent,
                 "history_held": held}
+
+    def identifiers(self) -> Dict[str, np.ndarray]:
+        """CellIdentifier and parent of every agent as (kind, a, b) rows (kind 0
+        Initial(a), 1 Division(a = voxel, b = counter), 3 no parent) and the history
+        entries each holds (crm_get_identifiers); they survive removals."""
+        n = self.n_agents
+        ident, parent = np.empty((n, 3), dtype=np.uint64), np.empty((n, 3), dtype=np.uint64)
+        held = np.empty(n, dtype=np.uint32)
+        self._check(self._lib.crm_get_identifiers(self._h, ident.ctypes.data_as(_abi.U64P),
+                                                  parent.ctypes.data_as(_abi.U64P),
+                                                  held.ctypes.data_as(_abi.U32P)))
+        return {"identifier": ident, "parent": parent, "history_held": held}
 
     def remove_agents(self, remove) -> None:
         """CycleEvent::Remove for the agents with remove[a] true (a boolean per
@@ -342,8 +359,10 @@ class FixedStepsize:
 
 @dataclass
 class Settings:
-    """chili::Settings (cellular_raza-core/src/backend/chili/setup.rs:17-27); n_threads is
-    accepted for signature parity, the GPU runs the single-subdomain semantics."""
+    """chili::Settings (cellular_raza-core/src/backend/chili/setup.rs:17-27).  n_threads
+    is chili's subdomain count; for point agents it selects the same pair weighting
+    (pairs across the borders of the n_threads lexicographic voxel chunks get the full
+    force), for rods the single-subdomain weighting is used."""
 
     time: FixedStepsize
     n_threads: int = 1
@@ -377,11 +396,14 @@ def run_simulation(
     the agents' Cycle returning CycleEvent::Remove in that step: the agents of a
     returned boolean mask (by agent index) are removed (MechanicsBackend.remove_agents,
     update_cycle.rs:41-55) before the next step or save point."""
+    kw = dict(backend_kwargs)
+    if getattr(mechanics, "n_vertices", 1) == 1 and kw.get("weighting", "reference") == "reference":
+        kw.setdefault("n_subdomains", settings.n_threads)
     be = MechanicsBackend(
         [InstanceSpec(domain, interaction, mechanics, domain_force)],
         dt=settings.time.dt,
         mechanics_solver_order=mechanics_solver_order,
-        **backend_kwargs,
+        **kw,
     )
     try:
         if cycle is not None:  # the aspects Cycle + NeighborSensing of the rods example
@@ -446,15 +468,14 @@ def _store_cells(be: MechanicsBackend, cells, iteration: int, templates: Sequenc
 
         zero = vec(np.zeros(be.K))
         cyc = be.cycle_state() if matrix else None
+        ids = be.identifiers()
         n_hist = snap["dx"].shape[1]
-        held = np.full(be.n_agents, n_hist)
-        if cyc is not None:  # rods born later hold fewer history entries
-            held = np.minimum(cyc["history_held"].astype(np.int64), n_hist)
+        held = np.minimum(ids["history_held"].astype(np.int64), n_hist)  # agents born later hold fewer
         pairs = []
         for a in range(be.n_agents):
             t = templates[int(snap["species"][a]) if len(templates) > 1 else 0]
-            ident = {"Initial": a} if cyc is None else st.identifier_json(cyc["identifier"][a])
-            parent = None if cyc is None else st.identifier_json(cyc["parent"][a])
+            ident = st.identifier_json(ids["identifier"][a])
+            parent = st.identifier_json(ids["parent"][a])
             sl = 0.0 if cyc is None else cyc["spring_length"][a]
             gr = 0.0 if cyc is None else cyc["growth_rate"][a]
             cellbox = {"identifier": ident, "parent": parent,

@@ -374,6 +374,12 @@ def interaction_fields(i, species: int, species_names: Sequence[str]):
         return {"species": species_names[species], "cell_radius": i.cell_radius,
                 "potential_strength": i.potential_strength,
                 "relative_interaction_range": i.relative_interaction_range}
+    if isinstance(i, bb.CellSpecificInteractionLangevin):  # main-langevin.rs:35-43
+        m = i.matrix()
+        rb = m[0][1] if len(m) > 1 else i.attraction_rb
+        bb_ = m[1][1] if len(m) > 1 else i.attraction_bb
+        return {"species": species_names[species], "cell_radius": i.cell_radius, "attraction_rr": m[0][0],
+                "attraction_bb": bb_, "attraction_rb": rb, "repulsion": i.repulsion, "cutoff": i.cutoff}
     if isinstance(i, bb.SpeciesMorsePotential):  # composite of this library: Morse fields + species index
         return {"radius": i.radius, "potential_stiffness": i.potential_stiffness, "cutoff": i.cutoff,
                 "strength": i.strength, "species": species}

@@ -17,6 +17,7 @@ import numpy as np
 
 from .building_blocks import (
     CartesianCuboid,
+    CellSpecificInteractionLangevin,
     Langevin,
     MiePotential,
     MorsePotential,
@@ -167,6 +168,28 @@ def bacterial_rods_example(seed: int = 5, steps: int = 15000, n_rods: int = 5,
     wl = Workload("bacterial rods example (Cycle + NeighborSensing)", [InstanceSpec(dom, pot, mech)],
                   0.1 * minute, steps, 2, pos, np.zeros_like(pos), None, [n_rods])
     return RodsExample(wl, spring, cycle)
+
+
+def langevin_sorting_example(seed: int = 6, steps: int = 50_000, n_blue: int = 600, n_red: int = 1000,
+                             domain_size: float = 200.0) -> Workload:
+    """cellular_raza-examples/cell_sorting/src/main-langevin.rs:9-146: 1600 cells
+    (indices 0..n_blue BlueCell -- the example's `n <= N_CELLS_BLUE` makes n_blue + 1
+    of them -- the rest RedCell) uniform in [0, 200)^3, Langevin3D (mass 10, damping 2,
+    kT 1), CellSpecificInteraction (cell radius 6, attraction rr 1.5, bb 0.075, rb
+    0.015, repulsion 5, cutoff 20), domain interaction range 2 x cutoff, dt 0.25,
+    50000 steps.  Species 0 = RedCell, 1 = BlueCell.  The example seeds ChaCha8;
+    this draws the same distributions from numpy's generator; the solver order is the
+    run_simulation! default (AB3)."""
+    n = n_blue + n_red
+    dom = CartesianCuboid.from_boundaries_and_interaction_range([0.0] * 3, [domain_size] * 3, 2.0 * 20.0)
+    pot = CellSpecificInteractionLangevin(cell_radius=6.0, attraction_rr=1.5, attraction_bb=0.05 * 1.5,
+                                          attraction_rb=0.01 * 1.5, repulsion=5.0, cutoff=20.0)
+    mech = Langevin(mass=10.0, damping=2.0, kb_temperature=1.0)
+    rng = np.random.default_rng(seed)
+    pos = _uniform_open(rng, n, 3, domain_size)
+    species = np.where(np.arange(n) <= n_blue, 1, 0).astype(np.int32)
+    return Workload("Langevin cell sorting example", [InstanceSpec(dom, pot, mech)], 0.25, steps, 2, pos,
+                    np.zeros_like(pos), species, [n], noise="philox")
 
 
 def c4_ensemble_2d(n_instances: int = 512, seed: int = 4, steps: int = 1000, n_min: int = 500,

@@ -66,7 +66,11 @@ typedef enum crm_potential_kind {
     CRM_POTENTIAL_CELL_SORTING = 4,       /* CellSpecificInteraction of
                                              cellular_raza-benchmarks/src/cell_sorting.rs:31-94 (sign = +1)
                                              and cellular_raza-examples/cell_sorting/src/main.rs:32-103 (sign = -1) */
-    CRM_POTENTIAL_NONE = 5                /* NoInteraction, interaction.rs:12-35 (Mechanics-only runs) */
+    CRM_POTENTIAL_NONE = 5,               /* NoInteraction, interaction.rs:12-35 (Mechanics-only runs) */
+    CRM_POTENTIAL_CELL_SORTING_LANGEVIN = 6 /* CellSpecificInteraction of
+                                             cellular_raza-examples/cell_sorting/src/main-langevin.rs:44-92:
+                                             repulsion (strength) * (R_o + R_e - r) in contact, Gaussian-tail
+                                             attraction species_matrix[own][ext] up to cutoff, (-force, force) */
 } crm_potential_kind;
 
 /* Mechanics kinds (cellular_raza-building-blocks/src/cell_building_blocks/mechanics.rs,
@@ -123,17 +127,21 @@ typedef struct crm_domain {
 
 typedef struct crm_potential {
     uint32_t kind;                    /* crm_potential_kind */
-    uint32_t n_species;               /* MORSE_SPECIES / CELL_SORTING: species are 0..n_species-1 */
+    uint32_t n_species;               /* MORSE_SPECIES / CELL_SORTING / CELL_SORTING_LANGEVIN: species are
+                                         0..n_species-1 */
     double radius;                    /* Morse / Mie radius; CellSpecificInteraction cell_radius */
     double potential_stiffness;       /* Morse lambda */
-    double cutoff;                    /* Morse / Mie / BoundLennardJones cutoff */
-    double strength;                  /* Morse V0, Mie epsilon ("strength"), CellSorting potential_strength */
+    double cutoff;                    /* Morse / Mie / BoundLennardJones / CellSortingLangevin cutoff */
+    double strength;                  /* Morse V0, Mie epsilon ("strength"), CellSorting potential_strength,
+                                         CellSortingLangevin repulsion */
     double bound;                     /* Mie / BoundLennardJones bound */
     double en, em;                    /* Mie exponents */
     double epsilon, sigma;            /* BoundLennardJones */
     double relative_interaction_range;/* CellSorting */
     double sign;                      /* CellSorting: +1 benchmark variant, -1 example variant */
-    double species_matrix[CRM_MAX_SPECIES * CRM_MAX_SPECIES]; /* MORSE_SPECIES: row = own species */
+    double species_matrix[CRM_MAX_SPECIES * CRM_MAX_SPECIES]; /* MORSE_SPECIES: strength factor;
+                                                                 CELL_SORTING_LANGEVIN: attraction
+                                                                 (rr, rb / br, bb); row = own species */
 } crm_potential;
 
 typedef struct crm_mechanics {
@@ -170,6 +178,15 @@ typedef struct crm_config {
     double dt;
     int32_t device;
     uint32_t flags;          /* CRM_FLAG_* */
+    /* CRM_WEIGHT_REFERENCE only: chili's subdomain count (Settings::n_threads,
+     * chili/settings.rs; create_subdomains, cartesian_cuboid_n.rs:19-59,493-584).
+     * 0 or 1: one subdomain.  k > 1: every instance's voxels are split into k
+     * lexicographic chunks by get_decomp_res, and a pair of agents in adjacent
+     * voxels owned by different chunks gets the full force (both halves, as
+     * update_mechanics_interaction_step_2/3 deliver them, update_mechanics.rs:
+     * 285-382) instead of 1/2 of the own evaluation.  Point agents only. */
+    uint32_t n_subdomains;
+    uint32_t reserved;
 } crm_config;
 
 typedef struct crm_info {
@@ -336,6 +353,16 @@ crm_status crm_set_rod_spring_lengths(crm_ctx* ctx, const double* spring_length)
 crm_status crm_get_cycle_state(crm_ctx* ctx, double* spring_length, double* growth_rate, uint64_t* identifier,
                                uint64_t* parent, uint32_t* history_held);
 
+/* CellIdentifier and parent of every agent by agent index (cellular_raza-concepts
+ * cell.rs:18-25, CellBox::identifier / parent, cell.rs:128-137), points and rods,
+ * each as (kind, a, b): kind 0 Initial(a) (a = the agent's index among its
+ * instance's initial cells, datastructures.rs:286-297), 1 Division(a = voxel plain
+ * index, b = counter), 3 no parent; history_held[N] = the AuxStorageMechanics
+ * entries each agent holds.  identifier and parent take 3 * n_agents values; any
+ * pointer may be NULL.  Identifiers survive removals (crm_remove_agents
+ * renumbers only the agent indices). */
+crm_status crm_get_identifiers(crm_ctx* ctx, uint64_t* identifier, uint64_t* parent, uint32_t* history_held);
+
 /* Removes the agents with remove[a] != 0 (remove[n_agents], by agent index):
  * CycleEvent::Remove as update_cell_cycle_4 applies it (chili/update_cycle.rs:
  * 41-55: the flagged cells leave their voxel's list, Vec::retain keeps the order of
@@ -387,8 +414,8 @@ crm_status crm_get_snapshot(crm_ctx* ctx, double* records, int32_t* species, uin
  * store_batch_elements and the file naming of storage/concepts.rs:702-790):
  *   <path>/<iteration:020>/batch_<storage_instance:020>_<n:020>.json
  * with n the first free counter, holding {"data": [CombinedSaveFormat ...]} with
- * one entry per agent: identifier {"Initial": agent index}, element [CellBox
- * {identifier, parent: null, cell}, AuxStorage {mechanics: {positions,
+ * one entry per agent: identifier (crm_get_identifiers), element [CellBox
+ * {identifier, parent, cell}, AuxStorage {mechanics: {positions,
  * velocities, current_force, zero_force}}].  `cell` is the caller's JSON text
  * of the agent struct with the placeholders {pos} and {vel}, one template per
  * species (n_templates > 1) or one for all; points write vectors as [x, y, z],

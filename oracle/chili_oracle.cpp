@@ -15,7 +15,8 @@
 //   potentials                       building-blocks/src/cell_building_blocks/interaction.rs:84-182,184-299,301-434,497-509
 //   mechanics                        mechanics.rs:14-226,228-387,389-571
 //   rods                             bacterial_rods.rs:146-337,489-555
-//   cell-sorting interaction         benchmarks/src/cell_sorting.rs:31-94, examples/cell_sorting/src/main.rs:32-103
+//   cell-sorting interaction         benchmarks/src/cell_sorting.rs:31-94, examples/cell_sorting/src/main.rs:32-103,
+//                                    examples/cell_sorting/src/main-langevin.rs:44-92
 //   main loop order                  core-proc-macro/src/run_sim.rs:220-409
 //
 // Arithmetic is restated operation by operation (no contraction: this file is
@@ -319,6 +320,37 @@ static void cell_sorting(int D, const crm_potential& P, const double* own, const
     }
 }
 
+// CellSpecificInteraction of examples/cell_sorting/src/main-langevin.rs:44-92.
+// Both agents share the parameters (cell_radius, repulsion = P.strength, cutoff);
+// the attraction of the (own, ext) species pair is species_matrix[own][ext]
+// (the example's match on (own, ext), 66-72).  Returns (-force, force) (89).
+static void sorting_langevin(int D, const crm_potential& P, const double* own, const double* ext, int sp_own,
+                             int sp_ext, double* f_own, double* f_ext) {
+    const double own_radius = P.radius, ext_radius = P.radius;
+    const double radius_combined = own_radius + ext_radius;
+    double z[3], dir[3];
+    for (int k = 0; k < D; k++) z[k] = own[k] - ext[k];
+    const double r = std::sqrt(norm_sq(z, D));
+    const double attraction = P.species_matrix[sp_own * CRM_MAX_SPECIES + sp_ext];
+    for (int k = 0; k < D; k++) dir[k] = r == 0.0 ? 0.0 : z[k] / r;  // normalize = z / |z|
+    double force[3];
+    if (r <= radius_combined) {
+        const double s = P.strength * (radius_combined - r);
+        for (int k = 0; k < D; k++) force[k] = s * dir[k];
+    } else if (r <= P.cutoff) {
+        const double sigma = r / radius_combined;
+        const double s = -attraction * (r - radius_combined) * (P.cutoff - r) / P.cutoff *
+                         std::exp(-1.0 * std::pow(sigma - 1.0, 2.0));
+        for (int k = 0; k < D; k++) force[k] = s * dir[k];
+    } else {
+        for (int k = 0; k < D; k++) force[k] = 0.0 * dir[k];
+    }
+    for (int k = 0; k < D; k++) {
+        f_own[k] = -force[k];
+        f_ext[k] = force[k];
+    }
+}
+
 static int point_force(int D, const crm_potential& P, const double* own, const double* ext, int sp_own,
                        int sp_ext, double* f_own, double* f_ext) {
     switch (P.kind) {
@@ -340,6 +372,9 @@ static int point_force(int D, const crm_potential& P, const double* own, const d
             return 0;
         case CRM_POTENTIAL_NONE:  // NoInteraction (interaction.rs:12-35)
             for (int k = 0; k < D; k++) f_own[k] = f_ext[k] = 0.0;
+            return 0;
+        case CRM_POTENTIAL_CELL_SORTING_LANGEVIN:
+            sorting_langevin(D, P, own, ext, sp_own, sp_ext, f_own, f_ext);
             return 0;
     }
     return 1;

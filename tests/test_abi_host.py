@@ -43,7 +43,7 @@ def test_struct_layouts_match_header_sizes():
     # sizes implied by the C declarations (x86-64 alignment rules)
     assert C.sizeof(_abi.Domain) == 72
     assert C.sizeof(_abi.Mechanics) == 64
-    assert C.sizeof(_abi.Config) == 48
+    assert C.sizeof(_abi.Config) == 56
     assert C.sizeof(_abi.Potential) == 8 + 11 * 8 + 16 * 8
 
 

@@ -160,3 +160,112 @@ def test_save_errors(tmp_path):
     with pytest.raises(SimulationError, match="InvalidArgument"):
         g.save_cells_json(str(tmp_path), 0, [])
     g.close()
+
+
+def rel_err(a, b):
+    a = np.asarray(a, dtype=float).reshape(len(a), -1)
+    b = np.asarray(b, dtype=float).reshape(len(b), -1)
+    den = np.abs(b).max()
+    return np.abs(a - b).max() / den if den > 0 else np.abs(a - b).max()
+
+
+@pytest.mark.parametrize("mech", ["newton", "langevin"])
+def test_snapshot_and_json_match_the_oracle(tmp_path, mech):
+    """The stored save point against the oracle's state and AuxStorageMechanics after
+    the same steps (AB3, so the ring is full): snapshot records and the JSON's cell
+    pos / vel and ring entries (oldest first) within 1e-12; the element in serde's
+    field order: CellBox {identifier, parent, cell} (cell.rs:128-137) and
+    AuxStorageMechanics {positions, velocities, current_force, zero_force}
+    (aux_storage.rs:41-46), inside CombinedSaveFormat {identifier, element}
+    (storage/concepts.rs:176-190) in a {"data": [...]} batch (serde_json.rs:19-74)."""
+    from oracle.oracle import OracleSim
+
+    wl = workloads.c2_sorting_3d(n=3000, seed=24)
+    inst = wl.instances[0]
+    noise = "philox"
+    if mech == "newton":
+        mechanics = NewtonDamped(damping_constant=2.0, mass=1.0)
+        wl.instances = [InstanceSpec(inst.domain, inst.interaction, mechanics)]
+        noise = "none"
+    g = MechanicsBackend(wl.instances, dt=wl.dt, mechanics_solver_order=2, noise=noise, seed=8)
+    o = OracleSim(wl.instances, dt=wl.dt, mechanics_solver_order=2, noise=noise, seed=8)
+    g.set_agents(wl.pos, wl.vel, wl.species)
+    o.set_agents(wl.pos, wl.vel, wl.species)
+    g.step(5)
+    o.step(5)
+    po, vo = o.get_state()
+    dxo, dvo, no = o.get_history()
+    s = g.snapshot()
+    assert s["dx"].shape[1] == no == 2
+    assert rel_err(s["pos"], po) <= 1e-12 and rel_err(s["vel"], vo) <= 1e-12
+    for i in range(no):
+        assert rel_err(s["dx"][:, i], dxo[i]) <= 1e-12 and rel_err(s["dv"][:, i], dvo[i]) <= 1e-12
+    templates = st.cell_templates(wl.instances[0].mechanics, wl.instances[0].interaction, n_species=2)
+    g.save_cells_json(str(tmp_path), 5, templates)
+    data = load_cells(str(tmp_path), 5)
+    assert len(data) == wl.n_agents
+    P, V = np.empty_like(po), np.empty_like(vo)
+    HX, HV = np.empty((no,) + po.shape), np.empty((no,) + vo.shape)
+    for d in data:
+        assert list(d) == ["identifier", "element"]
+        box, aux = d["element"]
+        assert list(box) == ["identifier", "parent", "cell"] and box["identifier"] == d["identifier"]
+        assert list(aux) == ["mechanics"]
+        assert list(aux["mechanics"]) == ["positions", "velocities", "current_force", "zero_force"]
+        a = d["identifier"]["Initial"]
+        P[a], V[a] = box["cell"]["mechanics"]["pos"], box["cell"]["mechanics"]["vel"]
+        HX[:, a], HV[:, a] = aux["mechanics"]["positions"], aux["mechanics"]["velocities"]
+    assert rel_err(P, po) <= 1e-12 and rel_err(V, vo) <= 1e-12
+    for i in range(no):
+        assert rel_err(HX[i], dxo[i]) <= 1e-12 and rel_err(HV[i], dvo[i]) <= 1e-12
+    g.close()
+    o.close()
+
+
+@pytest.mark.parametrize("option", [StorageOption.SerdeJson, StorageOption.Memory])
+def test_removal_keeps_identifiers_in_storage(tmp_path, option):
+    """CycleEvent::Remove keeps every survivor's CellIdentifier (update_cycle.rs:41-55):
+    after removals the stored cells are keyed by the survivors' original indices, and
+    the cell stored under each identifier is the oracle's agent with that identifier
+    (the oracle removes the same agents)."""
+    from oracle.oracle import OracleSim
+
+    rng = np.random.default_rng(26)
+    dom = CartesianCuboid.from_boundaries_and_interaction_range([0.0, 0.0], [60.0, 60.0], 4.0)
+    inter = MorsePotential(radius=1.0, potential_stiffness=1.0, cutoff=4.0, strength=0.2)
+    mech = NewtonDamped(damping_constant=1.0, mass=1.0)
+    n = 500
+    pos = rng.uniform(2.0, 58.0, size=(n, 2))
+    time = FixedStepsize.from_partial_save_steps(0.0, 0.01, 12, 4)
+    masks = {}
+
+    def remove(be, it):
+        if it % 3:
+            return None
+        m = np.random.default_rng(it).random(be.n_agents) < 0.1
+        masks[it] = m
+        return m
+
+    builder = StorageBuilder().location(tmp_path).priority([option]).add_date(False)
+    access = run_simulation(dom, inter, mech, pos, Settings(time=time, storage=builder), remove=remove,
+                            mechanics_solver_order=2)
+    # the same run on the oracle, tracking the original index of every agent
+    o = OracleSim([InstanceSpec(dom, inter, mech)], dt=0.01, mechanics_solver_order=2)
+    o.set_agents(pos)
+    orig = np.arange(n)
+    for it in range(1, 13):
+        o.step(1)
+        if it in masks:
+            o.remove_agents(masks[it])
+            orig = orig[~masks[it]]
+        if it % 4 == 0:
+            cells = access.cells.load_all_elements_at_iteration(it)
+            assert sorted(cells) == sorted(("Initial", int(a)) for a in orig)
+            po = o.get_state()[0]
+            got = np.array([cells[("Initial", int(a))][0]["cell"]["mechanics"]["pos"] for a in orig])
+            assert rel_err(got, po) <= 1e-12, it
+            for a in orig[:5]:
+                assert cells[("Initial", int(a))][0]["identifier"] == {"Initial": int(a)}
+                assert cells[("Initial", int(a))][0]["parent"] is None
+    assert sum(int(m.sum()) for m in masks.values()) > 0
+    o.close()

@@ -13,6 +13,7 @@ from cellular_raza_amd import (
     BoundLennardJones,
     CartesianCuboid,
     CellSpecificInteraction,
+    CellSpecificInteractionLangevin,
     InstanceSpec,
     MiePotential,
     MorsePotential,
@@ -255,6 +256,55 @@ def test_cell_sorting_interaction_cases():
     assert np.all(O.pair_force(bench, [1, 2, 3], [1, 2, 3])[1] == 0)
 
 
+def test_cell_sorting_langevin_regimes():
+    """main-langevin.rs:44-92.  Contact (r <= R = 2 cell_radius): linear in the
+    overlap, slope = repulsion, independent of species, zero at R.  Tail (R < r <=
+    cutoff): F c / ((r - R)(c - r)) = a exp(-(r / R - 1)^2), so its logarithm is a
+    parabola in r with second difference -2 h^2 / R^2 and the factor a is the (own,
+    ext) entry of the attraction matrix (a transposed matrix fails).  Beyond the
+    cutoff and at r = 0: zero, no error.  Sign: the example returns (-force, force),
+    so in contact the own agent is pushed towards the other (f_own . (own - ext) <
+    0) and the attraction term pushes it away."""
+    R1, rep, cut = 6.0, 5.0, 20.0
+    A = [[1.5, 0.015], [0.4, 0.075]]  # asymmetric on purpose: A[0][1] != A[1][0]
+    pot = CellSpecificInteractionLangevin(cell_radius=R1, attraction_rr=0, attraction_bb=0, attraction_rb=0,
+                                          repulsion=rep, cutoff=cut, attraction_matrix=A)
+    R = 2 * R1
+    u = np.array([0.48, -0.6, 0.64])  # unit vector
+    assert math.isclose(np.linalg.norm(u), 1.0, rel_tol=1e-15)
+
+    def f(r, a=0, b=0):
+        st, fo, fe = O.pair_force(pot, r * u, [0.0, 0.0, 0.0], a, b)
+        assert st == 0 and np.array_equal(fe, -fo)
+        return fo
+
+    # contact: F(r) = -rep (R - r) u for every species pair
+    for r in (0.5, 3.0, 7.5, 11.0):
+        for a, b in ((0, 0), (0, 1), (1, 1)):
+            fo = f(r, a, b)
+            assert np.allclose(fo, -rep * (R - r) * u, rtol=1e-14, atol=1e-14)
+            assert np.dot(fo, u) < 0
+    slope = (np.dot(f(4.0), u) - np.dot(f(9.0), u)) / 5.0
+    assert math.isclose(slope, -rep, rel_tol=1e-13)
+    assert np.all(np.abs(f(R)) < 1e-15)
+    # continuity at R and at the cutoff
+    assert np.abs(f(R + 1e-9)).max() < 1e-8 and np.abs(f(R - 1e-9)).max() < 1e-8
+    assert np.abs(f(cut - 1e-9)).max() < 1e-9
+    # tail: log(F c / ((r - R)(c - r)) / a) is -(r / R - 1)^2
+    h = 0.5
+    for a, b in ((0, 0), (0, 1), (1, 0), (1, 1)):
+        rs = 13.0 + h * np.arange(5)
+        g = np.array([np.dot(f(r, a, b), u) * cut / ((r - R) * (cut - r)) for r in rs])
+        assert np.all(g > 0)  # the tail term pushes the own agent along +u: the (-force, force) sign
+        lg = np.log(g)
+        d2 = lg[2:] - 2 * lg[1:-1] + lg[:-2]
+        assert np.allclose(d2, -2 * h * h / R ** 2, rtol=1e-9)
+        assert np.allclose(lg, np.log(A[a][b]) - (rs / R - 1) ** 2, rtol=0, atol=1e-13)
+    # beyond the cutoff and at zero distance
+    assert np.all(f(cut + 1e-9) == 0)
+    assert np.all(f(0.0) == 0)
+
+
 # ---------------------------------------------------------------------------
 # Rods
 # ---------------------------------------------------------------------------
@@ -341,6 +391,29 @@ def test_pair_weighting_same_adjacent_and_cross_subdomain():
     s2, _ = _two_body(3.0, n_sub=2)
     s2.set_agents(np.array([[18.0, 5.0], [21.0, 5.0]]))
     assert np.allclose(s2.compute_forces()[0], 2 * half, rtol=1e-15)
+
+
+@pytest.mark.parametrize("weighting,k", [("reference", 1), ("reference", 2), ("reference", 7), ("physical", 1)])
+def test_forces_sum_to_zero_every_weighting(weighting, k):
+    """The part of the pair weighting that the mathematics fixes: with an interaction
+    symmetric under exchange of the agents, each pair delivers opposite forces to its
+    two agents under every weighting (same voxel: both halves to both; adjacent voxel
+    in one subdomain: half of its own evaluation to each; across subdomains: both
+    halves to both), so the forces sum to zero.  Delivering a half to one agent only
+    leaves |F| / 2 per such pair."""
+    wl_dom = CartesianCuboid.from_boundaries_and_interaction_range([0.0] * 3, [24.0] * 3, 2.5)
+    pot = SpeciesMorsePotential(radius=1.0, potential_stiffness=1.0, cutoff=2.5, strength=0.5,
+                                species_matrix=[[1.0, 0.2], [0.2, 1.0]])
+    rng = np.random.default_rng(17)
+    pos = rng.uniform(1e-6, 24.0, (2500, 3))
+    sp = (rng.random(2500) < 0.5).astype(np.int32)
+    s = O.OracleSim([InstanceSpec(wl_dom, pot, NewtonDamped(1.0, 1.0))], dt=0.01, weighting=weighting,
+                    n_subdomains=k)
+    s.set_agents(pos, species=sp)
+    f = s.compute_forces()
+    scale = np.abs(f).sum(axis=0)
+    assert np.all(scale > 0)
+    assert np.all(np.abs(f.sum(axis=0)) <= 1e-12 * scale), (f.sum(axis=0), scale)
 
 
 def test_euler_damped_closed_form():
