@@ -531,6 +531,72 @@ __device__ __forceinline__ void agent_force(const Arrays& A, const StateBuf& S, 
         }
     };
 
+    if constexpr (std::is_same<W, Walk3>::value) {
+        // LDS source: the stream is three ranges of LDS indices, [j0, e0), [j1, e1),
+        // [j2, e2), read as one index space p in [0, L): p < B1 is range 0, p < B2
+        // range 1, the rest range 2 (empty ranges fold away), so the walk needs no
+        // per-candidate range bookkeeping.  The own record sits in range 1 at p_self
+        // and is unmarked once per chunk.  Pass 1 marks, pass 2 evaluates the marked
+        // positions in stream order -- the same pairs, order and arithmetic as the
+        // generic walk below.
+        const bool active = POT != CRM_POTENTIAL_NONE || sense;
+        const uint32_t j0 = cur.j, B1 = cur.je - cur.j, B2 = B1 + (cur.e1 - cur.j1);
+        const uint32_t L = active ? B2 + (cur.e2 - cur.j2) : 0u;
+        const uint32_t o1 = cur.j1 - cur.je, o2 = cur.j2 - cur.e1;  // modular offsets
+        const uint32_t p_self = B1 + (self - cur.j1);
+        auto at = [&](uint32_t p) -> uint32_t { return j0 + p + (p >= B1 ? o1 : 0u) + (p >= B2 ? o2 : 0u); };
+        // unit grids (equal fixed-point steps on every axis) need no axis scaling in
+        // pass 1 (r1 = r2 = 1 exactly, so the marks are the same)
+        const bool cubic = !MIXED || (r1 == 1.0f && r2 == 1.0f);
+        auto marked = [&](auto cub, uint32_t p) -> uint32_t {
+            const Rec rj = src.record(at(p), cur);
+            bool mark;
+            if constexpr (MIXED) {
+                float d2;
+                if constexpr (decltype(cub)::value) {
+                    const float c0 = (float)(me.x - rj.x), c1 = (float)(me.y - rj.y);
+                    d2 = fmaf(c1, c1, c0 * c0);
+                    if constexpr (D == 3) {
+                        const float c2 = (float)(me.z - rj.z);
+                        d2 = fmaf(c2, c2, d2);
+                    }
+                } else {
+                    d2 = mark_dist(rj);
+                }
+                mark = d2 <= mark_f || rj.w < 0;
+            } else {
+                mark = mark_dist(rj) <= mark_d;
+            }
+            return mark ? 1u : 0u;
+        };
+        auto pass1 = [&](auto cub, uint32_t p0, uint32_t qn, uint32_t& mlo, uint32_t& mhi) {
+            const uint32_t q1 = min(qn, 32u);
+            for (uint32_t q = 0; q < q1; q++) mlo |= marked(cub, p0 + q) << q;
+            for (uint32_t q = 32; q < qn; q++) mhi |= marked(cub, p0 + q) << (q - 32u);
+        };
+        for (uint32_t p0 = 0; p0 < L; p0 += 64u) {
+            const uint32_t qn = min(64u, L - p0);
+            uint32_t mlo = 0, mhi = 0;
+            if (cubic) pass1(std::true_type{}, p0, qn, mlo, mhi);
+            else pass1(std::false_type{}, p0, qn, mlo, mhi);
+            const uint32_t ds = p_self - p0;  // the self pair is never evaluated
+            if (ds < 32u) mlo &= ~(1u << ds);
+            else if (ds < 64u) mhi &= ~(1u << (ds - 32u));
+            W w = cur;
+            while (mlo | mhi) {
+                uint32_t q;
+                if (mlo) {
+                    q = (uint32_t)__builtin_ctz(mlo);
+                    mlo &= mlo - 1;
+                } else {
+                    q = 32u + (uint32_t)__builtin_ctz(mhi);
+                    mhi &= mhi - 1;
+                }
+                w.j = at(p0 + q);
+                evaluate(w);
+            }
+        }
+    } else {
     bool live = POT != CRM_POTENTIAL_NONE || sense;
     if (live) live = cur.settle();
     while (live) {
@@ -570,6 +636,7 @@ __device__ __forceinline__ void agent_force(const Arrays& A, const StateBuf& S, 
             evaluate(w);
         }
     }
+    }
     if (calc_err) raise_error(A.ctrl, CRM_CALC_ERROR, A.ctrl->step, S.id[s]);
     // the domain force follows the pair forces (calculate_custom_domain_force after
     // the subdomain's pairs, run_sim.rs:277-279, update_mechanics.rs:244-274)
@@ -593,6 +660,15 @@ constexpr int kSortBins = 64;           // candidate-count classes for the lane 
 
 struct ForceTile {
     uint32_t inst, a0, b0, b1, c0, c1;
+};
+
+// An own agent of a force tile, looked up once: its record's LDS index, its
+// voxel's LDS range, the three LDS ranges of its candidate stream, its slot.
+struct OwnEntry {
+    uint16_t self, ovb, ove;
+    uint16_t r[6];
+    uint16_t pad;
+    uint32_t slot;
 };
 
 // Inclusive prefix sum over the 64 lanes of a wave (all lanes active).
@@ -650,6 +726,7 @@ __global__ __launch_bounds__(kBlock, MIXED ? 5 : 4) void k_force_tiled(Arrays A,
     __shared__ uint32_t own_gs[kOwnVoxMax];          // global slot of own voxel u's first agent
     __shared__ uint8_t ord[kBlock];                  // own agents in lane order
     __shared__ uint32_t bins[kSortBins];
+    __shared__ OwnEntry own_tab[kBlock];             // own agent lookups (sorted case)
     __shared__ uint16_t lsub[CAP];                   // chili subdomain of each staged record (sub_on only)
     const Rec* __restrict__ rec = rec_array(A, std::integral_constant<bool, MIXED>());
     const ForceTile T = tiles[xcd_block(blockIdx.x, gridDim.x)];
@@ -773,6 +850,8 @@ __global__ __launch_bounds__(kBlock, MIXED ? 5 : 4) void k_force_tiled(Arrays A,
             // that a wave's lanes walk streams of similar length (a wave runs as
             // long as its longest stream).  Which lane computes an agent does not
             // change the agent's sums.
+            // The sorted case looks each own agent up once and keeps the result in
+            // LDS: self, own voxel range, the three stream ranges and the slot.
             const bool sorted = n_own <= (uint32_t)kBlock;
             if (sorted) {
                 if (threadIdx.x < kSortBins) bins[threadIdx.x] = 0;
@@ -782,10 +861,21 @@ __global__ __launch_bounds__(kBlock, MIXED ? 5 : 4) void k_force_tiled(Arrays A,
                     uint32_t self;
                     Span ov;
                     Walk3 w;
-                    own_agent(threadIdx.x, self, ov, w);
+                    const uint32_t s = own_agent(threadIdx.x, self, ov, w);
                     const uint32_t L = (w.je - w.j) + (w.e1 - w.j1) + (w.e2 - w.j2);
                     key = (uint32_t)(kSortBins - 1) - min(L >> 1, (uint32_t)(kSortBins - 1));
                     kr = atomicAdd(&bins[key], 1u);
+                    OwnEntry& e = own_tab[threadIdx.x];
+                    e.self = (uint16_t)self;
+                    e.ovb = (uint16_t)ov.b;
+                    e.ove = (uint16_t)ov.e;
+                    e.r[0] = (uint16_t)w.j;
+                    e.r[1] = (uint16_t)w.je;
+                    e.r[2] = (uint16_t)w.j1;
+                    e.r[3] = (uint16_t)w.e1;
+                    e.r[4] = (uint16_t)w.j2;
+                    e.r[5] = (uint16_t)w.e2;
+                    e.slot = s;
                 }
                 __syncthreads();
                 if (threadIdx.x < kSortBins) {  // exclusive scan of the class sizes (wave 0)
@@ -795,9 +885,22 @@ __global__ __launch_bounds__(kBlock, MIXED ? 5 : 4) void k_force_tiled(Arrays A,
                 __syncthreads();
                 if (threadIdx.x < n_own) ord[bins[key] + kr] = (uint8_t)threadIdx.x;
                 __syncthreads();
+                if (threadIdx.x < n_own) {
+                    const OwnEntry e = own_tab[ord[threadIdx.x]];
+                    Walk3 w;
+                    w.j = e.r[0];
+                    w.je = e.r[1];
+                    w.j1 = e.r[2];
+                    w.e1 = e.r[3];
+                    w.j2 = e.r[4];
+                    w.e2 = e.r[5];
+                    w.left = 2;
+                    agent_force<D, POT, MIXED, SENSE>(A, S, P, e.slot, e.self, lds[e.self], Span{e.ovb, e.ove}, w,
+                                                      physical, symmetric, src, unit);
+                }
+                return;
             }
-            for (uint32_t t0 = threadIdx.x; t0 < n_own; t0 += kBlock) {
-                const uint32_t t = sorted ? (uint32_t)ord[t0] : t0;
+            for (uint32_t t = threadIdx.x; t < n_own; t += kBlock) {
                 uint32_t self;
                 Span ov;
                 Walk3 w;
