@@ -160,6 +160,21 @@ static void fill_pot(PotParams<T>& q, const crm_potential& p) {
 
 enum { kNumKernelClasses = CRM_KERNEL_COUNT };
 
+// ---- device scratch of crm_remove_agents_noalloc ---------------------------
+// One static array per GPU, sized for kNoAllocMaxAgents agents in at most
+// kNoAllocMaxInstances instances: flags [n] bytes, four [n + 1] uint32 arrays
+// (survivor flags by slot and by agent index, their exclusive scans), removed
+// agents per instance, and the scan's temporary storage, at 256-byte aligned
+// offsets.  Host code uses the per-device address from hipGetSymbolAddress,
+// never the symbol itself.
+constexpr uint64_t kNoAllocMaxAgents = 1ull << 22;
+constexpr uint32_t kNoAllocMaxInstances = 1u << 16;
+constexpr size_t kNoAllocScanReserve = (size_t)1 << 20;
+constexpr size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+constexpr size_t kNoAllocScratchBytes = align256(kNoAllocMaxAgents) + 4 * align256(4 * (kNoAllocMaxAgents + 1)) +
+                                        align256(4 * (size_t)kNoAllocMaxInstances) + kNoAllocScanReserve;
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
 }  // namespace crm
 
 using namespace crm;
@@ -225,6 +240,9 @@ struct crm_ctx {
     uint32_t* d_inst_div = nullptr; // [n_inst] dividing rods per instance
     uint8_t* d_rm = nullptr;        // agent removal scratch (grow-only), rm_cap bytes
     size_t rm_cap = 0;
+    uint8_t* d_noalloc = nullptr;   // this device's copy of `scratch` (crm_remove_agents_noalloc)
+    std::vector<uint32_t> inst_host;  // [n_inst] host copy of per-instance counters
+    bool graphs_stale = false;      // captured graphs no longer match (destroyed by the next crm_step)
     std::vector<BlockWork> work_host;  // the work table last uploaded
     void* d_scan_tmp = nullptr;
     size_t scan_tmp_bytes = 0;
@@ -258,6 +276,25 @@ namespace {
 
 // Message of the last failed crm_create on this thread (crm_last_error(NULL, ...)).
 thread_local std::string g_create_error;
+
+// Address of `scratch` on each device, looked up once per device (with that
+// device current) by the first crm_create there.
+std::mutex g_scratch_mutex;
+std::map<int, uint8_t*> g_scratch_ptr;
+
+hipError_t scratch_address(int device, uint8_t*& out) {
+    std::lock_guard<std::mutex> lk(g_scratch_mutex);
+    auto it = g_scratch_ptr.find(device);
+    if (it != g_scratch_ptr.end()) {
+        out = it->second;
+        return hipSuccess;
+    }
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = out = static_cast<uint8_t*>(p);
+    return hipSuccess;
+}
 
 void free_device(crm_ctx* ctx) {
     void* ptrs[] = {ctx->d_work, ctx->d_tiles, ctx->d_start, ctx->d_count, ctx->d_bsum, ctx->d_key, ctx->d_rank,
@@ -306,6 +343,7 @@ void free_device(crm_ctx* ctx) {
             (void)hipGraphExecDestroy(g);
             g = nullptr;
         }
+    ctx->graphs_stale = false;
 }
 
 // d_count holds the arrivals per voxel, then (from this offset, 256-byte aligned)
@@ -1141,6 +1179,119 @@ void put_cell(std::string& o, const double* r, const Template& T, uint32_t nv, u
     o += "\n          }\n        }\n      ]\n    }";
 }
 
+// Checks of a removal call; n_rm = agents flagged.
+crm_status remove_prelude(crm_ctx* ctx, const uint8_t* remove, uint64_t& n_rm) {
+    if (!ctx || !remove) return CRM_INVALID_ARGUMENT;
+    if (!ctx->have_agents) {
+        ctx->err = "crm_set_agents must be called first";
+        return CRM_INVALID_ARGUMENT;
+    }
+    if (ctx->halted) return CRM_INVALID_ARGUMENT;
+    n_rm = 0;
+    for (uint64_t a = 0; a < ctx->N; a++) n_rm += remove[a] ? 1u : 0u;
+    if (n_rm == ctx->N) {
+        ctx->err = "the removal would leave no agents";
+        return CRM_INVALID_ARGUMENT;
+    }
+    return CRM_OK;
+}
+
+// Removal scratch for n_old agents, carved at 256-byte aligned offsets: flags
+// [n_old] bytes, four [n_old + 1] uint32 arrays, [n_inst] uint32, the scan's storage.
+struct RemoveLayout {
+    size_t keep_slot, keep_id, slot_before, id_before, inst, scan, scan_bytes, total;
+};
+
+crm_status remove_layout(crm_ctx* ctx, uint64_t n_old, RemoveLayout& L) {
+    const size_t n1 = n_old + 1;
+    size_t bytes = 0;
+    CRM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (uint32_t*)nullptr, (uint32_t*)nullptr, (int)n1,
+                                             ctx->stream));
+    L.keep_slot = align256(n_old);
+    L.keep_id = L.keep_slot + align256(4 * n1);
+    L.slot_before = L.keep_id + align256(4 * n1);
+    L.id_before = L.slot_before + align256(4 * n1);
+    L.inst = L.id_before + align256(4 * n1);
+    L.scan = L.inst + align256(4 * (size_t)ctx->n_inst);
+    L.scan_bytes = std::max<size_t>(bytes, 1);
+    L.total = L.scan + align256(L.scan_bytes);
+    return CRM_OK;
+}
+
+// CycleEvent::Remove on the device with the scratch at `base` (layout L): survivor
+// flags, their scans, compaction into the other state buffer, then a set-up sort.
+// Allocates no device memory.
+crm_status remove_agents_impl(crm_ctx* ctx, const uint8_t* remove, uint64_t n_rm, uint8_t* base,
+                              const RemoveLayout& L) {
+    const uint64_t n_old = ctx->N, n_new = n_old - n_rm;
+    CRM_HIP(hipSetDevice(ctx->device));
+    RemoveArgs X{};
+    X.flag = base;
+    X.keep_slot = reinterpret_cast<uint32_t*>(base + L.keep_slot);
+    X.keep_id = reinterpret_cast<uint32_t*>(base + L.keep_id);
+    X.slot_before = reinterpret_cast<uint32_t*>(base + L.slot_before);
+    X.id_before = reinterpret_cast<uint32_t*>(base + L.id_before);
+    X.inst_rm = reinterpret_cast<uint32_t*>(base + L.inst);
+    void* scan_tmp = base + L.scan;
+    size_t bytes = L.scan_bytes;
+    X.n_old = n_old;
+    X.n_inst = ctx->n_inst;
+    CRM_HIP(hipMemcpyAsync(base, remove, n_old, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_fill_u32, dim3((ctx->n_inst + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream,
+                       X.inst_rm, ctx->n_inst, 0u);
+    const StateBuf src = ctx->buf[ctx->cur], dst = ctx->buf[1 - ctx->cur];
+    hipLaunchKernelGGL(k_remove_mark, dim3((uint32_t)((n_old + 1 + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       ctx->stream, arrays(ctx), src, X);
+    CRM_HIP(hipGetLastError());
+    CRM_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, bytes, X.keep_slot, X.slot_before, (int)(n_old + 1),
+                                             ctx->stream));
+    CRM_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, bytes, X.keep_id, X.id_before, (int)(n_old + 1),
+                                             ctx->stream));
+    hipLaunchKernelGGL(k_remove_compact, dim3((uint32_t)((n_old + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       ctx->stream, src, dst, X, n_new, ctx->K, ctx->cfg.solver_order, (uint32_t)(is_rod(ctx) ? 1 : 0));
+    CRM_HIP(hipGetLastError());
+    uint32_t* inst_rm = ctx->inst_host.data();
+    CRM_HIP(hipMemcpyAsync(inst_rm, X.inst_rm, sizeof(uint32_t) * ctx->n_inst, hipMemcpyDeviceToHost, ctx->stream));
+    CRM_HIP(hipStreamSynchronize(ctx->stream));
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < ctx->n_inst; i++) {
+        if (inst_rm[i] > ctx->counts[i]) total = UINT64_MAX;
+        if (total != UINT64_MAX) total += inst_rm[i];
+    }
+    if (total != n_rm) {
+        ctx->err = "agent removal: the removed agents do not match the instances' voxels";
+        ctx->halted = true;
+        return CRM_DEVICE_ERROR;
+    }
+    for (uint32_t i = 0; i < ctx->n_inst; i++) ctx->counts[i] -= inst_rm[i];
+    if (!is_rod(ctx)) {  // identifiers follow the agent indices, in order (shrinks in place)
+        uint64_t m = 0;
+        for (uint64_t a = 0; a < n_old; a++) {
+            if (remove[a]) continue;
+            for (int q = 0; q < 3; q++) {
+                ctx->pt_ident[3 * m + q] = ctx->pt_ident[3 * a + q];
+                ctx->pt_parent[3 * m + q] = ctx->pt_parent[3 * a + q];
+            }
+            m++;
+        }
+        ctx->pt_ident.resize(3 * n_new);
+        ctx->pt_parent.resize(3 * n_new);
+    }
+    ctx->N = n_new;
+    ctx->cur = 1 - ctx->cur;
+    CRM_TRY(rebuild_work(ctx));
+    ctx->sense_parity = -1;
+    ctx->graphs_stale = true;  // captured for the old agent count
+    // the voxel lists of the survivors: a set-up sort with every agent in its voxel
+    CRM_HIP(hipMemsetAsync(ctx->d_count, 0, sizeof(uint32_t) * (stay_offset(ctx) + ctx->Vt), ctx->stream));
+    hipLaunchKernelGGL(k_remove_key, dim3((uint32_t)((n_new + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream,
+                       arrays(ctx), ctx->buf[ctx->cur]);
+    CRM_HIP(hipGetLastError());
+    CRM_TRY(launch_sort(ctx, false, nullptr));
+    CRM_HIP(hipStreamSynchronize(ctx->stream));
+    return check_errors(ctx, "agent removal");
+}
+
 }  // namespace
 
 extern "C" {
@@ -1266,6 +1417,9 @@ crm_status crm_create(const crm_config* cfg, const crm_instance* instances, uint
     ctx->device = cfg->device;
     hipError_t e = hipSetDevice(cfg->device);
     if (e != hipSuccess) return fail(CRM_DEVICE_ERROR, std::string("hipSetDevice: ") + hipGetErrorString(e));
+    e = scratch_address(cfg->device, ctx->d_noalloc);
+    if (e != hipSuccess) return fail(CRM_DEVICE_ERROR, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    ctx->inst_host.assign(n_instances, 0);
     e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e != hipSuccess) return fail(CRM_DEVICE_ERROR, std::string("hipStreamCreate: ") + hipGetErrorString(e));
     if (hipEventCreate(&ctx->ev_a) != hipSuccess || hipEventCreate(&ctx->ev_b) != hipSuccess)
@@ -1383,6 +1537,7 @@ crm_status crm_set_agents(crm_ctx* ctx, const uint64_t* counts, const double* po
         CRM_TRY(alloc_scan_tmp(ctx));
     }
     CRM_HIP(hipMemcpy(ctx->d_work, work.data(), sizeof(BlockWork) * work.size(), hipMemcpyHostToDevice));
+    ctx->work_host = work;  // later rebuilds (removals) shrink it in place
     CRM_TRY(upload_tiles(ctx, tiles));
     CRM_HIP(hipMemsetAsync(ctx->d_count, 0, sizeof(uint32_t) * (stay_offset(ctx) + Vt), ctx->stream));
 
@@ -1491,6 +1646,14 @@ crm_status crm_step(crm_ctx* ctx, uint32_t n_steps, const double* noise) {
         return check_errors(ctx, "step");
     }
     const bool graphs = (ctx->cfg.flags & CRM_FLAG_GRAPH) && !timing_on(ctx) && !use_noise_buf;
+    if (ctx->graphs_stale) {  // captured before a removal changed the agent count
+        for (auto& g : ctx->graph)
+            if (g) {
+                (void)hipGraphExecDestroy(g);
+                g = nullptr;
+            }
+        ctx->graphs_stale = false;
+    }
     uint32_t s = 0;
     if (graphs) {
         while (s + 2 <= n_steps) {
@@ -1824,108 +1987,38 @@ crm_status crm_get_identifiers(crm_ctx* ctx, uint64_t* identifier, uint64_t* par
 }
 
 crm_status crm_remove_agents(crm_ctx* ctx, const uint8_t* remove) {
-    if (!ctx || !remove) return CRM_INVALID_ARGUMENT;
-    if (!ctx->have_agents) {
-        ctx->err = "crm_set_agents must be called first";
-        return CRM_INVALID_ARGUMENT;
-    }
-    if (ctx->halted) return CRM_INVALID_ARGUMENT;
-    const uint64_t n_old = ctx->N;
     uint64_t n_rm = 0;
-    for (uint64_t a = 0; a < n_old; a++) n_rm += remove[a] ? 1u : 0u;
+    CRM_TRY(remove_prelude(ctx, remove, n_rm));
     if (n_rm == 0) return CRM_OK;
-    if (n_rm == n_old) {
-        ctx->err = "the removal would leave no agents";
-        return CRM_INVALID_ARGUMENT;
-    }
-    const uint64_t n_new = n_old - n_rm;
-    CRM_HIP(hipSetDevice(ctx->device));
-    // scratch, carved from one grow-only buffer at 256-byte aligned offsets: flags
-    // [n_old] bytes, four [n_old + 1] uint32 arrays, [n_inst] uint32, the scan's storage
-    const size_t n1 = n_old + 1;
-    size_t bytes = 0;
-    CRM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (uint32_t*)nullptr, (uint32_t*)nullptr, (int)n1,
-                                             ctx->stream));
-    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    const size_t o_keep_slot = al(n_old), o_keep_id = o_keep_slot + al(4 * n1), o_slot_before = o_keep_id + al(4 * n1),
-                 o_id_before = o_slot_before + al(4 * n1), o_inst = o_id_before + al(4 * n1),
-                 o_scan = o_inst + al(4 * (size_t)ctx->n_inst), need = o_scan + al(std::max<size_t>(bytes, 1));
-    if (need > ctx->rm_cap) {
+    RemoveLayout L;
+    CRM_TRY(remove_layout(ctx, ctx->N, L));
+    if (L.total > ctx->rm_cap) {  // grow-only scratch of the context
         if (ctx->d_rm) CRM_HIP(hipFree(ctx->d_rm));
         ctx->d_rm = nullptr;
         ctx->rm_cap = 0;
-        CRM_TRY(dalloc(ctx, ctx->d_rm, need));
-        ctx->rm_cap = need;
+        CRM_TRY(dalloc(ctx, ctx->d_rm, L.total));
+        ctx->rm_cap = L.total;
     }
-    RemoveArgs X{};
-    uint8_t* flag = ctx->d_rm;
-    X.keep_slot = reinterpret_cast<uint32_t*>(ctx->d_rm + o_keep_slot);
-    X.keep_id = reinterpret_cast<uint32_t*>(ctx->d_rm + o_keep_id);
-    X.slot_before = reinterpret_cast<uint32_t*>(ctx->d_rm + o_slot_before);
-    X.id_before = reinterpret_cast<uint32_t*>(ctx->d_rm + o_id_before);
-    X.inst_rm = reinterpret_cast<uint32_t*>(ctx->d_rm + o_inst);
-    void* scan_tmp = ctx->d_rm + o_scan;
-    X.flag = flag;
-    X.n_old = n_old;
-    X.n_inst = ctx->n_inst;
-    CRM_HIP(hipMemcpyAsync(flag, remove, n_old, hipMemcpyHostToDevice, ctx->stream));
-    CRM_HIP(hipMemsetAsync(X.inst_rm, 0, sizeof(uint32_t) * ctx->n_inst, ctx->stream));
-    const StateBuf src = ctx->buf[ctx->cur], dst = ctx->buf[1 - ctx->cur];
-    hipLaunchKernelGGL(k_remove_mark, dim3((uint32_t)((n_old + 1 + kBlock - 1) / kBlock)), dim3(kBlock), 0,
-                       ctx->stream, arrays(ctx), src, X);
-    CRM_HIP(hipGetLastError());
-    CRM_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, bytes, X.keep_slot, X.slot_before, (int)(n_old + 1),
-                                             ctx->stream));
-    CRM_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, bytes, X.keep_id, X.id_before, (int)(n_old + 1),
-                                             ctx->stream));
-    hipLaunchKernelGGL(k_remove_compact, dim3((uint32_t)((n_old + kBlock - 1) / kBlock)), dim3(kBlock), 0,
-                       ctx->stream, src, dst, X, n_new, ctx->K, ctx->cfg.solver_order, (uint32_t)(is_rod(ctx) ? 1 : 0));
-    CRM_HIP(hipGetLastError());
-    std::vector<uint32_t> inst_rm(ctx->n_inst);
-    CRM_HIP(hipMemcpyAsync(inst_rm.data(), X.inst_rm, sizeof(uint32_t) * ctx->n_inst, hipMemcpyDeviceToHost,
-                           ctx->stream));
-    CRM_HIP(hipStreamSynchronize(ctx->stream));
-    uint64_t total = 0;
-    for (uint32_t i = 0; i < ctx->n_inst; i++) {
-        if (inst_rm[i] > ctx->counts[i]) total = UINT64_MAX;
-        if (total != UINT64_MAX) total += inst_rm[i];
+    return remove_agents_impl(ctx, remove, n_rm, ctx->d_rm, L);
+}
+
+crm_status crm_remove_agents_noalloc(crm_ctx* ctx, const uint8_t* remove) {
+    uint64_t n_rm = 0;
+    CRM_TRY(remove_prelude(ctx, remove, n_rm));
+    if (n_rm == 0) return CRM_OK;
+    if (ctx->N > kNoAllocMaxAgents || ctx->n_inst > kNoAllocMaxInstances) {
+        ctx->err = "crm_remove_agents_noalloc handles at most " + std::to_string(kNoAllocMaxAgents) +
+                   " agents in " + std::to_string(kNoAllocMaxInstances) + " instances (the context holds " +
+                   std::to_string(ctx->N) + " in " + std::to_string(ctx->n_inst) + ")";
+        return CRM_INVALID_ARGUMENT;
     }
-    if (total != n_rm) {
-        ctx->err = "agent removal: the removed agents do not match the instances' voxels";
-        ctx->halted = true;
-        return CRM_DEVICE_ERROR;
+    RemoveLayout L;
+    CRM_TRY(remove_layout(ctx, ctx->N, L));
+    if (L.total > kNoAllocScratchBytes || L.scan_bytes > kNoAllocScanReserve) {
+        ctx->err = "crm_remove_agents_noalloc: the scan needs more scratch than reserved";
+        return CRM_INVALID_ARGUMENT;
     }
-    for (uint32_t i = 0; i < ctx->n_inst; i++) ctx->counts[i] -= inst_rm[i];
-    if (!is_rod(ctx)) {  // identifiers follow the agent indices, in order
-        uint64_t m = 0;
-        for (uint64_t a = 0; a < n_old; a++) {
-            if (remove[a]) continue;
-            for (int q = 0; q < 3; q++) {
-                ctx->pt_ident[3 * m + q] = ctx->pt_ident[3 * a + q];
-                ctx->pt_parent[3 * m + q] = ctx->pt_parent[3 * a + q];
-            }
-            m++;
-        }
-        ctx->pt_ident.resize(3 * n_new);
-        ctx->pt_parent.resize(3 * n_new);
-    }
-    ctx->N = n_new;
-    ctx->cur = 1 - ctx->cur;
-    CRM_TRY(rebuild_work(ctx));
-    ctx->sense_parity = -1;
-    for (auto& g : ctx->graph)  // captured for the old agent count
-        if (g) {
-            (void)hipGraphExecDestroy(g);
-            g = nullptr;
-        }
-    // the voxel lists of the survivors: a set-up sort with every agent in its voxel
-    CRM_HIP(hipMemsetAsync(ctx->d_count, 0, sizeof(uint32_t) * (stay_offset(ctx) + ctx->Vt), ctx->stream));
-    hipLaunchKernelGGL(k_remove_key, dim3((uint32_t)((n_new + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream,
-                       arrays(ctx), ctx->buf[ctx->cur]);
-    CRM_HIP(hipGetLastError());
-    CRM_TRY(launch_sort(ctx, false, nullptr));
-    CRM_HIP(hipStreamSynchronize(ctx->stream));
-    return check_errors(ctx, "agent removal");
+    return remove_agents_impl(ctx, remove, n_rm, ctx->d_noalloc, L);
 }
 
 crm_status crm_set_neighbor_sensing(crm_ctx* ctx, uint32_t enable, double radius) {

@@ -29,6 +29,12 @@ struct RemoveArgs {
     uint32_t n_inst;
 };
 
+// x[0 .. n) = v (clears counters without a memset node).
+__global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t* x, uint32_t n, uint32_t v) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) x[i] = v;
+}
+
 // Thread q in [0, n_old]: q is both a slot (survivor flag by slot, and the
 // instance count through the slot's voxel) and an agent index.
 __global__ __launch_bounds__(kBlock) void k_remove_mark(Arrays A, StateBuf S, RemoveArgs X) {

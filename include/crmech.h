@@ -379,6 +379,21 @@ crm_status crm_get_identifiers(crm_ctx* ctx, uint64_t* identifier, uint64_t* par
  * and when the agent count exceeds every earlier call's). */
 crm_status crm_remove_agents(crm_ctx* ctx, const uint8_t* remove);
 
+/* No counterpart in the reference: crm_remove_agents for callers that cannot
+ * allocate device memory (code under HIP graph capture elsewhere in the process,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same semantics and
+ * results as crm_remove_agents, but its scratch (removal flags, survivor flags
+ * by slot and by agent index with their exclusive scans, removed agents per
+ * instance, the scans' temporary storage) lives in one static device array of
+ * about 69 MiB per GPU, held until the process ends, instead of the context's
+ * grow-only allocation: it calls no hipMalloc / hipFree.  Limits: at most
+ * 4,194,304 agents in at most 65,536 instances, else CRM_INVALID_ARGUMENT with
+ * nothing changed.  Every call on one GPU uses that GPU's array: calls that
+ * overlap in time on one GPU (two host threads, or two contexts on that device
+ * driven concurrently) race and must be serialised by the caller.  Graphs
+ * captured before the call are released by the next crm_step, not here. */
+crm_status crm_remove_agents_noalloc(crm_ctx* ctx, const uint8_t* remove);
+
 /* Neighbour sensing fused into the force pass.  Replaces the NeighborSensing
  * hooks of the chili pair loop (concepts interaction.rs:64-79 trait,
  * update_mechanics.rs:125-139 neighbor_sensing_single, called per pair at
