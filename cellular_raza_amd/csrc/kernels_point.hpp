@@ -755,24 +755,46 @@ __global__ __launch_bounds__(kBlock, MIXED ? 5 : 4) void k_force_tiled(Arrays A,
 
     if (fits) {
         // staged voxel v = (rb * NC + c - cl) * NP + pa; thread t holds v = 3t .. 3t + 2
-        const float inv_np = 1.0f / (float)NP, inv_nc = 1.0f / (float)G.NC;
+        // (in 3-D the three planes of one (row, column)).  Their (plane, row, column)
+        // come from one decomposition of 3t, then step by one voxel each.
         uint32_t cnt[3], gs[3], sum = 0;
+        int pa_q[3], rb_q[3];
+        uint32_t c_q[3];
+        {
+            const uint32_t v0 = 3 * threadIdx.x;
+            uint32_t pa, rb, c;
+            if constexpr (NP == 3) {
+                pa = 0;
+                rb = small_div(threadIdx.x, 1.0f / (float)G.NC);
+                c = G.cl + (threadIdx.x - rb * G.NC);
+            } else {
+                pa = 0;
+                rb = small_div(v0, 1.0f / (float)G.NC);
+                c = G.cl + (v0 - rb * G.NC);
+            }
 #pragma unroll
-        for (int q = 0; q < 3; q++) {
-            const uint32_t v = 3 * threadIdx.x + q;
-            cnt[q] = 0;
-            gs[q] = 0;
-            if (v < n_vox) {
-                const uint32_t t = small_div(v, inv_np), pa = v - t * NP;
-                const uint32_t rb = small_div(t, inv_nc), c = G.cl + (t - rb * G.NC);
+            for (int q = 0; q < 3; q++) {
+                pa_q[q] = (int)pa;
+                rb_q[q] = (int)rb;
+                c_q[q] = c;
+                cnt[q] = 0;
+                gs[q] = 0;
                 const int a = (int)T.a0 + (int)pa - (int)PA_OWN, b = (int)T.b0 - 1 + (int)rb;
-                if (a >= 0 && a < (int)n0 && b >= 0 && b < (int)n1) {
+                if (v0 + q < n_vox && a >= 0 && a < (int)n0 && b >= 0 && b < (int)n1) {
                     const uint32_t g = voff + ((uint32_t)a * n1 + (uint32_t)b) * n2 + c;
                     gs[q] = A.start[g];
                     cnt[q] = A.start[g + 1] - gs[q];
                 }
+                sum += cnt[q];
+                // next voxel in the staged order
+                if constexpr (NP == 3) {
+                    pa++;
+                } else {
+                    const bool wrap = c + 1 == G.ch;
+                    c = wrap ? G.cl : c + 1;
+                    rb = wrap ? rb + 1 : rb;
+                }
             }
-            sum += cnt[q];
         }
         uint32_t total;
         uint32_t run = block_exclusive_scan(sum, total);
@@ -783,8 +805,7 @@ __global__ __launch_bounds__(kBlock, MIXED ? 5 : 4) void k_force_tiled(Arrays A,
                 const uint32_t v = 3 * threadIdx.x + q;
                 if (v < n_vox) {
                     vst[v] = (uint16_t)run;
-                    const uint32_t t = small_div(v, inv_np), pa = v - t * NP;
-                    const uint32_t rb = small_div(t, inv_nc), c = G.cl + (t - rb * G.NC);
+                    const uint32_t pa = (uint32_t)pa_q[q], rb = (uint32_t)rb_q[q], c = c_q[q];
                     for (uint32_t k = 0; k < cnt[q]; k++) {
                         const Rec r = rec[gs[q] + k];
                         if constexpr (MIXED) lds[run + k] = to_region<D>(r, (int)pa, (int)rb, G);
