@@ -292,6 +292,86 @@ def rms_rel_err(a, b):
     return float(np.sqrt(np.mean(np.sum((a - b) ** 2, axis=1))) / np.sqrt(np.mean(np.sum(b ** 2, axis=1))))
 
 
+def forces_at(wl, pos, precision):
+    """Forces of the workload's instances at positions `pos` (one context)."""
+    from cellular_raza_amd import MechanicsBackend
+
+    g = MechanicsBackend(wl.instances, dt=wl.dt, mechanics_solver_order=wl.solver_order, precision=precision)
+    g.set_agents(pos, None, wl.species, wl.counts)
+    f = g.compute_forces()
+    g.close()
+    return f
+
+
+def trajectory_error(wl, steps, seed=1):
+    """Mixed against FP64 from the same initial state after `steps` steps (same Philox
+    stream): max and RMS of |x_mixed - x_fp64| per agent (absolute, length units), and the
+    force error of the mixed path on the state the mixed trajectory reaches."""
+    from cellular_raza_amd import MechanicsBackend
+
+    end = {}
+    for precision in ("fp64", "mixed"):
+        g = MechanicsBackend(wl.instances, dt=wl.dt, mechanics_solver_order=wl.solver_order, noise=wl.noise,
+                             seed=seed, precision=precision, graph=True)
+        g.set_agents(wl.pos, wl.vel, wl.species, wl.counts)
+        g.step(steps)
+        end[precision] = g.get_state()[0]
+        g.close()
+    n = wl.n_agents
+    d = np.linalg.norm((end["mixed"] - end["fp64"]).reshape(n, -1), axis=1)
+    pm = end["mixed"]
+    f_m, f_d = forces_at(wl, pm, "mixed"), forces_at(wl, pm, "fp64")
+    return {"steps": steps, "position_max_abs": float(d.max()), "position_rms_abs": float(np.sqrt((d ** 2).mean())),
+            "force_rel_err_on_mixed_state": rel_err(f_m, f_d), "force_rms_rel_err_on_mixed_state": rms_rel_err(f_m, f_d)}
+
+
+def host_cpu():
+    """Model name and core counts of this host (lscpu)."""
+    import subprocess
+
+    info = {"logical_cpus": os.cpu_count()}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k in ("Model name", "CPU family", "Model", "Thread(s) per core", "Core(s) per socket", "Socket(s)",
+                     "CPU max MHz", "L3 cache"):
+                info[k] = v
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def cpu_calibration(threads):
+    """The reference's own benchmark workload (cell_sorting.rs sim-size, k = 6 and 8:
+    81,920 and 1,310,720 cells) on the oracle, 1 thread and `threads` threads, beside
+    the published 1-thread numbers (per executed step, 2.0 GHz desktop cores), and the
+    GPU (FP64 and mixed) on the same 1,310,720-cell workload."""
+    from cellular_raza_amd import workloads as W
+
+    published = {"81920": {"3700X-at-2000MHz": 2.11e4, "3960X-at-2000MHz": 2.30e4, "12700H-at-2000MHz": 2.06e4},
+                 "1310720": {"3700X-at-2000MHz": 2.02e4, "3960X-at-2000MHz": 2.22e4, "12700H-at-2000MHz": 2.04e4}}
+    out = {"host": host_cpu(), "published_1_thread_per_executed_step": published, "oracle": {}}
+    for k, steps1 in ((6, 10), (8, 2)):
+        wl = W.cell_sorting_benchmark(k)
+        row = {}
+        for t, st in ((1, steps1), (threads, 5 * steps1)):
+            v, sec = cpu_baseline(wl, t, st)
+            row[f"{t}_threads"] = {"agent_steps_per_s": v, "steps": st, "s": sec}
+            print(f"bench: calibration N={wl.n_agents} {t} threads {v:.3e} agent-steps/s", file=sys.stderr,
+                  flush=True)
+        out["oracle"][str(wl.n_agents)] = row
+    wl = W.cell_sorting_benchmark(8)
+    gpu = {}
+    for precision in ("fp64", "mixed"):
+        sec, _, _, _, _, _, _ = run_gpu(wl, precision, 200, 10, timing_steps=5)
+        gpu[precision] = wl.n_agents * 200 / sec
+    out["gpu_same_workload_agent_steps_per_s"] = gpu
+    out["gpu_mixed_over_published_3700X"] = gpu["mixed"] / published["1310720"]["3700X-at-2000MHz"]
+    return out
+
+
 # name -> (factory(args) -> Workload, mixed available, description, force kernel name for the profile lookup)
 def workload_table():
     from cellular_raza_amd import workloads as W
@@ -335,6 +415,11 @@ def main():
     ap.add_argument("--removal", action="store_true",
                     help="also time agent removal (CycleEvent::Remove): 10 %% of the workload's agents removed "
                          "between steps, three times")
+    ap.add_argument("--cpu-calibration", action="store_true",
+                    help="also time the oracle (1 thread and --cpu-threads) and the GPU on the reference's own "
+                         "cell-sorting benchmark workload (81,920 and 1,310,720 cells) beside its published numbers")
+    ap.add_argument("--traj-steps", type=int, default=0,
+                    help="steps of the mixed-vs-FP64 trajectory comparison (0: 500 for c5, 100 otherwise)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy (reference for the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -393,6 +478,15 @@ def main():
         extra["force_rel_err_mixed_vs_fp64"] = rel_err(f_main, f_fp64)
         extra["force_rms_rel_err_mixed_vs_fp64"] = rms_rel_err(f_main, f_fp64)
         extra["kernel_ms_fp64"] = {k: v["total_ms"] / max(v["launches"], 1) for k, v in tim_d.items()}
+        # the mixed path's force error on the state its own timed run ended in, and the
+        # trajectory error against FP64 from the same start
+        f_m, f_d = forces_at(wl, pos_main, "mixed"), forces_at(wl, pos_main, "fp64")
+        extra["force_rel_err_mixed_vs_fp64_at_end"] = rel_err(f_m, f_d)
+        extra["force_rms_rel_err_mixed_vs_fp64_at_end"] = rms_rel_err(f_m, f_d)
+        extra["mixed_end_state_step"] = args.warmup + args.steps
+        traj_steps = args.traj_steps or (500 if args.workload == "c5" else 100)
+        extra["trajectory_mixed_vs_fp64"] = trajectory_error(wl, traj_steps)
+        print(f"bench: trajectory {extra['trajectory_mixed_vs_fp64']}", file=sys.stderr, flush=True)
     if args.workload == "c4":
         sec_s = run_sequential(wl, main_prec, args.seq_steps, 5)
         seq_value = N * args.seq_steps / sec_s
@@ -425,20 +519,37 @@ def main():
         bf = force_bytes_per_agent(cbar, rec_bytes, D, 3 ** D, 8 * D)
     t_force_ms = tim_m["force"]["total_ms"] / max(tim_m["force"]["launches"], 1)
     achieved = bf * N / (t_force_ms * 1e-3) / 1e9
+    traffic = profile_figure("hbm_bytes_per_dispatch", force_kernel, args.workload)
+    step_bytes = None
+    if nv == 1:  # the step's byte model: force + integrator + counting sort (points)
+        langevin = wl.noise == "philox"
+        force_w = 4 * D if has_mixed else 8 * D
+        b_i = (8 * D + 8 * D + force_w + 32 * D + (8 * D if langevin else 0)) + (8 * D + 8 * D + 16 * D + 4)
+        n_vox = len(start) - 1
+        b_s = 4 + 4 + 8 * n_vox / N + 8 + 2 * (48 * D + 8)
+        step_bytes = bf + b_i + b_s
 
     cpu = None
     if not args.no_cpu:
         threads = min(args.cpu_threads, os.cpu_count() or 1)
         cpu_val, cpu_sec = cpu_baseline(wl, threads, args.cpu_steps)
+        one_steps = max(2, args.cpu_steps // 10)
+        cpu1_val, cpu1_sec = cpu_baseline(wl, 1, one_steps)
         cpu = {
             "value": cpu_val,
             "unit": "agent-steps/s",
             "cores": threads,
             "kind": "port",
             "sample": f"same {N}-agent workload, {args.cpu_steps} steps after 1 warm-up step, "
-                      f"chili restatement with {threads} subdomain threads ({cpu_sec:.1f} s)",
+                      f"chili restatement with {threads} subdomain threads ({cpu_sec:.1f} s); "
+                      f"1 thread: {one_steps} steps ({cpu1_sec:.1f} s)",
+            "value_1_thread": cpu1_val,
+            "host": host_cpu(),
         }
         print(f"bench: cpu {cpu_val:.4e} agent-steps/s on {threads} threads", file=sys.stderr, flush=True)
+
+    if args.cpu_calibration:
+        extra["cpu_calibration"] = cpu_calibration(min(args.cpu_threads, os.cpu_count() or 1))
 
     line = {
         "metric": "agent-steps/s on 1 MI355X, mixed-FP32 vs FP64 at <=1e-5 rel force err",
@@ -472,13 +583,19 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": profile_figure("hbm_bytes_per_dispatch", force_kernel, args.workload),
+            "traffic": traffic,
             "valu_issue_utilisation": profile_figure("valu_issue_utilisation", force_kernel, args.workload),
             "traffic_source": PROFILES[args.workload] + " (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated; "
                               "SQ_INSTS_VALU for the VALU issue utilisation)" if args.workload in PROFILES else None,
             "kernel": force_kernel,
             "bytes_per_agent": bf,
             "candidates_per_agent": cbar,
+            # achieved/frac use the byte model (every candidate record charged to HBM);
+            # the kernel serves candidates from LDS, so HBM is not what binds it
+            "binding_resource": "VALU issue" if nv == 1 else "VALU issue (divergent rod pairs)",
+            "hbm_traffic_frac": (traffic / (t_force_ms * 1e-3) / 1e9 / HBM_PEAK_GBS) if traffic else None,
+            "step_model_bytes_per_agent": step_bytes,
+            "step_model_frac": (step_bytes * value / 1e9 / HBM_PEAK_GBS) if step_bytes else None,
         },
         "cpu_baseline": cpu,
     }

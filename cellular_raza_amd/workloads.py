@@ -17,6 +17,7 @@ import numpy as np
 
 from .building_blocks import (
     CartesianCuboid,
+    CellSpecificInteraction,
     CellSpecificInteractionLangevin,
     Langevin,
     MiePotential,
@@ -190,6 +191,29 @@ def langevin_sorting_example(seed: int = 6, steps: int = 50_000, n_blue: int = 6
     species = np.where(np.arange(n) <= n_blue, 1, 0).astype(np.int32)
     return Workload("Langevin cell sorting example", [InstanceSpec(dom, pot, mech)], 0.25, steps, 2, pos,
                     np.zeros_like(pos), species, [n], noise="philox")
+
+
+def cell_sorting_benchmark(size_index: int = 8, seed: int = 7, steps: int = 10) -> Workload:
+    """The reference's own CPU benchmark (cellular_raza-benchmarks/src/cell_sorting.rs:
+    116-177, sim-size sweep 280-313): 2 x 10 * 4^k cells (indices 0..n1 BlueCell --
+    `n <= n_cells_1` makes n1 + 1 of them -- the rest RedCell) uniform in [0, L)^3 with
+    L = round(360 * 4^(k/3)) / 10, NewtonDamped3D (damping 2, mass 1), the benchmark's
+    CellSpecificInteraction (cell radius 6, strength 2, relative range 1.5), voxels from
+    the interaction range 18, dt 0.1, the default solver (AB3).  k = 8: 1,310,720 cells
+    in L = 1451.4, the largest size published for all three machines.  The benchmark
+    seeds ChaCha8; this draws the same distributions from numpy's generator."""
+    n1 = 10 * 4 ** size_index
+    n = 2 * n1
+    L = round(360.0 * 4.0 ** (size_index / 3.0)) * 0.1
+    dom = CartesianCuboid.from_boundaries_and_interaction_range([0.0] * 3, [L] * 3, 1.5 * 6.0 * 2.0)
+    pot = CellSpecificInteraction(cell_radius=6.0, potential_strength=2.0, relative_interaction_range=1.5,
+                                  variant="benchmark")
+    mech = NewtonDamped(damping_constant=2.0, mass=1.0)
+    rng = np.random.default_rng(seed)
+    pos = _uniform_open(rng, n, 3, L)
+    species = np.where(np.arange(n) <= n1, 1, 0).astype(np.int32)
+    return Workload("reference cell-sorting benchmark", [InstanceSpec(dom, pot, mech)], 0.1, steps, 2, pos,
+                    np.zeros_like(pos), species, [n])
 
 
 def c4_ensemble_2d(n_instances: int = 512, seed: int = 4, steps: int = 1000, n_min: int = 500,
