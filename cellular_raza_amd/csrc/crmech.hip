@@ -297,6 +297,14 @@ hipError_t scratch_address(int device, uint8_t*& out) {
 }
 
 void free_device(crm_ctx* ctx) {
+    // the graph executables first: their kernel nodes hold pointers into the buffers
+    // below, so they must not outlive them
+    for (auto& g : ctx->graph)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+    ctx->graphs_stale = false;
     void* ptrs[] = {ctx->d_work, ctx->d_tiles, ctx->d_start, ctx->d_count, ctx->d_bsum, ctx->d_key, ctx->d_rank,
                     ctx->d_perm, ctx->d_force, ctx->d_rec64, ctx->d_recq, ctx->d_noise, ctx->d_recrod, ctx->d_bnd,
                     ctx->d_sense, ctx->d_snap, ctx->d_snap_sp, ctx->d_snap_rod, ctx->d_snap_id, ctx->d_snap_st, ctx->d_divflag, ctx->d_drank, ctx->d_dctr,
@@ -338,12 +346,6 @@ void free_device(crm_ctx* ctx) {
     ctx->scan_tmp_bytes = 0;
     ctx->cap_agents = 0;
     ctx->noise_cap = 0;
-    for (auto& g : ctx->graph)
-        if (g) {
-            (void)hipGraphExecDestroy(g);
-            g = nullptr;
-        }
-    ctx->graphs_stale = false;
 }
 
 // d_count holds the arrivals per voxel, then (from this offset, 256-byte aligned)
@@ -412,8 +414,9 @@ crm_status t_end(crm_ctx* ctx, int cls) {
         if (s_ != CRM_OK) return s_;   \
     } while (0)
 
+// rank != 0: the force kernel also ranks the step's re-binning (keys binned ahead).
 template <int D, int POT>
-crm_status launch_force_dp(crm_ctx* ctx, StateBuf S) {
+crm_status launch_force_dp(crm_ctx* ctx, StateBuf S, uint32_t rank) {
     Arrays A = arrays(ctx);
     const uint32_t phys = ctx->cfg.weighting == CRM_WEIGHT_PHYSICAL;
     const uint32_t sym = ctx->symmetric;
@@ -423,30 +426,31 @@ crm_status launch_force_dp(crm_ctx* ctx, StateBuf S) {
     const bool mixed = ctx->cfg.precision == CRM_PRECISION_MIXED, sense = A.sense != nullptr;
     if (mixed && sense)
         hipLaunchKernelGGL((k_force_tiled<D, POT, true, true>), g, b, 0, ctx->stream, A, S, ctx->d_tiles, phys, sym,
-                           no_lds);
+                           no_lds, rank);
     else if (mixed)
         hipLaunchKernelGGL((k_force_tiled<D, POT, true, false>), g, b, 0, ctx->stream, A, S, ctx->d_tiles, phys, sym,
-                           no_lds);
+                           no_lds, rank);
     else if (sense)
         hipLaunchKernelGGL((k_force_tiled<D, POT, false, true>), g, b, 0, ctx->stream, A, S, ctx->d_tiles, phys, sym,
-                           no_lds);
+                           no_lds, rank);
     else
         hipLaunchKernelGGL((k_force_tiled<D, POT, false, false>), g, b, 0, ctx->stream, A, S, ctx->d_tiles, phys, sym,
-                           no_lds);
+                           no_lds, rank);
     return CRM_OK;
 }
 
 template <int D>
-crm_status launch_force_d(crm_ctx* ctx, StateBuf S) {
+crm_status launch_force_d(crm_ctx* ctx, StateBuf S, uint32_t rank) {
     switch (ctx->pot_kind) {
-        case CRM_POTENTIAL_MORSE: return launch_force_dp<D, CRM_POTENTIAL_MORSE>(ctx, S);
-        case CRM_POTENTIAL_MIE: return launch_force_dp<D, CRM_POTENTIAL_MIE>(ctx, S);
-        case CRM_POTENTIAL_BOUND_LENNARD_JONES: return launch_force_dp<D, CRM_POTENTIAL_BOUND_LENNARD_JONES>(ctx, S);
-        case CRM_POTENTIAL_MORSE_SPECIES: return launch_force_dp<D, CRM_POTENTIAL_MORSE_SPECIES>(ctx, S);
-        case CRM_POTENTIAL_CELL_SORTING: return launch_force_dp<D, CRM_POTENTIAL_CELL_SORTING>(ctx, S);
-        case CRM_POTENTIAL_NONE: return launch_force_dp<D, CRM_POTENTIAL_NONE>(ctx, S);
+        case CRM_POTENTIAL_MORSE: return launch_force_dp<D, CRM_POTENTIAL_MORSE>(ctx, S, rank);
+        case CRM_POTENTIAL_MIE: return launch_force_dp<D, CRM_POTENTIAL_MIE>(ctx, S, rank);
+        case CRM_POTENTIAL_BOUND_LENNARD_JONES:
+            return launch_force_dp<D, CRM_POTENTIAL_BOUND_LENNARD_JONES>(ctx, S, rank);
+        case CRM_POTENTIAL_MORSE_SPECIES: return launch_force_dp<D, CRM_POTENTIAL_MORSE_SPECIES>(ctx, S, rank);
+        case CRM_POTENTIAL_CELL_SORTING: return launch_force_dp<D, CRM_POTENTIAL_CELL_SORTING>(ctx, S, rank);
+        case CRM_POTENTIAL_NONE: return launch_force_dp<D, CRM_POTENTIAL_NONE>(ctx, S, rank);
         case CRM_POTENTIAL_CELL_SORTING_LANGEVIN:
-            return launch_force_dp<D, CRM_POTENTIAL_CELL_SORTING_LANGEVIN>(ctx, S);
+            return launch_force_dp<D, CRM_POTENTIAL_CELL_SORTING_LANGEVIN>(ctx, S, rank);
     }
     return CRM_INVALID_ARGUMENT;
 }
@@ -485,13 +489,17 @@ crm_status launch_force_rod_d(crm_ctx* ctx, StateBuf S) {
                                                      : launch_force_rod_dm<D, false>(ctx, S);
 }
 
-crm_status launch_force(crm_ctx* ctx) {
+// rank: point agents binned ahead -- the force kernel also ranks the step's
+// re-binning (only inside a step: a force evaluation on its own must leave the
+// histogram alone).
+crm_status launch_force(crm_ctx* ctx, bool rank = false) {
     CRM_TRY(t_begin(ctx));
     StateBuf S = ctx->buf[ctx->cur];
     ctx->sense_parity = ctx->cur;
     crm_status st;
+    const uint32_t r = rank ? 1u : 0u;
     if (is_rod(ctx)) st = ctx->D == 2 ? launch_force_rod_d<2>(ctx, S) : launch_force_rod_d<3>(ctx, S);
-    else st = ctx->D == 2 ? launch_force_d<2>(ctx, S) : launch_force_d<3>(ctx, S);
+    else st = ctx->D == 2 ? launch_force_d<2>(ctx, S, r) : launch_force_d<3>(ctx, S, r);
     if (st) return st;
     return t_end(ctx, CRM_KERNEL_FORCE);
 }
@@ -676,14 +684,15 @@ bool keys_ahead(const crm_ctx* ctx) {
 
 crm_status launch_step(crm_ctx* ctx, const double* noise_step) {
     const StepArgs sa = step_args(ctx, noise_step);
-    CRM_TRY(launch_force(ctx));
-    if (keys_ahead(ctx)) {
+    const bool fused_rank = keys_ahead(ctx) && !is_rod(ctx);  // the force tiles rank their voxels
+    CRM_TRY(launch_force(ctx, fused_rank));
+    if (!keys_ahead(ctx)) {
+        CRM_TRY(launch_key(ctx, sa));
+    } else if (!fused_rank) {
         CRM_TRY(t_begin(ctx));
         hipLaunchKernelGGL(k_rank_ahead, dim3((uint32_t)((ctx->Vt + kBlock - 1) / kBlock)), dim3(kBlock), 0,
                            ctx->stream, arrays(ctx));
         CRM_TRY(t_end(ctx, CRM_KERNEL_INTEGRATE));
-    } else {
-        CRM_TRY(launch_key(ctx, sa));
     }
     CRM_TRY(launch_sort(ctx, true, &sa));
     return CRM_OK;
@@ -2254,7 +2263,10 @@ size_t crm_last_error(crm_ctx* ctx, char* buf, size_t len) {
 
 void crm_destroy(crm_ctx* ctx) {
     if (!ctx) return;
-    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->stream) {
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+    }
     free_device(ctx);
     if (ctx->d_ip) (void)hipFree(ctx->d_ip);
     if (ctx->d_ctrl) (void)hipFree(ctx->d_ctrl);

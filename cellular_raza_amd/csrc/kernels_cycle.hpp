@@ -147,11 +147,15 @@ __global__ __launch_bounds__(kBlock) void k_cycle_voxels(Arrays A, StateBuf S, C
         Y.ndiv[g] = 0;  // the scan's last element: its exclusive sum is the total
         return;
     }
-    uint64_t local = g;
-    for (uint32_t i = 0; i < Y.n_inst; i++) {
-        const InstParams& P = A.ip[i];
-        if (g >= P.voxel_offset && g < P.voxel_offset + P.n_voxels) local = g - P.voxel_offset;
+    // the voxel's plain index in its instance: the last instance whose first voxel
+    // is at or below g (voxel offsets ascend with the instance)
+    uint32_t lo = 0, hi = Y.n_inst;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (A.ip[mid].voxel_offset <= g) lo = mid;
+        else hi = mid;
     }
+    const uint64_t local = g - A.ip[lo].voxel_offset;
     const uint64_t N = A.N;
     const uint32_t b = A.start[g], e = A.start[g + 1];
     const uint64_t ctr = Y.vcount[g];

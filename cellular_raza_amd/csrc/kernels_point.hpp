@@ -715,7 +715,7 @@ template <int D, int POT, bool MIXED, bool SENSE>
 __global__ __launch_bounds__(kBlock, MIXED ? 5 : 4) void k_force_tiled(Arrays A, StateBuf S,
                                                                      const ForceTile* __restrict__ tiles,
                                                                      uint32_t physical, uint32_t symmetric,
-                                                                     uint32_t no_lds) {
+                                                                     uint32_t no_lds, uint32_t rank) {
     using Rec = typename RecOf<MIXED>::type;
     constexpr int CAP = MIXED ? kLdsRecordsMixed : kLdsRecordsFP64;
     constexpr uint32_t NP = D == 3 ? 3u : 1u;         // staged planes
@@ -752,6 +752,29 @@ __global__ __launch_bounds__(kBlock, MIXED ? 5 : 4) void k_force_tiled(Arrays A,
     const uint32_t n_vox = G.RW * G.NC * NP;
     const uint32_t n_ownv = R * W;
     const bool fits = !no_lds && n_vox < (uint32_t)kLdsVoxelStarts && n_ownv <= (uint32_t)kOwnVoxMax;
+
+    // The step's re-binning ranks for the tile's own voxels (k_rank_ahead's work;
+    // the tiles cover every voxel once): an agent whose next voxel, binned ahead by
+    // the previous gather, is its current one takes the next stayer rank in slot
+    // order, the others an atomic arrival rank in their next voxel.  The force
+    // computation reads none of key, rank, stay or count, so each wave does this
+    // after its agents' forces, beside the waves still computing theirs.
+    auto rank_tile = [&]() {
+        if (!rank) return;
+        const float inv_w = 1.0f / (float)W;
+        for (uint32_t u = threadIdx.x; u < n_ownv; u += kBlock) {
+            const uint32_t r = small_div(u, inv_w), c = u - r * W;
+            const uint32_t g = voff + (T.a0 * n1 + T.b0 + r) * n2 + T.c0 + c;
+            const uint32_t b = A.start[g], e = A.start[g + 1];
+            uint32_t n = 0;
+            for (uint32_t s = b; s < e; s++) {
+                const uint32_t k = A.key[s];
+                if (k == g) A.rank[s] = n++;
+                else A.rank[s] = kMoved | atomicAdd(&A.count[k], 1u);
+            }
+            A.stay[g] = n;
+        }
+    };
 
     if (fits) {
         // staged voxel v = (rb * NC + c - cl) * NP + pa; thread t holds v = 3t .. 3t + 2
@@ -919,6 +942,7 @@ __global__ __launch_bounds__(kBlock, MIXED ? 5 : 4) void k_force_tiled(Arrays A,
                     agent_force<D, POT, MIXED, SENSE>(A, S, P, e.slot, e.self, lds[e.self], Span{e.ovb, e.ove}, w,
                                                       physical, symmetric, src, unit);
                 }
+                rank_tile();
                 return;
             }
             for (uint32_t t = threadIdx.x; t < n_own; t += kBlock) {
@@ -928,6 +952,7 @@ __global__ __launch_bounds__(kBlock, MIXED ? 5 : 4) void k_force_tiled(Arrays A,
                 const uint32_t s = own_agent(t, self, ov, w);
                 agent_force<D, POT, MIXED, SENSE>(A, S, P, s, self, lds[self], ov, w, physical, symmetric, src, unit);
             }
+            rank_tile();
             return;
         }
     }
@@ -960,6 +985,7 @@ __global__ __launch_bounds__(kBlock, MIXED ? 5 : 4) void k_force_tiled(Arrays A,
             agent_force<D, POT, MIXED, SENSE>(A, S, P, s, s, src.record(s, wm), ov, w, physical, symmetric, src, unit);
         }
     }
+    rank_tile();
 }
 
 // ---------------------------------------------------------------------------

@@ -49,14 +49,17 @@ __global__ __launch_bounds__(kBlock) void k_remove_mark(Arrays A, StateBuf S, Re
     const bool rm = X.flag[S.id[q]] != 0;
     X.keep_slot[q] = rm ? 0u : 1u;
     if (!rm) return;
+    // the instance whose voxel range holds the agent's voxel: the last instance
+    // whose first voxel is at or below it (voxel offsets ascend with the instance)
     const uint32_t g = S.vox[q];
-    for (uint32_t i = 0; i < X.n_inst; i++) {
-        const InstParams& P = A.ip[i];
-        if (g >= P.voxel_offset && g < P.voxel_offset + P.n_voxels) {
-            atomicAdd(&X.inst_rm[i], 1u);
-            break;
-        }
+    uint32_t lo = 0, hi = X.n_inst;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (A.ip[mid].voxel_offset <= g) lo = mid;
+        else hi = mid;
     }
+    const InstParams& P = A.ip[lo];
+    if (g >= P.voxel_offset && g < P.voxel_offset + P.n_voxels) atomicAdd(&X.inst_rm[lo], 1u);
 }
 
 // Survivor in slot s of src (stride n_old) -> slot slot_before[s] of dst (stride
