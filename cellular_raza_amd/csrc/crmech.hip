@@ -616,7 +616,7 @@ crm_status launch_sort(crm_ctx* ctx, bool advance, const StepArgs* sa) {
     CRM_TRY(t_begin(ctx));
     hipLaunchKernelGGL(k_scan_local, dim3(ctx->n_tiles), dim3(kBlock), 0, ctx->stream, ctx->d_count,
                        ctx->d_count + stay_offset(ctx), ctx->d_start,
-                       ctx->d_bsum, Vt);
+                       ctx->d_bsum, Vt, ctx->d_ctrl);
     hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->d_bsum, ctx->n_tiles, ctx->d_start,
                        Vt, ctx->d_ctrl, (uint32_t)(advance ? 1 : 0));
     CRM_TRY(t_end(ctx, CRM_KERNEL_SCAN));
@@ -955,7 +955,9 @@ crm_status launch_cycle_step(crm_ctx* ctx, const double* noise_step) {
     CRM_HIP(hipMemcpyAsync(inst_div.data(), ctx->d_inst_div, ctx->n_inst * sizeof(uint32_t), hipMemcpyDeviceToHost,
                            ctx->stream));
     CRM_HIP(hipStreamSynchronize(ctx->stream));
-    if (c.error_flags) return check_errors(ctx, "step");
+    // an error of this step lets the step end (as the plain step does); one of
+    // the step before halted this step's kernels
+    if (c.error_flags && (c.error_key >> 32) < ctx->step) return check_errors(ctx, "step");
     const uint64_t n_old = ctx->N, n_new = n_old + c.n_div;
     if (c.n_div == 0) {
         ctx->cur = 1 - ctx->cur;  // the updated state is in the other buffer
@@ -1672,10 +1674,21 @@ crm_status crm_step(crm_ctx* ctx, uint32_t n_steps, const double* noise) {
         }
     }
     for (; s < n_steps; s++) CRM_TRY(launch_step(ctx, use_noise_buf ? ctx->d_noise + per_step * s : nullptr));
-    CRM_HIP(hipStreamSynchronize(ctx->stream));
-    ctx->step += n_steps;
+    Control c;
+    CRM_TRY(read_control(ctx, c));
+    // The steps after a failing one were launched but did nothing: the state is
+    // the failing step's end, in the buffer that step wrote.
+    uint64_t done = n_steps;
+    if (c.error_flags) {
+        const uint64_t t_err = c.error_key >> 32;
+        if (t_err >= ctx->step && t_err < ctx->step + n_steps) {
+            done = t_err + 1 - ctx->step;
+            if ((n_steps - done) & 1) ctx->cur = 1 - ctx->cur;
+        }
+    }
+    ctx->step += done;
     // the last step's forces (and counts) read the buffer before its gather
-    if (n_steps > 0) ctx->sense_parity = 1 - ctx->cur;
+    if (done > 0) ctx->sense_parity = 1 - ctx->cur;
     return check_errors(ctx, "step");
 }
 

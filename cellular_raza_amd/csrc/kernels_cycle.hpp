@@ -71,6 +71,7 @@ template <int D>
 __global__ __launch_bounds__(kBlock) void k_rod_mech(Arrays A, StateBuf src, StateBuf dst, RodArgs R, StepArgs sa,
                                                       CycleArgs Y) {
     __shared__ double win[kRodWinK];
+    if (halted(A.ctrl, A.ctrl->step)) return;
     const BlockWork w = A.work[blockIdx.x];
     uint64_t s;
     uint32_t v;

@@ -81,6 +81,14 @@ __device__ __forceinline__ void raise_error(Control* c, uint32_t kind, uint64_t 
     atomicMin((unsigned long long*)&c->error_key, k);
 }
 
+// An earlier step than t raised an error.  The kernels of the steps after a
+// failing one do nothing (the ones before the sort check this, the ones after it
+// read Control::skip, set by k_scan_tiles), so the state stays at the end of the
+// failing step, where run_simulation! returns the error (run_sim.rs:380-403).
+__device__ __forceinline__ bool halted(const Control* c, uint64_t t) {
+    return c->error_flags != 0 && (c->error_key >> 32) < t;
+}
+
 template <typename T>
 __device__ __forceinline__ const PotParams<T>& pot_of(const InstParams& P);
 template <>
@@ -719,6 +727,7 @@ __global__ __launch_bounds__(kBlock, MIXED ? 5 : 4) void k_force_tiled(Arrays A,
     using Rec = typename RecOf<MIXED>::type;
     constexpr int CAP = MIXED ? kLdsRecordsMixed : kLdsRecordsFP64;
     constexpr uint32_t NP = D == 3 ? 3u : 1u;         // staged planes
+    if (halted(A.ctrl, A.ctrl->step)) return;
     constexpr uint32_t PA_OWN = D == 3 ? 1u : 0u;     // the tile's own plane among them
     __shared__ Rec lds[CAP];
     __shared__ uint16_t vst[kLdsVoxelStarts];        // LDS index of each staged voxel's first agent
@@ -1164,6 +1173,7 @@ __device__ __forceinline__ bool point_step(const InstParams& P, const StepArgs& 
 // position whose index has no voxel keeps the old one (update_mechanics.rs:430-453).
 template <int D, int MECH>
 __global__ __launch_bounds__(kBlock) void k_key_point(Arrays A, StateBuf S, StepArgs sa) {
+    if (halted(A.ctrl, A.ctrl->step)) return;
     const BlockWork w = A.work[blockIdx.x];
     if (threadIdx.x >= w.count) return;
     const uint64_t s = w.first + threadIdx.x;
@@ -1253,7 +1263,8 @@ constexpr int kScanTile = kBlock * kScanItems;
 __global__ __launch_bounds__(kBlock) void k_scan_local(const uint32_t* __restrict__ count,
                                                         const uint32_t* __restrict__ stay,
                                                         uint32_t* __restrict__ start, uint32_t* __restrict__ bsum,
-                                                        uint32_t Vt) {
+                                                        uint32_t Vt, const Control* ctrl) {
+    if (halted(ctrl, ctrl->step)) return;
     static_assert(kScanItems == 8, "two uint4 per group");
     const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
     const bool whole = base + kScanItems <= Vt;
@@ -1306,6 +1317,10 @@ __global__ __launch_bounds__(kBlock) void k_scan_local(const uint32_t* __restric
 __global__ __launch_bounds__(kBlock) void k_scan_tiles(uint32_t* __restrict__ bsum, uint32_t n_tiles,
                                                         uint32_t* __restrict__ start, uint32_t Vt, Control* ctrl,
                                                         uint32_t advance) {
+    const bool skip = halted(ctrl, ctrl->step);
+    __syncthreads();  // every thread has read the control block before thread 0 writes it
+    if (threadIdx.x == 0) ctrl->skip = skip ? 1u : 0u;
+    if (skip) return;
     uint32_t carry = 0;
     for (uint32_t base = 0; base < n_tiles; base += kBlock) {
         uint32_t q = base + threadIdx.x;
@@ -1324,6 +1339,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_tiles(uint32_t* __restrict__ bs
 // Between k_scan_local and k_fixup, start[v] holds v's offset inside its scan
 // tile and tile_off[v / kScanTile] (scanned by k_scan_tiles) the tile's offset.
 __global__ __launch_bounds__(kBlock) void k_scatter(Arrays A) {
+    if (A.ctrl->skip) return;
     const uint64_t s = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     if (s >= A.N) return;
     const uint32_t k = A.key[s], r = A.rank[s];
@@ -1340,6 +1356,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter(Arrays A) {
 // update_mechanics.rs:418-513).  Set-up sorts rank every agent atomically with
 // stay = 0 and get the whole voxel in order of previous slot.
 __global__ __launch_bounds__(kBlock) void k_fixup(Arrays A) {
+    if (A.ctrl->skip) return;
     const uint64_t v = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     if (v >= A.Vt) return;
     const uint32_t stay = A.stay[v];
@@ -1428,6 +1445,7 @@ __device__ __forceinline__ void bin_ahead(const Arrays& A, const InstParams& P, 
 // among the arrivals of their next voxel (count), flagged with kMoved, which the
 // scatter places after that voxel's stayers; k_fixup puts the arrivals in order.
 __global__ __launch_bounds__(kBlock) void k_rank_ahead(Arrays A) {
+    if (halted(A.ctrl, A.ctrl->step)) return;
     const uint64_t v = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     if (v >= A.Vt) return;
     const uint32_t b = A.start[v], e = A.start[v + 1];
@@ -1445,6 +1463,7 @@ __global__ __launch_bounds__(kBlock) void k_rank_ahead(Arrays A) {
 // With AHEAD the next step's voxels are binned as well.
 template <int D, bool MIXED, bool AHEAD, int MECH>
 __global__ __launch_bounds__(kBlock) void k_gather_point(Arrays A, StateBuf src, StateBuf dst, StepArgs sa) {
+    if (A.ctrl->skip) return;
     const BlockWork w = A.work[blockIdx.x];
     if (threadIdx.x >= w.count) return;
     const uint64_t s = w.first + threadIdx.x;
@@ -1490,6 +1509,7 @@ __global__ __launch_bounds__(kBlock) void k_gather_point(Arrays A, StateBuf src,
 template <int D, bool MIXED, int MECH>
 __global__ __launch_bounds__(kBlock) void k_gather_integrate_point(Arrays A, StateBuf src, StateBuf dst,
                                                                      StepArgs sa) {
+    if (A.ctrl->skip) return;
     const BlockWork w = A.work[blockIdx.x];
     if (threadIdx.x >= w.count) return;
     const uint64_t s = w.first + threadIdx.x;

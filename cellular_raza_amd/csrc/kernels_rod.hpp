@@ -55,6 +55,7 @@ __device__ __forceinline__ bool rod_lane(const BlockWork& w, uint32_t nv, uint64
 // ---------------------------------------------------------------------------
 template <int D, int POT, bool MIXED>
 __global__ __launch_bounds__(kBlock) void k_force_rod(Arrays A, StateBuf S, RodArgs R, uint32_t physical) {
+    if (halted(A.ctrl, A.ctrl->step)) return;
     const BlockWork w = A.work[blockIdx.x];
     uint64_t s;
     uint32_t v;
@@ -619,6 +620,7 @@ constexpr int kRodLdsK = kBlock * 3;  // one new position per lane
 // first lane sums them in vertex order and divides by nv (row_sum / nrows).
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_key_rod(Arrays A, StateBuf S, RodArgs R, StepArgs sa) {
+    if (halted(A.ctrl, A.ctrl->step)) return;
     __shared__ double newpos[kRodLdsK];
     __shared__ uint32_t outflag[kBlock];
     const BlockWork w = A.work[blockIdx.x];
@@ -721,6 +723,7 @@ __global__ __launch_bounds__(kBlock) void k_gather_rod(Arrays A, StateBuf src, S
     __shared__ double newpos[kRodLdsK];
     __shared__ double win[INTEGRATE ? kRodWinK : 1];
     __shared__ double aheadpos[AHEAD ? kRodLdsK : 1];
+    if (A.ctrl->skip) return;
     const BlockWork w = A.work[blockIdx.x];
     uint64_t s;
     uint32_t v;

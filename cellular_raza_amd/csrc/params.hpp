@@ -77,6 +77,8 @@ struct Control {
     uint32_t error_flags;   // bit (1 << crm_status) per error kind seen
     uint32_t n_div;         // rods dividing in the current step (cycle)
     uint64_t out_of_grid;   // agents that kept their voxel because the new index had no voxel
+    uint32_t skip;          // set by k_scan_tiles: this step's sort is skipped (an earlier step failed)
+    uint32_t pad;
 };
 
 }  // namespace crm

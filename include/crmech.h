@@ -238,7 +238,11 @@ crm_status crm_set_agents(crm_ctx* ctx, const uint64_t* counts, const double* po
 
 /* Advances n_steps steps: forces -> integrator -> boundary -> re-binning
  * (run_sim.rs:380-403).  noise: CRM_NOISE_BUFFER only, n_steps * n_agents *
- * n_vertices * dim standard normals, step-major, by agent index. */
+ * n_vertices * dim standard normals, step-major, by agent index.  A step that
+ * fails (CalcError, BoundaryError, DivisionError) ends, and no later step runs:
+ * the call returns the error, crm_get_info reports the failing step and agent,
+ * the step counter and the state (positions, history, voxel lists) are those at
+ * the failing step's end, and the context accepts no further step. */
 crm_status crm_step(crm_ctx* ctx, uint32_t n_steps, const double* noise);
 
 /* Interaction force accumulated for the current state, by agent index

@@ -331,6 +331,55 @@ def test_boundary_error_same_step_as_oracle():
     assert e.value.kind == eo.value.kind == "BoundaryError"
     inf = g.info()
     assert inf.error_step_valid and inf.error_step == 0 and inf.error_agent == 1
+    assert inf.step == 1  # the two steps after the failing one did not run
+
+
+@pytest.mark.parametrize("graph", [True, False])
+@pytest.mark.parametrize("case", ["newton", "brownian"])
+def test_error_halts_at_the_failing_step(case, graph):
+    """The step loop breaks at the first step that fails (run_sim.rs:380-403): a
+    10-step call whose step 4 (NewtonDamped, anti-damped so the speed grows; agents
+    binned ahead by the gather) or step 3 (Brownian, binned after the forces) leaves
+    the domain runs no later step, in graph replays of two steps as in single
+    launches.  The failing step is the oracle's, the step count stops at its end, and
+    the state equals that of a context stepped one step at a time up to the failure."""
+    dom = CartesianCuboid.from_boundaries_and_interaction_range([0.0, 0.0], [10.0, 10.0], 2.0)
+    rng = np.random.default_rng(1)
+    if case == "newton":
+        spec = [InstanceSpec(dom, MorsePotential(1.0, 1.0, 2.0, 1.0), NewtonDamped(-5.0, 1.0))]
+        pos, vel = rng.uniform(1, 9, (20, 2)), rng.normal(0, 3, (20, 2))
+        kw, t_err = {}, 4
+    else:
+        spec = [InstanceSpec(dom, NoInteraction(), Brownian(100.0, 1.0))]
+        pos, vel = rng.uniform(1, 9, (50, 2)), None
+        kw, t_err = dict(noise="philox", seed=5), 3
+    o = OracleSim(spec, dt=0.1, mechanics_solver_order=2, **kw)
+    o.set_agents(pos, vel)
+    o.step(t_err)
+    with pytest.raises(OracleError) as eo:
+        o.step(1)
+    g = MechanicsBackend(spec, dt=0.1, mechanics_solver_order=2, graph=graph, **kw)
+    g.set_agents(pos, vel)
+    with pytest.raises(SimulationError) as e:
+        g.step(10)
+    assert e.value.kind == eo.value.kind == "BoundaryError"
+    inf = g.info()
+    assert inf.error_step_valid and inf.error_step == t_err and inf.step == t_err + 1
+    one = MechanicsBackend(spec, dt=0.1, mechanics_solver_order=2, graph=False, **kw)
+    one.set_agents(pos, vel)
+    for _ in range(t_err):
+        one.step(1)
+    with pytest.raises(SimulationError):
+        one.step(1)
+    assert one.info().step == t_err + 1
+    for a, b in zip(g.get_state(), one.get_state()):
+        assert np.array_equal(a, b)
+    ha, hb = g.get_history(), one.get_history()
+    assert ha[2] == hb[2] == 2
+    assert np.array_equal(ha[0], hb[0]) and np.array_equal(ha[1], hb[1])
+    assert np.array_equal(g.voxel_order(), one.voxel_order())
+    g.close()
+    one.close()
 
 
 def test_mie_identical_positions_calc_error():

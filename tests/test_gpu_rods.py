@@ -206,6 +206,7 @@ def test_rod_boundary_error_same_step_as_oracle():
     assert e.value.kind == eo.value.kind == "BoundaryError"
     inf = g.info()
     assert inf.error_step_valid and inf.error_step == 0 and inf.error_agent == 1
+    assert inf.step == 1  # the two steps after the failing one did not run
 
 
 @pytest.mark.parametrize("pot", ["morse", "mie", "blj", "species"])
