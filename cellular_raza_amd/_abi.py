@@ -196,7 +196,7 @@ SIGNATURES = {
     "crm_get_info": (C.c_int, [P, C.POINTER(Info)]),
     "crm_autotune": (C.c_int, [P, C.c_char_p, C.POINTER(C.c_uint32)]),
     "crm_get_tiling": (C.c_int, [P, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
-                                 C.POINTER(C.c_uint32)]),
+                                 C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "crm_set_neighbor_sensing": (C.c_int, [P, C.c_uint32, C.c_double]),
     "crm_get_neighbor_counts": (C.c_int, [P, U32P]),
     "crm_get_snapshot": (C.c_int, [P, DP, C.POINTER(C.c_int32), U32P]),

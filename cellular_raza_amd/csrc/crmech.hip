@@ -202,6 +202,7 @@ struct crm_ctx {
     uint32_t n_force_tiles = 0;
     size_t tiles_cap = 0;
     uint32_t policy = 0;                  // tiling policy (index into kTilePolicies)
+    uint32_t lanes_log2 = 0;              // lanes per own agent in the force kernel (policy's)
     std::vector<uint32_t> tile_rows, tile_cols, tile_count;  // per instance
     Control* d_ctrl = nullptr;
     uint32_t* d_start = nullptr;
@@ -426,16 +427,16 @@ crm_status launch_force_dp(crm_ctx* ctx, StateBuf S, uint32_t rank) {
     const bool mixed = ctx->cfg.precision == CRM_PRECISION_MIXED, sense = A.sense != nullptr;
     if (mixed && sense)
         hipLaunchKernelGGL((k_force_tiled<D, POT, true, true>), g, b, 0, ctx->stream, A, S, ctx->d_tiles, phys, sym,
-                           no_lds, rank);
+                           no_lds, rank, ctx->lanes_log2);
     else if (mixed)
         hipLaunchKernelGGL((k_force_tiled<D, POT, true, false>), g, b, 0, ctx->stream, A, S, ctx->d_tiles, phys, sym,
-                           no_lds, rank);
+                           no_lds, rank, ctx->lanes_log2);
     else if (sense)
         hipLaunchKernelGGL((k_force_tiled<D, POT, false, true>), g, b, 0, ctx->stream, A, S, ctx->d_tiles, phys, sym,
-                           no_lds, rank);
+                           no_lds, rank, ctx->lanes_log2);
     else
         hipLaunchKernelGGL((k_force_tiled<D, POT, false, false>), g, b, 0, ctx->stream, A, S, ctx->d_tiles, phys, sym,
-                           no_lds, rank);
+                           no_lds, rank, ctx->lanes_log2);
     return CRM_OK;
 }
 
@@ -759,9 +760,11 @@ crm_status reset_control(crm_ctx* ctx, uint64_t step) {
 // region.  Policy 0 is the default.
 struct TilePolicy {
     double own_frac, margin;
+    uint32_t lanes_log2;  // lanes per own agent: 1, 2 or 4 (agents per wavefront 64, 32, 16)
 };
-constexpr TilePolicy kTilePolicies[] = {{0.94, 1.2}, {0.94, 1.1}, {0.94, 1.4}, {0.80, 1.2},
-                                        {0.70, 1.2}, {0.55, 1.2}, {0.40, 1.2}, {0.70, 1.4}};
+constexpr TilePolicy kTilePolicies[] = {{0.94, 1.2, 0}, {0.94, 1.1, 0}, {0.94, 1.4, 0}, {0.80, 1.2, 0},
+                                        {0.70, 1.2, 0}, {0.55, 1.2, 0}, {0.40, 1.2, 0}, {0.70, 1.4, 0},
+                                        {0.94, 1.2, 1}, {0.70, 1.2, 1}, {0.94, 1.2, 2}, {0.70, 1.2, 2}};
 constexpr uint32_t kNumTilePolicies = sizeof(kTilePolicies) / sizeof(kTilePolicies[0]);
 
 void build_tiles(const InstParams& P, uint32_t dim, uint32_t inst, uint64_t n_agents, uint32_t cap,
@@ -770,7 +773,7 @@ void build_tiles(const InstParams& P, uint32_t dim, uint32_t inst, uint64_t n_ag
     const double rho = (double)n_agents / ((double)n0 * n1 * n2);
     const uint32_t np = dim == 3 ? 3u : 1u;  // staged planes
     const double planes = (double)std::min(np, n0 > 1 ? 3u : 1u);
-    const double margin = pol.margin, own_max = pol.own_frac * kBlock;
+    const double margin = pol.margin, own_max = pol.own_frac * kBlock / (double)(1u << pol.lanes_log2);
     uint32_t bR = 1, bW = 1;
     double best_own = -1.0, best_staged = 0.0;
     for (uint32_t r0 = 1; r0 <= std::min<uint32_t>(n1, kTileRowsMax); r0++) {
@@ -819,6 +822,7 @@ void make_tiles(crm_ctx* ctx, uint32_t pol, std::vector<ForceTile>& tiles) {
                     ctx->tile_cols[i]);
         ctx->tile_count[i] = (uint32_t)(tiles.size() - before);
     }
+    ctx->lanes_log2 = kTilePolicies[pol].lanes_log2;
 }
 
 crm_status upload_tiles(crm_ctx* ctx, const std::vector<ForceTile>& tiles) {
@@ -2217,11 +2221,13 @@ crm_status crm_save_cells_json(crm_ctx* ctx, const char* path, uint64_t iteratio
     return CRM_OK;
 }
 
-crm_status crm_get_tiling(crm_ctx* ctx, uint32_t inst, uint32_t* rows, uint32_t* cols, uint32_t* n_tiles) {
+crm_status crm_get_tiling(crm_ctx* ctx, uint32_t inst, uint32_t* rows, uint32_t* cols, uint32_t* n_tiles,
+                          uint32_t* lanes_per_agent) {
     if (!ctx || !ctx->have_agents || inst >= ctx->n_inst || is_rod(ctx)) return CRM_INVALID_ARGUMENT;
     if (rows) *rows = ctx->tile_rows[inst];
     if (cols) *cols = ctx->tile_cols[inst];
     if (n_tiles) *n_tiles = ctx->tile_count[inst];
+    if (lanes_per_agent) *lanes_per_agent = 1u << ctx->lanes_log2;
     return CRM_OK;
 }
 

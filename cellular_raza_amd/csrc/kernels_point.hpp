@@ -308,11 +308,16 @@ struct Row4 {
 // exactly zero, so the sums are those of the full loop without its zero terms.
 // The wave pays max-over-lanes of the marked count for the expensive pass
 // instead of max-over-lanes of the candidate count.
+//
+// Lanes per agent (LDS source): 2^lg consecutive lanes of a wave share one agent;
+// lane `sub` of the group walks the sub-th of 2^lg equal parts of the stream, and
+// the group adds its partial sums with wavefront shuffles (a butterfly, so every
+// lane of the group holds the same total) before lane 0 writes the force.
 template <int D, int POT, bool MIXED, bool SENSE, typename Src>
 __device__ __forceinline__ void agent_force(const Arrays& A, const StateBuf& S, const InstParams& P, uint64_t s,
                                             uint32_t self, const typename RecOf<MIXED>::type me, Span ov,
                                             typename Src::W cur, uint32_t physical, uint32_t symmetric,
-                                            const Src& src, const float* unit) {
+                                            const Src& src, const float* unit, uint32_t sub = 0, uint32_t lg = 0) {
     using T = typename std::conditional<MIXED, float, double>::type;
     using Rec = typename RecOf<MIXED>::type;
     using W = typename Src::W;
@@ -582,8 +587,11 @@ __device__ __forceinline__ void agent_force(const Arrays& A, const StateBuf& S, 
             for (uint32_t q = 0; q < q1; q++) mlo |= marked(cub, p0 + q) << q;
             for (uint32_t q = 32; q < qn; q++) mhi |= marked(cub, p0 + q) << (q - 32u);
         };
-        for (uint32_t p0 = 0; p0 < L; p0 += 64u) {
-            const uint32_t qn = min(64u, L - p0);
+        // this lane's part of the stream
+        const uint32_t part = (L + (1u << lg) - 1u) >> lg;
+        const uint32_t pb = min(L, sub * part), pe = min(L, pb + part);
+        for (uint32_t p0 = pb; p0 < pe; p0 += 64u) {
+            const uint32_t qn = min(64u, pe - p0);
             uint32_t mlo = 0, mhi = 0;
             if (cubic) pass1(std::true_type{}, p0, qn, mlo, mhi);
             else pass1(std::false_type{}, p0, qn, mlo, mhi);
@@ -645,13 +653,27 @@ __device__ __forceinline__ void agent_force(const Arrays& A, const StateBuf& S, 
         }
     }
     }
+    double fsum[D];
+#pragma unroll
+    for (int k = 0; k < D; k++) fsum[k] = MIXED ? (double)ks[k] + (double)(-kc[k]) : accd[k];
+    if (lg) {  // the group's partial sums, by xor butterfly (same order on every lane)
+        uint32_t err = calc_err ? 1u : 0u;
+        for (uint32_t o = 1; o < (1u << lg); o <<= 1) {
+#pragma unroll
+            for (int k = 0; k < D; k++) fsum[k] = fsum[k] + __shfl_xor(fsum[k], (int)o, 64);
+            err |= (uint32_t)__shfl_xor((int)err, (int)o, 64);
+            if constexpr (SENSE) n_sense += (uint32_t)__shfl_xor((int)n_sense, (int)o, 64);
+        }
+        calc_err = err != 0;
+        if (sub) return;
+    }
     if (calc_err) raise_error(A.ctrl, CRM_CALC_ERROR, A.ctrl->step, S.id[s]);
     // the domain force follows the pair forces (calculate_custom_domain_force after
     // the subdomain's pairs, run_sim.rs:277-279, update_mechanics.rs:244-274)
     const bool spring = P.dforce == CRM_DOMAIN_FORCE_SPRING;
 #pragma unroll
     for (int k = 0; k < D; k++) {
-        double f = MIXED ? (double)ks[k] + (double)(-kc[k]) : accd[k];
+        double f = fsum[k];
         if (spring) f = f + (-P.dk[k]) * (S.pos[(uint64_t)k * N + s] - P.dc[k]);
         A.force[(uint64_t)k * N + s] = f;
     }
@@ -723,7 +745,7 @@ template <int D, int POT, bool MIXED, bool SENSE>
 __global__ __launch_bounds__(kBlock, MIXED ? 5 : 4) void k_force_tiled(Arrays A, StateBuf S,
                                                                      const ForceTile* __restrict__ tiles,
                                                                      uint32_t physical, uint32_t symmetric,
-                                                                     uint32_t no_lds, uint32_t rank) {
+                                                                     uint32_t no_lds, uint32_t rank, uint32_t lg) {
     using Rec = typename RecOf<MIXED>::type;
     constexpr int CAP = MIXED ? kLdsRecordsMixed : kLdsRecordsFP64;
     constexpr uint32_t NP = D == 3 ? 3u : 1u;         // staged planes
@@ -905,7 +927,8 @@ __global__ __launch_bounds__(kBlock, MIXED ? 5 : 4) void k_force_tiled(Arrays A,
             // change the agent's sums.
             // The sorted case looks each own agent up once and keeps the result in
             // LDS: self, own voxel range, the three stream ranges and the slot.
-            const bool sorted = n_own <= (uint32_t)kBlock;
+            const uint32_t gmask = (1u << lg) - 1u;  // lane within the agent's group
+            const bool sorted = (n_own << lg) <= (uint32_t)kBlock;
             if (sorted) {
                 if (threadIdx.x < kSortBins) bins[threadIdx.x] = 0;
                 __syncthreads();
@@ -938,8 +961,8 @@ __global__ __launch_bounds__(kBlock, MIXED ? 5 : 4) void k_force_tiled(Arrays A,
                 __syncthreads();
                 if (threadIdx.x < n_own) ord[bins[key] + kr] = (uint8_t)threadIdx.x;
                 __syncthreads();
-                if (threadIdx.x < n_own) {
-                    const OwnEntry e = own_tab[ord[threadIdx.x]];
+                if (threadIdx.x < (n_own << lg)) {
+                    const OwnEntry e = own_tab[ord[threadIdx.x >> lg]];
                     Walk3 w;
                     w.j = e.r[0];
                     w.je = e.r[1];
@@ -949,17 +972,18 @@ __global__ __launch_bounds__(kBlock, MIXED ? 5 : 4) void k_force_tiled(Arrays A,
                     w.e2 = e.r[5];
                     w.left = 2;
                     agent_force<D, POT, MIXED, SENSE>(A, S, P, e.slot, e.self, lds[e.self], Span{e.ovb, e.ove}, w,
-                                                      physical, symmetric, src, unit);
+                                                      physical, symmetric, src, unit, threadIdx.x & gmask, lg);
                 }
                 rank_tile();
                 return;
             }
-            for (uint32_t t = threadIdx.x; t < n_own; t += kBlock) {
+            for (uint32_t t = threadIdx.x; t < (n_own << lg); t += kBlock) {
                 uint32_t self;
                 Span ov;
                 Walk3 w;
-                const uint32_t s = own_agent(t, self, ov, w);
-                agent_force<D, POT, MIXED, SENSE>(A, S, P, s, self, lds[self], ov, w, physical, symmetric, src, unit);
+                const uint32_t s = own_agent(t >> lg, self, ov, w);
+                agent_force<D, POT, MIXED, SENSE>(A, S, P, s, self, lds[self], ov, w, physical, symmetric, src, unit,
+                                                  t & gmask, lg);
             }
             rank_tile();
             return;

@@ -271,10 +271,12 @@ class MechanicsBackend:
         return out
 
     def tiling(self, inst: int = 0) -> Dict[str, int]:
-        """Force tiling of instance `inst`: voxel rows and columns per tile, tile count."""
-        r, c, n = C.c_uint32(), C.c_uint32(), C.c_uint32()
-        self._check(self._lib.crm_get_tiling(self._h, inst, C.byref(r), C.byref(c), C.byref(n)))
-        return {"rows": int(r.value), "cols": int(c.value), "tiles": int(n.value)}
+        """Force tiling of instance `inst`: voxel rows and columns per tile, tile count,
+        lanes per agent."""
+        r, c, n, lanes = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint32()
+        self._check(self._lib.crm_get_tiling(self._h, inst, C.byref(r), C.byref(c), C.byref(n), C.byref(lanes)))
+        return {"rows": int(r.value), "cols": int(c.value), "tiles": int(n.value),
+                "lanes_per_agent": int(lanes.value)}
 
     def set_domain_force(self, force, inst: int = 0) -> None:
         """Domain force of instance `inst` (crm_set_domain_force); None removes it."""

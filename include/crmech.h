@@ -273,19 +273,25 @@ crm_status crm_get_info(crm_ctx* ctx, crm_info* info);
 
 /* Tiling autotuner (no counterpart in the reference, whose CPU backend has no
  * work tiles): times the force kernel on the current agents under each of the
- * library's tiling policies (share of a workgroup's lanes filled by the tile's
- * own agents, density margin of the staged region) and keeps the fastest.  The
+ * library's tiling policies and keeps the fastest.  A policy sets the voxel tile
+ * shape (through the share of a 256-lane workgroup the tile's own agents may
+ * fill and the density margin of the staged region) and the lanes per agent
+ * (1, 2 or 4: 64, 32 or 16 agents per wavefront; the lanes of an agent split its
+ * candidate stream and add their partial forces with wavefront shuffles).  The
  * choice is cached per workload key (dimension, precision, potential, weighting,
  * instance count, agent count, grid) for the process and, when cache_path is not
  * NULL, in that JSON file, which is read first and rewritten.  FP64 forces do
- * not depend on the tiling; mixed-precision forces quantise positions with the
- * tile region's fixed-point step and so move with the tiling at that step's
- * level (reproducible for one tiling).  chosen (may be NULL) receives the policy
- * index. */
+ * not depend on the tile shape, and with more than one lane per agent differ only
+ * by the association of the partial sums (last bits); mixed-precision forces
+ * quantise positions with the tile region's fixed-point step and so move with the
+ * tiling at that step's level (reproducible for one tiling).  chosen (may be
+ * NULL) receives the policy index. */
 crm_status crm_autotune(crm_ctx* ctx, const char* cache_path, uint32_t* chosen);
 
-/* Force tiling of instance inst: voxel rows and columns per tile, tile count. */
-crm_status crm_get_tiling(crm_ctx* ctx, uint32_t inst, uint32_t* rows, uint32_t* cols, uint32_t* n_tiles);
+/* Force tiling of instance inst: voxel rows and columns per tile, tile count,
+ * lanes per agent (any output may be NULL). */
+crm_status crm_get_tiling(crm_ctx* ctx, uint32_t inst, uint32_t* rows, uint32_t* cols, uint32_t* n_tiles,
+                          uint32_t* lanes_per_agent);
 
 /* Force of the domain on every agent (the DomainForce aspect: SubDomainForce::
  * calculate_custom_force, concepts domain.rs:250-285, applied per cell after the

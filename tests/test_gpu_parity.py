@@ -462,7 +462,9 @@ def test_full_size_c2_invariants():
 
 def test_autotune_keeps_results_and_caches(tmp_path):
     """The tiling autotuner changes only the work decomposition.  FP64 forces and
-    states are bitwise those of the default tiling.  Mixed precision quantises
+    states are bitwise those of the default tiling under a one-lane policy, and
+    within 1e-14 under a policy with several lanes per agent (the partial sums'
+    association; states within 1e-12 after 5 steps).  Mixed precision quantises
     positions with the tile region's fixed-point step, so its forces move with the
     tiling at that step's level (far inside the mixed bar); for one tiling they are
     bitwise reproducible.  The choice is written to and read from the JSON cache."""
@@ -477,10 +479,11 @@ def test_autotune_keeps_results_and_caches(tmp_path):
         assert tuned.tiling_policy is not None and path.exists()
         assert f"precision={1 if precision == 'mixed' else 0}" in path.read_text()
         t = tuned.tiling()
-        assert t["rows"] >= 1 and t["cols"] >= 1 and t["tiles"] >= 1
+        assert t["rows"] >= 1 and t["cols"] >= 1 and t["tiles"] >= 1 and t["lanes_per_agent"] in (1, 2, 4)
+        one_lane = t["lanes_per_agent"] == 1
         f_tuned, f_ref = tuned.compute_forces(), ref.compute_forces()
         if precision == "fp64":
-            assert np.array_equal(f_tuned, f_ref)
+            assert np.array_equal(f_tuned, f_ref) if one_lane else rel_err(f_tuned, f_ref) <= 1e-14
         else:
             assert rel_err(f_tuned, f_ref) <= 1e-6
         again = MechanicsBackend(wl.instances, dt=wl.dt, precision=precision, noise="philox", seed=2,
@@ -493,6 +496,9 @@ def test_autotune_keeps_results_and_caches(tmp_path):
         again.step(5)
         assert np.array_equal(again.get_state()[0], tuned.get_state()[0])
         if precision == "fp64":
-            assert np.array_equal(tuned.get_state()[0], ref.get_state()[0])
+            if one_lane:
+                assert np.array_equal(tuned.get_state()[0], ref.get_state()[0])
+            else:
+                assert rel_err(tuned.get_state()[0], ref.get_state()[0]) <= 1e-12
         for h in (ref, tuned, again):
             h.close()
