@@ -207,6 +207,7 @@ SIGNATURES = {
     "crm_get_identifiers": (C.c_int, [P, U64P, U64P, U32P]),
     "crm_remove_agents": (C.c_int, [P, U8P]),
     "crm_remove_agents_noalloc": (C.c_int, [P, U8P]),
+    "crm_insert_agents": (C.c_int, [P, C.c_uint64, U64P, DP, DP, DP, DP, I32P]),
     "crm_save_cells_json": (C.c_int, [P, C.c_char_p, C.c_uint64, C.POINTER(JsonCells)]),
     "crm_get_timing": (C.c_int, [P, C.POINTER(Timing)]),
     "crm_reset_timing": (C.c_int, [P]),

@@ -260,6 +260,8 @@ struct crm_ctx {
     // point agents: CellIdentifier and parent by agent index, (kind, a, b) each (rods
     // keep theirs on the device); renumbered with the agent indices on removal
     std::vector<uint64_t> pt_ident, pt_parent;
+    std::vector<uint64_t> pt_idc;  // points: Voxel::id_counter of every voxel (divisions number from it)
+    bool pt_births = false;        // points: buf[].birth allocated (agents inserted by crm_insert_agents)
     bool have_agents = false;
     bool halted = false;
     uint64_t step = 0;
@@ -415,29 +417,35 @@ crm_status t_end(crm_ctx* ctx, int cls) {
         if (s_ != CRM_OK) return s_;   \
     } while (0)
 
-// rank != 0: the force kernel also ranks the step's re-binning (keys binned ahead).
-template <int D, int POT>
-crm_status launch_force_dp(crm_ctx* ctx, StateBuf S, uint32_t rank) {
+template <int D, int POT, bool MULTI>
+crm_status launch_force_dpm(crm_ctx* ctx, StateBuf S, uint32_t rank) {
     Arrays A = arrays(ctx);
     const uint32_t phys = ctx->cfg.weighting == CRM_WEIGHT_PHYSICAL;
     const uint32_t sym = ctx->symmetric;
     const uint32_t no_lds = (ctx->cfg.flags & CRM_FLAG_NO_LDS) ? 1u : 0u;
     const dim3 g(ctx->n_force_tiles), b(kBlock);
+    const uint32_t lg = ctx->lanes_log2;
     // sensing has kernels of its own, so the force-only kernels carry none of it
     const bool mixed = ctx->cfg.precision == CRM_PRECISION_MIXED, sense = A.sense != nullptr;
     if (mixed && sense)
-        hipLaunchKernelGGL((k_force_tiled<D, POT, true, true>), g, b, 0, ctx->stream, A, S, ctx->d_tiles, phys, sym,
-                           no_lds, rank, ctx->lanes_log2);
+        hipLaunchKernelGGL((k_force_tiled<D, POT, true, true, MULTI>), g, b, 0, ctx->stream, A, S, ctx->d_tiles, phys,
+                           sym, no_lds, rank, lg);
     else if (mixed)
-        hipLaunchKernelGGL((k_force_tiled<D, POT, true, false>), g, b, 0, ctx->stream, A, S, ctx->d_tiles, phys, sym,
-                           no_lds, rank, ctx->lanes_log2);
+        hipLaunchKernelGGL((k_force_tiled<D, POT, true, false, MULTI>), g, b, 0, ctx->stream, A, S, ctx->d_tiles, phys,
+                           sym, no_lds, rank, lg);
     else if (sense)
-        hipLaunchKernelGGL((k_force_tiled<D, POT, false, true>), g, b, 0, ctx->stream, A, S, ctx->d_tiles, phys, sym,
-                           no_lds, rank, ctx->lanes_log2);
+        hipLaunchKernelGGL((k_force_tiled<D, POT, false, true, MULTI>), g, b, 0, ctx->stream, A, S, ctx->d_tiles, phys,
+                           sym, no_lds, rank, lg);
     else
-        hipLaunchKernelGGL((k_force_tiled<D, POT, false, false>), g, b, 0, ctx->stream, A, S, ctx->d_tiles, phys, sym,
-                           no_lds, rank, ctx->lanes_log2);
+        hipLaunchKernelGGL((k_force_tiled<D, POT, false, false, MULTI>), g, b, 0, ctx->stream, A, S, ctx->d_tiles,
+                           phys, sym, no_lds, rank, lg);
     return CRM_OK;
+}
+
+// rank != 0: the force kernel also ranks the step's re-binning (keys binned ahead).
+template <int D, int POT>
+crm_status launch_force_dp(crm_ctx* ctx, StateBuf S, uint32_t rank) {
+    return ctx->lanes_log2 ? launch_force_dpm<D, POT, true>(ctx, S, rank) : launch_force_dpm<D, POT, false>(ctx, S, rank);
 }
 
 template <int D>
@@ -1028,7 +1036,8 @@ crm_status gather_snapshot_device(crm_ctx* ctx, uint32_t& n_hist, size_t& len_ou
     a.n_hist = history_slots(ctx, a.slot);
     n_hist = a.n_hist;
     const size_t len = (size_t)ctx->N * ctx->K * (2u + 2u * a.n_hist);
-    if (len > ctx->snap_cap) {
+    const bool held = is_rod(ctx) || ctx->pt_births;  // agents hold rings of different fill
+    if (len > ctx->snap_cap || (held && !ctx->d_snap_st)) {
         void* old[] = {ctx->d_snap, ctx->d_snap_sp, ctx->d_snap_rod, ctx->d_snap_id, ctx->d_snap_st};
         for (void* p : old)
             if (p) CRM_HIP(hipFree(p));
@@ -1043,15 +1052,15 @@ crm_status gather_snapshot_device(crm_ctx* ctx, uint32_t& n_hist, size_t& len_ou
         if (is_rod(ctx)) {
             CRM_TRY(dalloc(ctx, ctx->d_snap_rod, 2 * ctx->N));
             CRM_TRY(dalloc(ctx, ctx->d_snap_id, 6 * ctx->N));
-            CRM_TRY(dalloc(ctx, ctx->d_snap_st, ctx->N));
         }
+        if (held) CRM_TRY(dalloc(ctx, ctx->d_snap_st, ctx->N));
         ctx->snap_cap = len;
     }
     a.out = ctx->d_snap;
     a.species = ctx->d_snap_sp;
     a.rod = ctx->d_snap_rod;
     a.ident = ctx->d_snap_id;
-    a.stored = ctx->d_snap_st;
+    a.stored = held ? ctx->d_snap_st : nullptr;
     a.step = ctx->step;
     hipLaunchKernelGGL(k_snapshot, dim3((uint32_t)((ctx->N + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream,
                        ctx->buf[ctx->cur], a);
@@ -1305,6 +1314,293 @@ crm_status remove_agents_impl(crm_ctx* ctx, const uint8_t* remove, uint64_t n_rm
     CRM_TRY(launch_sort(ctx, false, nullptr));
     CRM_HIP(hipStreamSynchronize(ctx->stream));
     return check_errors(ctx, "agent removal");
+}
+
+// ---- insertion of divided point agents (crm_insert_agents) ------------------
+
+// apply_boundary of one point (cartesian_cuboid_n.rs:594-631), the reflection
+// point_step applies; true if a coordinate is still outside afterwards.
+bool reflect_point(const InstParams& P, uint32_t D, double* x, double* v) {
+    bool out = false;
+    for (uint32_t k = 0; k < D; k++) {
+        const uint32_t ax = k + 3 - D;
+        if (x[k] < P.min[ax]) {
+            x[k] = 2.0 * P.min[ax] - x[k];
+            v[k] = std::fabs(v[k]);
+        }
+        if (x[k] > P.max[ax]) {
+            x[k] = 2.0 * P.max[ax] - x[k];
+            v[k] = -std::fabs(v[k]);
+        }
+        out = out || x[k] < P.min[ax] || x[k] > P.max[ax];
+    }
+    return out;
+}
+
+// The device's axis_index (cartesian_cuboid_n.rs:469-491) on the host: the same
+// IEEE operations, so the same voxel.
+bool axis_index_host(double p, double mn, double dx, uint32_t n, uint32_t& out) {
+    const double x = (p - mn) / dx;
+    if (!(x > -1.0) || !(x < 4294967296.0)) return false;
+    const double t = std::trunc(x);
+    const uint32_t i = t <= 0.0 ? 0u : (uint32_t)t;
+    if (i >= n) return false;
+    out = i;
+    return true;
+}
+
+uint32_t instance_of_voxel(const crm_ctx* ctx, uint64_t g) {
+    uint32_t lo = 0, hi = ctx->n_inst;  // largest i with voxel_offset <= g
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (ctx->hparams[mid].voxel_offset <= g) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Every agent-sized device array for C agents (points), the graphs captured on
+// the old ones released first.  The state buffers get birth steps.
+crm_status realloc_point_arrays(crm_ctx* ctx, uint64_t C) {
+    for (auto& g : ctx->graph)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+    void* old[] = {ctx->d_work, ctx->d_key, ctx->d_rank, ctx->d_perm, ctx->d_force, ctx->d_recq, ctx->d_rec64,
+                   ctx->d_sense, ctx->d_snap, ctx->d_snap_sp, ctx->d_snap_st};
+    for (void* q : old)
+        if (q) CRM_HIP(hipFree(q));
+    ctx->d_work = nullptr;
+    ctx->d_key = ctx->d_rank = ctx->d_perm = nullptr;
+    ctx->d_force = nullptr;
+    ctx->d_recq = nullptr;
+    ctx->d_rec64 = nullptr;
+    ctx->d_sense = nullptr;
+    ctx->d_snap = nullptr;
+    ctx->d_snap_sp = nullptr;
+    ctx->d_snap_st = nullptr;
+    ctx->snap_cap = 0;
+    for (auto& b : ctx->buf) {
+        void* q[] = {b.pos, b.vel, b.hdx, b.hdv, b.species, b.id, b.vox, b.birth};
+        for (void* x : q)
+            if (x) CRM_HIP(hipFree(x));
+        b = StateBuf{};
+    }
+    const uint32_t K = ctx->K;
+    CRM_TRY(dalloc(ctx, ctx->d_work, C / kBlock + ctx->n_inst + 1));
+    CRM_TRY(dalloc(ctx, ctx->d_key, 2 * C));
+    CRM_TRY(dalloc(ctx, ctx->d_rank, C));
+    CRM_TRY(dalloc(ctx, ctx->d_perm, C));
+    CRM_TRY(dalloc(ctx, ctx->d_force, (size_t)K * C));
+    if (ctx->cfg.precision == CRM_PRECISION_MIXED) CRM_TRY(dalloc(ctx, ctx->d_recq, C));
+    else CRM_TRY(dalloc(ctx, ctx->d_rec64, C));
+    if (ctx->sense_on) CRM_TRY(dalloc(ctx, ctx->d_sense, C));
+    for (auto& b : ctx->buf) {
+        CRM_TRY(dalloc(ctx, b.pos, (size_t)K * C));
+        CRM_TRY(dalloc(ctx, b.vel, (size_t)K * C));
+        CRM_TRY(dalloc(ctx, b.hdx, (size_t)2 * K * C));
+        CRM_TRY(dalloc(ctx, b.hdv, (size_t)2 * K * C));
+        CRM_TRY(dalloc(ctx, b.species, C));
+        CRM_TRY(dalloc(ctx, b.id, C));
+        CRM_TRY(dalloc(ctx, b.vox, C));
+        CRM_TRY(dalloc(ctx, b.birth, C));
+    }
+    ctx->cap_agents = C;
+    return CRM_OK;
+}
+
+// CycleEvent::Division for point agents (update_cycle.rs:28-66), applied between
+// steps.  Voxel by voxel in list order, each mother takes Division(voxel,
+// ++id_counter) with its old identifier as parent and the caller's state after
+// C::divide (history kept); then its voxel's daughters, in the mothers' order,
+// join the end of the voxel's list with Division(voxel, ++id_counter), the
+// mother's old identifier as parent and an empty AuxStorage.  Mothers and
+// daughters then get apply_boundary and the re-binning of a step (stayers keep
+// their order, arrivals follow by old voxel and old position).
+crm_status insert_agents_impl(crm_ctx* ctx, uint64_t n, const uint64_t* mother, const double* mpos,
+                              const double* mvel, const double* dpos, const double* dvel, const int32_t* dsp) {
+    const uint64_t N = ctx->N, Vt = ctx->Vt, N2 = N + n;
+    const uint32_t K = ctx->K, D = ctx->D;
+    if (N2 >= 0xffffffffull) {
+        ctx->err = "too many agents (more than 2^32 - 1)";
+        return CRM_INVALID_ARGUMENT;
+    }
+    CRM_HIP(hipSetDevice(ctx->device));
+    CRM_HIP(hipStreamSynchronize(ctx->stream));
+    const StateBuf S = ctx->buf[ctx->cur];
+    std::vector<double> pos((size_t)K * N), vel((size_t)K * N), hdx((size_t)2 * K * N), hdv((size_t)2 * K * N);
+    std::vector<int32_t> sp(N);
+    std::vector<uint32_t> id(N), vox(N), birth(N, 0), start(Vt + 1);
+    CRM_HIP(hipMemcpyAsync(pos.data(), S.pos, pos.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    CRM_HIP(hipMemcpyAsync(vel.data(), S.vel, vel.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    CRM_HIP(hipMemcpyAsync(hdx.data(), S.hdx, hdx.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    CRM_HIP(hipMemcpyAsync(hdv.data(), S.hdv, hdv.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    CRM_HIP(hipMemcpyAsync(sp.data(), S.species, N * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    CRM_HIP(hipMemcpyAsync(id.data(), S.id, N * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    CRM_HIP(hipMemcpyAsync(vox.data(), S.vox, N * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    if (S.birth)
+        CRM_HIP(hipMemcpyAsync(birth.data(), S.birth, N * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    CRM_HIP(hipMemcpyAsync(start.data(), ctx->d_start, (Vt + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    CRM_HIP(hipStreamSynchronize(ctx->stream));
+
+    std::vector<uint64_t> slot_of(N);
+    for (uint64_t s = 0; s < N; s++) slot_of[id[s]] = s;
+    std::vector<int64_t> mother_at(N, -1);  // slot -> call index
+    for (uint64_t i = 0; i < n; i++) {
+        if (mother[i] >= N) {
+            ctx->err = "mother " + std::to_string(mother[i]) + " is not an agent index";
+            return CRM_INVALID_ARGUMENT;
+        }
+        const uint64_t s = slot_of[mother[i]];
+        if (mother_at[s] >= 0) {
+            ctx->err = "agent " + std::to_string(mother[i]) + " divides twice";
+            return CRM_INVALID_ARGUMENT;
+        }
+        mother_at[s] = (int64_t)i;
+    }
+    // the divided states, reflected into the domain (nothing changes on an error)
+    std::vector<double> mp((size_t)K * n), mv((size_t)K * n), dp((size_t)K * n), dv((size_t)K * n);
+    std::vector<int32_t> dspecies(n);
+    const uint32_t nsp = ctx->inst[0].potential.n_species;
+    for (uint64_t i = 0; i < n; i++) {
+        const uint64_t s = slot_of[mother[i]];
+        const InstParams& P = ctx->hparams[instance_of_voxel(ctx, vox[s])];
+        for (uint32_t k = 0; k < K; k++) {
+            mp[i * K + k] = mpos ? mpos[i * K + k] : pos[(uint64_t)k * N + s];
+            mv[i * K + k] = mvel ? mvel[i * K + k] : vel[(uint64_t)k * N + s];
+            dp[i * K + k] = dpos[i * K + k];
+        }
+        for (uint32_t k = 0; k < K; k++) dv[i * K + k] = dvel ? dvel[i * K + k] : mv[i * K + k];
+        dspecies[i] = dsp ? dsp[i] : sp[s];
+        if (dsp && species_kind(ctx->pot_kind) && (dsp[i] < 0 || (uint32_t)dsp[i] >= nsp)) {
+            ctx->err = "species out of range (daughter " + std::to_string(i) + ")";
+            return CRM_INVALID_ARGUMENT;
+        }
+        const bool out_m = reflect_point(P, D, &mp[i * K], &mv[i * K]);
+        const bool out_d = reflect_point(P, D, &dp[i * K], &dv[i * K]);
+        if (out_m || out_d) {
+            ctx->err = "BoundaryError: Particle is out of domain (" + std::string(out_m ? "mother" : "daughter") +
+                       " of division " + std::to_string(i) + ")";
+            return CRM_BOUNDARY_ERROR;
+        }
+    }
+
+    // the slot order with the daughters at the end of their mothers' voxels, the
+    // voxel each agent moves to, and the identifiers
+    std::vector<double> pos2((size_t)K * N2), vel2((size_t)K * N2), hdx2((size_t)2 * K * N2, 0.0),
+        hdv2((size_t)2 * K * N2, 0.0);
+    std::vector<int32_t> sp2(N2);
+    std::vector<uint32_t> id2(N2), vox2(N2), birth2(N2), key2(N2), start2(Vt + 1);
+    ctx->pt_ident.resize(3 * N2);
+    ctx->pt_parent.resize(3 * N2);
+    std::vector<uint64_t> old_ident((size_t)3 * n);
+    std::vector<uint64_t> born;
+    std::vector<uint32_t> inst_born(ctx->n_inst, 0);
+    const uint32_t t_now = (uint32_t)ctx->step;  // a daughter's first push is the next step's
+    uint64_t w = 0;
+    auto put = [&](uint64_t w, const double* p, const double* v) {
+        for (uint32_t k = 0; k < K; k++) {
+            pos2[(uint64_t)k * N2 + w] = p[k];
+            vel2[(uint64_t)k * N2 + w] = v[k];
+        }
+    };
+    // the voxel a divided agent moves to (its list's voxel g if its position has none)
+    auto voxel_key = [&](const InstParams& P, const double* x, uint64_t g) -> uint32_t {
+        uint32_t idx[3] = {0, 0, 0};
+        bool ok = true;
+        for (uint32_t k = 0; k < D; k++) {
+            const uint32_t ax = k + 3 - D;
+            ok = ok && axis_index_host(x[k], P.min[ax], P.dx[ax], P.n[ax], idx[ax]);
+        }
+        return ok ? (uint32_t)(P.voxel_offset + ((uint64_t)idx[0] * P.n[1] + idx[1]) * P.n[2] + idx[2])
+                  : (uint32_t)g;
+    };
+    for (uint64_t g = 0; g < Vt; g++) {
+        start2[g] = (uint32_t)w;
+        born.clear();
+        const uint32_t ii = start[g + 1] > start[g] ? instance_of_voxel(ctx, g) : 0u;
+        const uint64_t local = g - ctx->hparams[ii].voxel_offset;
+        for (uint64_t s = start[g]; s < start[g + 1]; s++, w++) {
+            for (uint32_t k = 0; k < K; k++) {
+                pos2[(uint64_t)k * N2 + w] = pos[(uint64_t)k * N + s];
+                vel2[(uint64_t)k * N2 + w] = vel[(uint64_t)k * N + s];
+                for (uint32_t h = 0; h < 2; h++) {
+                    hdx2[((uint64_t)h * K + k) * N2 + w] = hdx[((uint64_t)h * K + k) * N + s];
+                    hdv2[((uint64_t)h * K + k) * N2 + w] = hdv[((uint64_t)h * K + k) * N + s];
+                }
+            }
+            sp2[w] = sp[s];
+            id2[w] = id[s];
+            vox2[w] = vox[s];
+            birth2[w] = birth[s];
+            key2[w] = vox[s];
+            if (mother_at[s] < 0) continue;
+            const uint64_t i = (uint64_t)mother_at[s], a = id[s];
+            put(w, &mp[i * K], &mv[i * K]);
+            key2[w] = voxel_key(ctx->hparams[ii], &mp[i * K], g);
+            for (int q = 0; q < 3; q++) old_ident[3 * i + q] = ctx->pt_ident[3 * a + q];
+            ctx->pt_idc[g] += 1;
+            ctx->pt_ident[3 * a] = 1;
+            ctx->pt_ident[3 * a + 1] = local;
+            ctx->pt_ident[3 * a + 2] = ctx->pt_idc[g];
+            for (int q = 0; q < 3; q++) ctx->pt_parent[3 * a + q] = old_ident[3 * i + q];
+            born.push_back(i);
+        }
+        const InstParams& P = ctx->hparams[ii];
+        for (const uint64_t i : born) {
+            const uint64_t a = N + i;
+            put(w, &dp[i * K], &dv[i * K]);
+            sp2[w] = dspecies[i];
+            id2[w] = (uint32_t)a;
+            vox2[w] = (uint32_t)g;
+            birth2[w] = t_now;
+            key2[w] = voxel_key(P, &dp[i * K], g);
+            ctx->pt_idc[g] += 1;
+            ctx->pt_ident[3 * a] = 1;
+            ctx->pt_ident[3 * a + 1] = local;
+            ctx->pt_ident[3 * a + 2] = ctx->pt_idc[g];
+            for (int q = 0; q < 3; q++) ctx->pt_parent[3 * a + q] = old_ident[3 * i + q];
+            inst_born[ii] += 1;
+            w++;
+        }
+    }
+    start2[Vt] = (uint32_t)w;
+
+    // device: room for N2 agents, the new order uploaded, then the re-binning
+    if (N2 > ctx->cap_agents || !ctx->pt_births) CRM_TRY(realloc_point_arrays(ctx, N2 + N2 / 8));
+    ctx->pt_births = true;
+    ctx->N = N2;
+    for (uint32_t i = 0; i < ctx->n_inst; i++) ctx->counts[i] += inst_born[i];
+    ctx->cur = 0;
+    const StateBuf T = ctx->buf[0];
+    CRM_HIP(hipMemcpyAsync(T.pos, pos2.data(), pos2.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    CRM_HIP(hipMemcpyAsync(T.vel, vel2.data(), vel2.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    CRM_HIP(hipMemcpyAsync(T.hdx, hdx2.data(), hdx2.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    CRM_HIP(hipMemcpyAsync(T.hdv, hdv2.data(), hdv2.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    CRM_HIP(hipMemcpyAsync(T.species, sp2.data(), N2 * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+    CRM_HIP(hipMemcpyAsync(T.id, id2.data(), N2 * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+    CRM_HIP(hipMemcpyAsync(T.vox, vox2.data(), N2 * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+    CRM_HIP(hipMemcpyAsync(T.birth, birth2.data(), N2 * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+    CRM_HIP(hipMemcpyAsync(ctx->d_start, start2.data(), (Vt + 1) * sizeof(uint32_t), hipMemcpyHostToDevice,
+                           ctx->stream));
+    CRM_HIP(hipMemcpyAsync(ctx->d_key, key2.data(), N2 * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+    CRM_TRY(rebuild_work(ctx));
+    {
+        std::vector<ForceTile> tiles;  // the tile shape follows the density
+        make_tiles(ctx, ctx->policy, tiles);
+        CRM_TRY(upload_tiles(ctx, tiles));
+    }
+    CRM_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->sense_parity = -1;
+    ctx->graphs_stale = true;
+    hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)((stay_offset(ctx) + Vt + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       ctx->stream, ctx->d_count, (uint32_t)(stay_offset(ctx) + Vt), 0u);
+    CRM_TRY(launch_rank(ctx));
+    CRM_TRY(launch_sort(ctx, false, nullptr));
+    CRM_HIP(hipStreamSynchronize(ctx->stream));
+    return check_errors(ctx, "agent insertion");
 }
 
 }  // namespace
@@ -1624,6 +1920,16 @@ crm_status crm_set_agents(crm_ctx* ctx, const uint64_t* counts, const double* po
     if (is_rod(ctx) && ctx->cycle_on)
         hipLaunchKernelGGL(k_init_vcount, dim3((uint32_t)((Vt + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream,
                            ctx->d_start, ctx->d_vcount, (uint32_t)Vt);
+    ctx->pt_births = false;
+    ctx->pt_idc.clear();
+    if (!is_rod(ctx)) {  // Voxel::id_counter after insert_initial_cells (datastructures.rs:286-297)
+        std::vector<uint32_t> st(Vt + 1);
+        CRM_HIP(hipMemcpyAsync(st.data(), ctx->d_start, (Vt + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                               ctx->stream));
+        CRM_HIP(hipStreamSynchronize(ctx->stream));
+        ctx->pt_idc.resize(Vt);
+        for (uint64_t v = 0; v < Vt; v++) ctx->pt_idc[v] = st[v + 1] - st[v];
+    }
     CRM_HIP(hipStreamSynchronize(ctx->stream));
     ctx->have_agents = true;
     return CRM_OK;
@@ -1730,6 +2036,34 @@ crm_status crm_get_history(crm_ctx* ctx, double* dx_hist, double* dv_hist, uint3
     std::vector<uint32_t> id;
     CRM_TRY(download_ids(ctx, id));
     const uint64_t slot_elems = (uint64_t)ctx->N * ctx->K;
+    if (ctx->pt_births) {  // inserted agents hold the pushes since their birth, oldest first, then zeros
+        const uint64_t N = ctx->N;
+        const uint32_t K = ctx->K;
+        std::vector<double> hx(2 * slot_elems), hv(2 * slot_elems);
+        std::vector<uint32_t> birth(N);
+        const StateBuf& S = ctx->buf[ctx->cur];
+        CRM_HIP(hipMemcpyAsync(hx.data(), S.hdx, hx.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        CRM_HIP(hipMemcpyAsync(hv.data(), S.hdv, hv.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        CRM_HIP(hipMemcpyAsync(birth.data(), S.birth, N * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+        CRM_HIP(hipStreamSynchronize(ctx->stream));
+        for (uint64_t s = 0; s < N; s++) {
+            const uint64_t b = std::min<uint64_t>(birth[s], t);
+            const uint32_t sz = cap == 0 ? 0u : (uint32_t)std::min<uint64_t>(t - b, cap);
+            for (uint32_t i = 0; i < 2; i++)
+                for (uint32_t k = 0; k < K; k++) {
+                    const uint64_t o = i * slot_elems + (uint64_t)id[s] * K + k;
+                    double a = 0.0, c = 0.0;
+                    if (i < sz) {
+                        const uint64_t slot = (t - sz + i) % cap;
+                        a = hx[(slot * K + k) * N + s];
+                        c = hv[(slot * K + k) * N + s];
+                    }
+                    if (dx_hist) dx_hist[o] = a;
+                    if (dv_hist) dv_hist[o] = c;
+                }
+        }
+        return CRM_OK;
+    }
     for (uint32_t i = 0; i < 2; i++) {
         if (i < size) {
             uint32_t slot = (oldest + i) % cap;
@@ -2006,8 +2340,20 @@ crm_status crm_get_identifiers(crm_ctx* ctx, uint64_t* identifier, uint64_t* par
     if (identifier) std::copy(ctx->pt_ident.begin(), ctx->pt_ident.end(), identifier);
     if (parent) std::copy(ctx->pt_parent.begin(), ctx->pt_parent.end(), parent);
     if (history_held) {
-        const uint32_t held = (uint32_t)std::min<uint64_t>(ctx->step, ctx->cfg.solver_order);
+        const uint32_t cap = ctx->cfg.solver_order;
+        const uint32_t held = (uint32_t)std::min<uint64_t>(ctx->step, cap);
         std::fill(history_held, history_held + N, held);
+        if (ctx->pt_births) {  // daughters hold the pushes since their birth
+            CRM_HIP(hipSetDevice(ctx->device));
+            std::vector<uint32_t> id(N), birth(N);
+            CRM_HIP(hipMemcpyAsync(id.data(), ctx->buf[ctx->cur].id, N * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+            CRM_HIP(hipMemcpyAsync(birth.data(), ctx->buf[ctx->cur].birth, N * sizeof(uint32_t),
+                                   hipMemcpyDeviceToHost, ctx->stream));
+            CRM_HIP(hipStreamSynchronize(ctx->stream));
+            for (uint64_t s = 0; s < N; s++)
+                history_held[id[s]] = (uint32_t)std::min<uint64_t>(ctx->step - std::min<uint64_t>(birth[s], ctx->step), cap);
+        }
     }
     return CRM_OK;
 }
@@ -2045,6 +2391,27 @@ crm_status crm_remove_agents_noalloc(crm_ctx* ctx, const uint8_t* remove) {
         return CRM_INVALID_ARGUMENT;
     }
     return remove_agents_impl(ctx, remove, n_rm, ctx->d_noalloc, L);
+}
+
+crm_status crm_insert_agents(crm_ctx* ctx, uint64_t n, const uint64_t* mother, const double* mother_pos,
+                             const double* mother_vel, const double* daughter_pos, const double* daughter_vel,
+                             const int32_t* daughter_species) {
+    if (!ctx) return CRM_INVALID_ARGUMENT;
+    if (!ctx->have_agents) {
+        ctx->err = "crm_set_agents must be called first";
+        return CRM_INVALID_ARGUMENT;
+    }
+    if (ctx->halted) return CRM_INVALID_ARGUMENT;
+    if (is_rod(ctx)) {
+        ctx->err = "rods divide through the rods example's cycle (crm_set_rod_cycle)";
+        return CRM_INVALID_ARGUMENT;
+    }
+    if (n == 0) return CRM_OK;
+    if (!mother || !daughter_pos) return CRM_INVALID_ARGUMENT;
+    crm_status st = insert_agents_impl(ctx, n, mother, mother_pos, mother_vel, daughter_pos, daughter_vel,
+                                       daughter_species);
+    if (st == CRM_DEVICE_ERROR) ctx->halted = true;
+    return st;
 }
 
 crm_status crm_set_neighbor_sensing(crm_ctx* ctx, uint32_t enable, double radius) {
@@ -2139,9 +2506,14 @@ crm_status crm_save_cells_json(crm_ctx* ctx, const char* path, uint64_t iteratio
     std::vector<double> rodv;
     std::vector<uint64_t> ids;
     std::vector<uint32_t> held;
-    if (!is_rod(ctx)) {  // identifiers from the host copy; every agent holds the whole ring
+    if (!is_rod(ctx)) {  // identifiers from the host copy; every agent holds the whole ring but daughters
         ids.resize(6 * N);
         held.assign(N, nh);
+        if (ctx->pt_births) {
+            CRM_HIP(hipMemcpyAsync(held.data(), ctx->d_snap_st, N * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+            CRM_HIP(hipStreamSynchronize(ctx->stream));
+        }
         for (uint64_t a = 0; a < N; a++)
             for (int q = 0; q < 3; q++) {
                 ids[6 * a + q] = ctx->pt_ident[3 * a + q];

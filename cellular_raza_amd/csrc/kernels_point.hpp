@@ -553,16 +553,31 @@ __device__ __forceinline__ void agent_force(const Arrays& A, const StateBuf& S, 
         // positions in stream order -- the same pairs, order and arithmetic as the
         // generic walk below.
         const bool active = POT != CRM_POTENTIAL_NONE || sense;
-        const uint32_t j0 = cur.j, B1 = cur.je - cur.j, B2 = B1 + (cur.e1 - cur.j1);
+        const uint32_t B1 = cur.je - cur.j, B2 = B1 + (cur.e1 - cur.j1);
         const uint32_t L = active ? B2 + (cur.e2 - cur.j2) : 0u;
         const uint32_t o1 = cur.j1 - cur.je, o2 = cur.j2 - cur.e1;  // modular offsets
-        const uint32_t p_self = B1 + (self - cur.j1);
-        auto at = [&](uint32_t p) -> uint32_t { return j0 + p + (p >= B1 ? o1 : 0u) + (p >= B2 ? o2 : 0u); };
+        // this lane's part [pb, pe) of the stream, walked by the offset r = p - pb
+        // (the same for every lane of a wave, so the loop counters stay scalar)
+        const uint32_t part = (L + (1u << lg) - 1u) >> lg;
+        const uint32_t pb = min(L, sub * part), Lr = min(L, pb + part) - pb;
+        const uint32_t j0 = cur.j + pb + (pb >= B1 ? o1 : 0u) + (pb >= B2 ? o2 : 0u);
+        // range boundaries ahead of the part's start, and their offsets
+        const uint32_t R1 = B1 > pb ? B1 - pb : 0u, R2 = B2 > pb ? B2 - pb : 0u;
+        const uint32_t off1 = pb < B1 ? o1 : 0u, off2 = pb < B2 ? o2 : 0u;
+        const uint32_t r_self = B1 + (self - cur.j1) - pb;  // wraps above Lr when before the part
+        // LDS byte offset of stream offset r (the range offsets scaled once, not per
+        // candidate), its record and its LDS index
+        constexpr uint32_t RB = (uint32_t)sizeof(Rec);
+        const uint32_t jb0 = j0 * RB, off1b = off1 * RB, off2b = off2 * RB;
+        const char* lds_b = reinterpret_cast<const char*>(src.lds);
+        auto atb = [&](uint32_t r) -> uint32_t { return jb0 + r * RB + (r >= R1 ? off1b : 0u) + (r >= R2 ? off2b : 0u); };
+        auto rec_at = [&](uint32_t r) -> Rec { return *reinterpret_cast<const Rec*>(lds_b + atb(r)); };
+        auto at = [&](uint32_t r) -> uint32_t { return atb(r) / RB; };
         // unit grids (equal fixed-point steps on every axis) need no axis scaling in
         // pass 1 (r1 = r2 = 1 exactly, so the marks are the same)
         const bool cubic = !MIXED || (r1 == 1.0f && r2 == 1.0f);
         auto marked = [&](auto cub, uint32_t p) -> uint32_t {
-            const Rec rj = src.record(at(p), cur);
+            const Rec rj = rec_at(p);
             bool mark;
             if constexpr (MIXED) {
                 float d2;
@@ -587,15 +602,12 @@ __device__ __forceinline__ void agent_force(const Arrays& A, const StateBuf& S, 
             for (uint32_t q = 0; q < q1; q++) mlo |= marked(cub, p0 + q) << q;
             for (uint32_t q = 32; q < qn; q++) mhi |= marked(cub, p0 + q) << (q - 32u);
         };
-        // this lane's part of the stream
-        const uint32_t part = (L + (1u << lg) - 1u) >> lg;
-        const uint32_t pb = min(L, sub * part), pe = min(L, pb + part);
-        for (uint32_t p0 = pb; p0 < pe; p0 += 64u) {
-            const uint32_t qn = min(64u, pe - p0);
+        for (uint32_t p0 = 0; p0 < Lr; p0 += 64u) {
+            const uint32_t qn = min(64u, Lr - p0);
             uint32_t mlo = 0, mhi = 0;
             if (cubic) pass1(std::true_type{}, p0, qn, mlo, mhi);
             else pass1(std::false_type{}, p0, qn, mlo, mhi);
-            const uint32_t ds = p_self - p0;  // the self pair is never evaluated
+            const uint32_t ds = r_self - p0;  // the self pair is never evaluated
             if (ds < 32u) mlo &= ~(1u << ds);
             else if (ds < 64u) mhi &= ~(1u << (ds - 32u));
             W w = cur;
@@ -741,11 +753,15 @@ __device__ __forceinline__ uint32_t small_div(uint32_t a, float inv_d) {
 // starts) in LDS, then one lane per own agent sums its forces from LDS.  A tile
 // whose region does not fit (local crowding) reads global memory instead; the
 // candidate order and the arithmetic are the same.
-template <int D, int POT, bool MIXED, bool SENSE>
+// MULTI: the tiling policy gives each agent 2^lanes_log2 lanes (a separate
+// instantiation, so that the one-lane kernel carries none of the group logic).
+template <int D, int POT, bool MIXED, bool SENSE, bool MULTI>
 __global__ __launch_bounds__(kBlock, MIXED ? 5 : 4) void k_force_tiled(Arrays A, StateBuf S,
                                                                      const ForceTile* __restrict__ tiles,
                                                                      uint32_t physical, uint32_t symmetric,
-                                                                     uint32_t no_lds, uint32_t rank, uint32_t lg) {
+                                                                     uint32_t no_lds, uint32_t rank,
+                                                                     uint32_t lanes_log2) {
+    const uint32_t lg = MULTI ? lanes_log2 : 0u;
     using Rec = typename RecOf<MIXED>::type;
     constexpr int CAP = MIXED ? kLdsRecordsMixed : kLdsRecordsFP64;
     constexpr uint32_t NP = D == 3 ? 3u : 1u;         // staged planes
@@ -972,7 +988,8 @@ __global__ __launch_bounds__(kBlock, MIXED ? 5 : 4) void k_force_tiled(Arrays A,
                     w.e2 = e.r[5];
                     w.left = 2;
                     agent_force<D, POT, MIXED, SENSE>(A, S, P, e.slot, e.self, lds[e.self], Span{e.ovb, e.ove}, w,
-                                                      physical, symmetric, src, unit, threadIdx.x & gmask, lg);
+                                                      physical, symmetric, src, unit, MULTI ? threadIdx.x & gmask : 0u,
+                                                      lg);
                 }
                 rank_tile();
                 return;
@@ -983,7 +1000,7 @@ __global__ __launch_bounds__(kBlock, MIXED ? 5 : 4) void k_force_tiled(Arrays A,
                 Walk3 w;
                 const uint32_t s = own_agent(t >> lg, self, ov, w);
                 agent_force<D, POT, MIXED, SENSE>(A, S, P, s, self, lds[self], ov, w, physical, symmetric, src, unit,
-                                                  t & gmask, lg);
+                                                  MULTI ? t & gmask : 0u, lg);
             }
             rank_tile();
             return;
@@ -1097,6 +1114,25 @@ __device__ __forceinline__ Ring ring_at(uint64_t t, uint32_t cap, uint32_t newes
     return r;
 }
 
+// The ring of a point agent whose first push is at step b (0, or the step after
+// it was inserted by a division, whose AuxStorage starts empty): the entries are
+// the pushes of steps max(b, t + 1 - cap) .. t (ring_at(t) when b == 0).  The
+// capacity is 0, 1 or 2, so the slot arithmetic is a parity.
+__device__ __forceinline__ Ring ring_agent(uint64_t t, uint64_t b, uint32_t cap, uint32_t newest_first) {
+    Ring r;
+    r.cap = cap;
+    r.wslot = cap == 2 ? (uint32_t)(t & 1u) : 0u;
+    const uint32_t size = (uint32_t)min<uint64_t>(t - b + 1, cap);
+    const uint32_t oldest = cap == 2 ? (uint32_t)((t + 1 - size) & 1u) : 0u;
+    r.n_read = (cap == 2 && size == 2) ? 2u : 1u;
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        const uint32_t q = newest_first ? oldest + size + 1u - (uint32_t)i : oldest + (uint32_t)i;
+        r.it[i] = cap == 2 ? (q & 1u) : 0u;
+    }
+    return r;
+}
+
 // Standard normals of this agent and step (buffer or Philox).
 template <int D>
 __device__ __forceinline__ bool point_noise(const StepArgs& sa, uint32_t id, uint64_t t, double* z) {
@@ -1205,7 +1241,7 @@ __global__ __launch_bounds__(kBlock) void k_key_point(Arrays A, StateBuf S, Step
     const uint64_t N = A.N;
     const uint64_t t = A.ctrl->step;
     const uint32_t id = S.id[s];
-    const Ring rg = ring_at(t, sa.cap, sa.newest_first);
+    const Ring rg = ring_agent(t, S.birth ? S.birth[s] : 0u, sa.cap, sa.newest_first);
     double x[D], v[D], f[D], nx[D], nv[D], dx[D], dv[D];
 #pragma unroll
     for (int k = 0; k < D; k++) {
@@ -1442,8 +1478,8 @@ __device__ __forceinline__ void write_record(const Arrays& A, const InstParams& 
 template <int D, int MECH>
 __device__ __forceinline__ void bin_ahead(const Arrays& A, const InstParams& P, const StepArgs& sa, uint64_t t1,
                                           uint32_t id, uint64_t s, const double* x, const double* v,
-                                          const double* r0, const double* r1, uint32_t vox) {
-    const Ring rg = ring_at(t1, sa.cap, sa.newest_first);
+                                          const double* r0, const double* r1, uint32_t vox, uint64_t birth) {
+    const Ring rg = ring_agent(t1, birth, sa.cap, sa.newest_first);
     double f[D], nx[D], nv[D], dx[D], dv[D];
 #pragma unroll
     for (int k = 0; k < D; k++) f[k] = 0.0;
@@ -1495,7 +1531,9 @@ __global__ __launch_bounds__(kBlock) void k_gather_point(Arrays A, StateBuf src,
     const uint64_t N = A.N;
     const uint32_t o = A.perm[s];
     const uint64_t t = A.ctrl->step;
-    const uint32_t n_hist = (uint32_t)min<uint64_t>(t, sa.cap);
+    const uint64_t birth = src.birth ? src.birth[o] : 0u;
+    // every slot when agents hold rings of different fill (inserted daughters)
+    const uint32_t n_hist = src.birth ? sa.cap : (uint32_t)min<uint64_t>(t, sa.cap);
     double p[D], v[D], r0[D], r1[D];
 #pragma unroll
     for (int k = 0; k < D; k++) {
@@ -1522,7 +1560,8 @@ __global__ __launch_bounds__(kBlock) void k_gather_point(Arrays A, StateBuf src,
     const uint32_t key = A.key[o];
     dst.vox[s] = key;
     write_record<D, MIXED>(A, P, s, key, p, sp);
-    if constexpr (AHEAD) bin_ahead<D, MECH>(A, P, sa, t, id, s, p, v, r0, r1, key);
+    if (dst.birth) dst.birth[s] = (uint32_t)birth;
+    if constexpr (AHEAD) bin_ahead<D, MECH>(A, P, sa, t, id, s, p, v, r0, r1, key, birth);
 }
 
 // Gather + integrate: the agent that lands in sorted slot s comes from slot
@@ -1542,7 +1581,8 @@ __global__ __launch_bounds__(kBlock) void k_gather_integrate_point(Arrays A, Sta
     const uint64_t t = A.ctrl->step - 1;
     const uint32_t o = A.perm[s];
     const uint32_t id = src.id[o];
-    const Ring rg = ring_at(t, sa.cap, sa.newest_first);
+    const uint64_t birth = src.birth ? src.birth[o] : 0u;
+    const Ring rg = ring_agent(t, birth, sa.cap, sa.newest_first);
     double x[D], v[D], f[D], nx[D], nv[D], dx[D], dv[D];
 #pragma unroll
     for (int k = 0; k < D; k++) {
@@ -1550,13 +1590,14 @@ __global__ __launch_bounds__(kBlock) void k_gather_integrate_point(Arrays A, Sta
         v[k] = src.vel[k * N + o];
         f[k] = A.force[k * N + o];
     }
-    // the ring before this step (min(t, cap) entries), in registers, except the
-    // slot this step overwrites: point_step reads the new increment in its place
-    const uint32_t n_old = (uint32_t)min<uint64_t>(t, rg.cap);
+    // the ring before this step (min(t - birth, cap) entries: with cap <= 2 the
+    // slot not overwritten now, once filled), in registers, except the slot this
+    // step overwrites: point_step reads the new increment in its place
+    const uint32_t n_old = (uint32_t)min<uint64_t>(t - birth, rg.cap);
     double hx[2][D], hv[2][D];
 #pragma unroll
     for (int h = 0; h < 2; h++) {
-        const bool live = (uint32_t)h < n_old && (uint32_t)h != rg.wslot;  // uniform
+        const bool live = (uint32_t)h < rg.cap && (uint32_t)h != rg.wslot && n_old > 0;
 #pragma unroll
         for (int k = 0; k < D; k++) {
             const uint64_t e = ((uint64_t)h * D + k) * N + o;
@@ -1577,16 +1618,16 @@ __global__ __launch_bounds__(kBlock) void k_gather_integrate_point(Arrays A, Sta
     }
     // ring: the slot written this step gets (dx, dv), the other slot is carried over
     if (rg.cap > 0) {
-        const uint32_t n_hist = (uint32_t)min<uint64_t>(t + 1, rg.cap);
 #pragma unroll
         for (int h = 0; h < 2; h++) {
+            const bool keep = (uint32_t)h < rg.cap && ((uint32_t)h == rg.wslot || n_old > 0);
 #pragma unroll
             for (int k = 0; k < D; k++) {
                 if (h == (int)rg.wslot) {
                     hx[h][k] = dx[k];
                     hv[h][k] = dv[k];
                 }
-                if ((uint32_t)h < n_hist) {
+                if (keep) {
                     const uint64_t e = ((uint64_t)h * D + k) * N + s;
                     dst.hdx[e] = hx[h][k];
                     dst.hdv[e] = hv[h][k];
@@ -1600,8 +1641,9 @@ __global__ __launch_bounds__(kBlock) void k_gather_integrate_point(Arrays A, Sta
     const uint32_t key = A.key[o];
     dst.vox[s] = key;
     write_record<D, MIXED>(A, P, s, key, nx, sp);
+    if (dst.birth) dst.birth[s] = (uint32_t)birth;
     if constexpr (MECH != CRM_MECHANICS_BROWNIAN)
-        bin_ahead<D, MECH>(A, P, sa, t + 1, id, s, nx, nv, hx[0], hx[1], key);
+        bin_ahead<D, MECH>(A, P, sa, t + 1, id, s, nx, nv, hx[0], hx[1], key, birth);
 }
 
 }  // namespace crm

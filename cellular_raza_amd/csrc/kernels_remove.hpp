@@ -82,6 +82,7 @@ __global__ __launch_bounds__(kBlock) void k_remove_compact(StateBuf src, StateBu
     dst.id[m] = X.id_before[src.id[s]];
     dst.vox[m] = src.vox[s];
     if (rod) copy_rod_extras(src, s, dst, m, n_old, n_new);
+    else if (src.birth) dst.birth[m] = src.birth[s];  // point agents inserted by a division
 }
 
 // Next order of the compacted agents: each stays in its voxel; k_fixup orders

@@ -231,6 +231,25 @@ class MechanicsBackend:
         self._check(st)
         self.n_agents -= int(np.count_nonzero(rm))
 
+    def insert_agents(self, mother, daughter_pos, daughter_vel=None, mother_pos=None, mother_vel=None,
+                      daughter_species=None) -> None:
+        """CycleEvent::Division for point agents (crm_insert_agents,
+        update_cycle.rs:28-66): mother[i] takes the state C::divide left it
+        (mother_pos / mother_vel, None = unchanged) and a Division identifier;
+        daughter i becomes agent n_agents + i with an empty history."""
+        m = np.ascontiguousarray(mother, dtype=np.uint64).reshape(-1)
+        n = m.shape[0]
+
+        def arr(x):
+            return None if x is None else np.ascontiguousarray(x, dtype=np.float64).reshape(n, self.K)
+
+        dp, dv, mp, mv = arr(daughter_pos), arr(daughter_vel), arr(mother_pos), arr(mother_vel)
+        sp = None if daughter_species is None else np.ascontiguousarray(daughter_species, dtype=np.int32).reshape(n)
+        st = self._lib.crm_insert_agents(self._h, n, m.ctypes.data_as(_abi.U64P), _dp(mp), _dp(mv), _dp(dp),
+                                         _dp(dv), None if sp is None else sp.ctypes.data_as(_abi.I32P))
+        self._check(st)
+        self.n_agents += n
+
     def compute_forces(self) -> np.ndarray:
         out = np.empty((self.n_agents, self.K), dtype=np.float64)
         self._check(self._lib.crm_compute_forces(self._h, _dp(out)))

@@ -253,7 +253,9 @@ crm_status crm_compute_forces(crm_ctx* ctx, double* force);
 crm_status crm_get_state(crm_ctx* ctx, double* pos, double* vel);
 
 /* Adams-Bashforth ring contents, oldest slot first: dx_hist, dv_hist each
- * 2 * n_agents * n_vertices * dim doubles (slot-major); n_stored in {0,1,2}. */
+ * 2 * n_agents * n_vertices * dim doubles (slot-major); n_stored in {0,1,2}.  A
+ * point agent inserted by crm_insert_agents holds only the pushes since its
+ * insertion (crm_get_identifiers' history_held): they come first, zeros after. */
 crm_status crm_get_history(crm_ctx* ctx, double* dx_hist, double* dv_hist, uint32_t* n_stored);
 
 /* Voxel of every agent (instance voxel offset + plain index, plain index as in
@@ -388,6 +390,34 @@ crm_status crm_get_identifiers(crm_ctx* ctx, uint64_t* identifier, uint64_t* par
  * context keeps the call's scratch memory (device allocation on the first call
  * and when the agent count exceeds every earlier call's). */
 crm_status crm_remove_agents(crm_ctx* ctx, const uint8_t* remove);
+
+/* CycleEvent::Division of a caller's Cycle implementation for point agents
+ * (update_cycle.rs:28-66; the reference calls C::divide and appends the new cells
+ * in Voxel::update_cell_cycle_4).  Call it between steps for the agents whose
+ * update_cycle returned Division in the step just taken, with the states C::divide
+ * produced: mother_pos / mother_vel (n * dim doubles each, NULL = unchanged) replace
+ * mother[i]'s state, which keeps its history; daughter i (daughter_pos, and
+ * daughter_vel or, if NULL, the mother's new velocity; daughter_species or, if
+ * NULL, the mother's species) becomes agent index N + i with an empty AuxStorage
+ * (no history: its first step is an Euler step, as in the reference).
+ * Identifiers: voxel by voxel in list order, each mother takes
+ * Division(voxel, ++id_counter) and its old identifier as parent; then the
+ * voxel's daughters, in the mothers' order, take Division(voxel, ++id_counter) and
+ * their mother's old identifier as parent, and join the end of the voxel's list.
+ * Then apply_boundary reflects mothers and daughters and the re-binning of a step
+ * moves those now outside their list's voxel (arrivals after stayers, by old
+ * voxel and old position).  The reference divides after a step's mechanics update
+ * and before its boundary and re-binning; here the step's re-binning has already
+ * moved the mothers, so a mother that changed voxel in that step divides in its
+ * new voxel, and a daughter joins its voxel after that step's arrivals.  A mother
+ * listed twice or not an agent, a species out of range: CRM_INVALID_ARGUMENT; a
+ * state that apply_boundary cannot bring into the domain: CRM_BOUNDARY_ERROR;
+ * nothing changes in either case.  Point agents only (rods divide through
+ * crm_set_rod_cycle).  Costs one host round trip of the state (device
+ * reallocation when the context's agent capacity is exceeded). */
+crm_status crm_insert_agents(crm_ctx* ctx, uint64_t n, const uint64_t* mother, const double* mother_pos,
+                             const double* mother_vel, const double* daughter_pos, const double* daughter_vel,
+                             const int32_t* daughter_species);
 
 /* No counterpart in the reference: crm_remove_agents for callers that cannot
  * allocate device memory (code under HIP graph capture elsewhere in the process,

@@ -633,6 +633,8 @@ struct SimBase {
     virtual int set_spring_lengths(const double* sl) = 0;
     virtual void cycle_out(double* spring_length, double* growth_rate, uint64_t* ident, uint64_t* parent) = 0;
     virtual int remove_agents(const uint8_t* flags) = 0;
+    virtual int insert_agents(uint64_t n, const uint64_t* mother, const double* mpos, const double* mvel,
+                              const double* dpos, const double* dvel, const int32_t* dsp) = 0;
     bool cycle_on = false;
     crm_rod_cycle cyc{};
 };
@@ -1257,6 +1259,82 @@ struct Sim : SimBase {
         return CRM_OK;
     }
 
+    // CycleEvent::Division of a caller's Cycle for point agents, as
+    // Voxel::update_cell_cycle_4 applies it (update_cycle.rs:28-66), between steps:
+    // voxel by voxel in list order each mother takes the state C::divide left it and
+    // Division(voxel, ++id_counter) with its old identifier as parent; the voxel's
+    // daughters follow the loop in the mothers' order with Division(voxel,
+    // ++id_counter), the same parent and a fresh AuxStorage (empty ring).  Then
+    // apply_boundary and sort_cells_in_voxels (update_mechanics.rs:418-513) as at
+    // the end of a step.  Daughter i is agent n_agents + i.
+    int insert_agents(uint64_t n, const uint64_t* mother, const double* mpos, const double* mvel,
+                      const double* dpos, const double* dvel, const int32_t* dsp) override {
+        if (nv != 1) return CRM_INVALID_ARGUMENT;
+        std::vector<int64_t> call(n_agents, -1);
+        for (uint64_t i = 0; i < n; i++) {
+            if (mother[i] >= n_agents || call[mother[i]] >= 0) return CRM_INVALID_ARGUMENT;
+            call[mother[i]] = (int64_t)i;
+        }
+        // the divided states first: on a BoundaryError nothing changes
+        std::vector<Cell<KM>> mnew(n), dnew(n);
+        for (auto& I : inst)
+            for (auto& cells : I.voxels)
+                for (auto& c : cells) {
+                    if (call[c.id] < 0) continue;
+                    const uint64_t i = (uint64_t)call[c.id];
+                    Cell<KM> m = c, d;
+                    std::memset(&d, 0, sizeof(d));
+                    for (int k = 0; k < K; k++) {
+                        if (mpos) m.pos[k] = mpos[i * K + k];
+                        if (mvel) m.vel[k] = mvel[i * K + k];
+                    }
+                    d.id = n_agents + i;
+                    d.species = dsp ? dsp[i] : c.species;
+                    for (int k = 0; k < K; k++) {
+                        d.pos[k] = dpos[i * K + k];
+                        d.vel[k] = dvel ? dvel[i * K + k] : m.vel[k];
+                    }
+                    if (apply_boundary(I, m) || apply_boundary(I, d)) {
+                        err = "BoundaryError: Particle is out of domain";
+                        return CRM_BOUNDARY_ERROR;
+                    }
+                    mnew[i] = m;
+                    dnew[i] = d;
+                }
+        for (auto& I : inst)
+            for (size_t v = 0; v < I.voxels.size(); v++) {
+                auto& cells = I.voxels[v];
+                std::vector<Cell<KM>> born;
+                for (auto& c : cells) {
+                    if (call[c.id] < 0) continue;
+                    const uint64_t i = (uint64_t)call[c.id];
+                    const uint64_t par[3] = {c.ident[0], c.ident[1], c.ident[2]};
+                    c = mnew[i];
+                    I.id_counter[v] += 1;
+                    c.ident[0] = 1;
+                    c.ident[1] = v;
+                    c.ident[2] = I.id_counter[v];
+                    std::memcpy(c.parent, par, sizeof(par));
+                    Cell<KM> d = dnew[i];
+                    std::memcpy(d.parent, par, sizeof(par));
+                    born.push_back(d);
+                }
+                for (auto& d : born) {
+                    I.id_counter[v] += 1;
+                    d.ident[0] = 1;
+                    d.ident[1] = v;
+                    d.ident[2] = I.id_counter[v];
+                    cells.push_back(d);
+                }
+            }
+        n_agents += n;
+        for (size_t ii = 0; ii < inst.size(); ii++) {
+            for (uint32_t sub = 0; sub < n_sub; sub++) sort_step1(ii, sub);
+            for (uint32_t sub = 0; sub < n_sub; sub++) sort_step2(ii, sub);
+        }
+        return CRM_OK;
+    }
+
     void cycle_out(double* spring_length, double* growth_rate, uint64_t* ident, uint64_t* parent) override {
         for_each_cell([&](Cell<KM>& c) {
             if (spring_length) spring_length[c.id] = c.spring_length;
@@ -1654,6 +1732,12 @@ int orc_set_spring_lengths(void* h, const double* sl) {
 int orc_remove_agents(void* h, const uint8_t* flags) {
     if (!h || !flags) return CRM_INVALID_ARGUMENT;
     return static_cast<SimBase*>(h)->remove_agents(flags);
+}
+
+int orc_insert_agents(void* h, uint64_t n, const uint64_t* mother, const double* mpos, const double* mvel,
+                      const double* dpos, const double* dvel, const int32_t* dsp) {
+    if (!h || (n && (!mother || !dpos))) return CRM_INVALID_ARGUMENT;
+    return static_cast<SimBase*>(h)->insert_agents(n, mother, mpos, mvel, dpos, dvel, dsp);
 }
 
 int orc_cycle_out(void* h, double* spring_length, double* growth_rate, uint64_t* ident, uint64_t* parent) {

@@ -60,6 +60,7 @@ def load():
         "orc_set_spring_lengths": (C.c_int, [P, _abi.DP]),
         "orc_cycle_out": (C.c_int, [P, _abi.DP, _abi.DP, _abi.U64P, _abi.U64P]),
         "orc_remove_agents": (C.c_int, [P, _abi.U8P]),
+        "orc_insert_agents": (C.c_int, [P, C.c_uint64, _abi.U64P, _abi.DP, _abi.DP, _abi.DP, _abi.DP, _abi.I32P]),
         "orc_n_voxels_total": (C.c_uint64, [P]),
         "orc_n_agents": (C.c_uint64, [P]),
         "orc_last_error": (C.c_size_t, [P, C.c_char_p, C.c_size_t]),
@@ -174,6 +175,20 @@ class OracleSim:
         rm = np.ascontiguousarray(remove, dtype=np.uint8)
         self._check(self._lib.orc_remove_agents(self._h, rm.ctypes.data_as(_abi.U8P)))
         self.n_agents = int(self._lib.orc_n_agents(self._h))
+
+    def insert_agents(self, mother, daughter_pos, daughter_vel=None, mother_pos=None, mother_vel=None,
+                      daughter_species=None):
+        m = np.ascontiguousarray(mother, dtype=np.uint64).reshape(-1)
+        n = m.shape[0]
+
+        def arr(x):
+            return None if x is None else np.ascontiguousarray(x, dtype=np.float64).reshape(n, self.K)
+
+        dp, dv, mp, mv = arr(daughter_pos), arr(daughter_vel), arr(mother_pos), arr(mother_vel)
+        sp = None if daughter_species is None else np.ascontiguousarray(daughter_species, dtype=np.int32).reshape(n)
+        self._check(self._lib.orc_insert_agents(self._h, n, m.ctypes.data_as(_abi.U64P), _dp(mp), _dp(mv), _dp(dp),
+                                                _dp(dv), None if sp is None else sp.ctypes.data_as(_abi.I32P)))
+        self.n_agents += n
 
     def cycle_state(self):
         n = self.n_agents
