@@ -29,6 +29,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3  # MI355X vector FP32
+# Useful FP32 work of the force kernel (FMA = 2 flops; sqrt, reciprocal, exp = 1):
+# per candidate the displacement and squared distance (3 sub, 1 mul, 2 FMA), per
+# pair inside the cutoff about 30 more (root and reciprocal, the potential's
+# exponential or powers, direction and weight, the compensated sum of 3 components).
+FLOPS_PER_CANDIDATE = 8
+FLOPS_PER_PAIR = 30
 # committed rocprofv3 summaries of the workloads that have one
 PROFILES = {"c2": "profiles/c2_cell_sorting_1M.md", "c3": "profiles/c3_rods_2d_200k.md",
             "c5": "profiles/c5_mie_3d_4M.md"}
@@ -226,6 +233,13 @@ def run_gpu(wl, precision, steps, warmup, seed=1, timing_steps=20):
     t_get0 = time.perf_counter()
     pos_end = g.get_state()[0]
     t_get = time.perf_counter() - t_get0  # download of positions and velocities to host buffers
+    in_range = None
+    cut = pair_cutoff(wl.instances[0])
+    if tiling is not None and cut:  # pairs inside the cutoff per agent, on the end state
+        g.set_neighbor_sensing(cut)
+        g.compute_forces()
+        in_range = float(g.neighbor_counts().mean())
+        g.set_neighbor_sensing(None)
     g.close()
     # per-kernel HIP-event timing over a second timed region on the same stream
     gt = make_backend(wl, precision, False, True, seed)
@@ -234,7 +248,14 @@ def run_gpu(wl, precision, steps, warmup, seed=1, timing_steps=20):
     gt.step(timing_steps)
     tim = gt.timing()
     gt.close()
-    return sec, f0, start, tim, pos_end, t_set + t_get, tiling
+    return sec, f0, start, tim, pos_end, t_set + t_get, tiling, in_range
+
+
+def pair_cutoff(spec):
+    """The interaction's cutoff radius (None when it has none)."""
+    inter = spec.interaction
+    c = getattr(inter, "cutoff", None)
+    return float(c) if c else None
 
 
 def run_sequential(wl, precision, steps, warmup, seed=1):
@@ -365,7 +386,7 @@ def cpu_calibration(threads):
     wl = W.cell_sorting_benchmark(8)
     gpu = {}
     for precision in ("fp64", "mixed"):
-        sec, _, _, _, _, _, _ = run_gpu(wl, precision, 200, 10, timing_steps=5)
+        sec = run_gpu(wl, precision, 200, 10, timing_steps=5)[0]
         gpu[precision] = wl.n_agents * 200 / sec
     out["gpu_same_workload_agent_steps_per_s"] = gpu
     out["gpu_mixed_over_published_3700X"] = gpu["mixed"] / published["1310720"]["3700X-at-2000MHz"]
@@ -379,16 +400,16 @@ def workload_table():
     return {
         "c2": (lambda a: W.c2_sorting_3d(n=a.n or 1_000_000, seed=2, steps=a.steps), True,
                "3-D two-species cell sorting, Langevin, N=1,000,000, volume fraction 0.3",
-               "k_force_tiled<3, 3, true, false>"),
+               "k_force_tiled<3, 3, true, false, false>"),
         "c1": (lambda a: W.c1_morse_2d(n=a.n or 1000, seed=1, steps=a.steps), False,
-               "2-D Morse NewtonDamped, N=1000 in 250x250, Euler", "k_force_tiled<2, 0, false, false>"),
+               "2-D Morse NewtonDamped, N=1000 in 250x250, Euler", "k_force_tiled<2, 0, false, false, true>"),
         "c3": (lambda a: W.c3_rods_2d(n=a.n or 200_000, seed=3, steps=a.steps), True,
                "2-D bacterial rods, 200,000 rods x 8 vertices in 2000x2000, AB3", "k_force_rod<2, 0, true>"),
         "c4": (lambda a: W.c4_ensemble_2d(n_instances=a.n or 512, seed=4, steps=a.steps), True,
                "512-instance 2-D Morse ensemble (500..20000 agents each), Euler, one context",
-               "k_force_tiled<2, 0, true, false>"),
+               "k_force_tiled<2, 0, true, false, false>"),
         "c5": (lambda a: W.c5_mie_3d(n=a.n or 4_000_000, seed=5, steps=a.steps), True,
-               "3-D Mie lattice, N=4,000,000, AB3", "k_force_tiled<3, 1, true, false>"),
+               "3-D Mie lattice, N=4,000,000, AB3", "k_force_tiled<3, 1, true, false, false>"),
     }
 
 
@@ -459,7 +480,7 @@ def main():
           flush=True)
 
     main_prec = "mixed" if has_mixed else "fp64"
-    sec_m, f_main, start, tim_m, pos_main, host_io, tiling = run_gpu(wl, main_prec, args.steps, args.warmup)
+    sec_m, f_main, start, tim_m, pos_main, host_io, tiling, in_range = run_gpu(wl, main_prec, args.steps, args.warmup)
     value = N * args.steps / sec_m
     print(f"bench: {main_prec} {value:.4e} agent-steps/s", file=sys.stderr, flush=True)
     extra = {
@@ -470,7 +491,7 @@ def main():
     }
     if has_mixed:
         fp64_steps = max(args.steps // 4, 10)
-        sec_d, f_fp64, _, tim_d, _, _, _ = run_gpu(wl, "fp64", fp64_steps, args.warmup)
+        sec_d, f_fp64, _, tim_d = run_gpu(wl, "fp64", fp64_steps, args.warmup)[:4]
         value_fp64 = N * fp64_steps / sec_d
         print(f"bench: fp64 {value_fp64:.4e} agent-steps/s", file=sys.stderr, flush=True)
         extra["fp64_value"] = value_fp64
@@ -487,7 +508,7 @@ def main():
         traj_steps = args.traj_steps or (500 if args.workload == "c5" else 100)
         extra["trajectory_mixed_vs_fp64"] = trajectory_error(wl, traj_steps)
         print(f"bench: trajectory {extra['trajectory_mixed_vs_fp64']}", file=sys.stderr, flush=True)
-    if args.workload == "c4":
+    if args.workload == "c4" and args.seq_steps > 0:
         sec_s = run_sequential(wl, main_prec, args.seq_steps, 5)
         seq_value = N * args.seq_steps / sec_s
         extra["sequential_value"] = seq_value
@@ -519,6 +540,14 @@ def main():
         bf = force_bytes_per_agent(cbar, rec_bytes, D, 3 ** D, 8 * D)
     t_force_ms = tim_m["force"]["total_ms"] / max(tim_m["force"]["launches"], 1)
     achieved = bf * N / (t_force_ms * 1e-3) / 1e9
+    flops_agent = None
+    achieved_tflops = None
+    if in_range is not None:
+        flops_agent = FLOPS_PER_CANDIDATE * cbar + FLOPS_PER_PAIR * in_range
+    elif nv > 1:  # rods: a vertex-segment distance (~20 flops) per vertex pair of a candidate rod
+        flops_agent = 20.0 * cbar * nv * (nv - 1)
+    if flops_agent is not None:
+        achieved_tflops = flops_agent * N / (t_force_ms * 1e-3) / 1e12
     traffic = profile_figure("hbm_bytes_per_dispatch", force_kernel, args.workload)
     step_bytes = None
     if nv == 1:  # the step's byte model: force + integrator + counting sort (points)
@@ -577,23 +606,28 @@ def main():
         **{k: v for k, v in extra.items() if k in ("fp64_value", "mixed_over_fp64", "force_rel_err_mixed_vs_fp64",
                                                   "force_rms_rel_err_mixed_vs_fp64")},
         "kernel_ms": {k: v["total_ms"] / max(v["launches"], 1) for k, v in tim_m.items()},
+        # The force kernel serves its candidates from LDS and is bound by VALU
+        # instruction issue (the profile's SQ_INSTS_VALU): `achieved` is its useful
+        # FP32 work over the live HIP-event time; the HBM byte model and the
+        # measured HBM traffic stand beside it.
         "roofline": {
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
+            "bound": "valu",
+            "achieved": achieved_tflops,
+            "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved_tflops / FP32_PEAK_TFLOPS if achieved_tflops is not None else None,
             "traffic": traffic,
-            "valu_issue_utilisation": profile_figure("valu_issue_utilisation", force_kernel, args.workload),
-            "traffic_source": PROFILES[args.workload] + " (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated; "
-                              "SQ_INSTS_VALU for the VALU issue utilisation)" if args.workload in PROFILES else None,
+            "valu_issue_frac": profile_figure("valu_issue_utilisation", force_kernel, args.workload),
             "kernel": force_kernel,
-            "bytes_per_agent": bf,
+            "kernel_us": t_force_ms * 1e3,
+            "flops_per_agent": flops_agent,
             "candidates_per_agent": cbar,
-            # achieved/frac use the byte model (every candidate record charged to HBM);
-            # the kernel serves candidates from LDS, so HBM is not what binds it
-            "binding_resource": "VALU issue" if nv == 1 else "VALU issue (divergent rod pairs)",
+            "pairs_in_cutoff_per_agent": in_range,
+            "hbm_model": {"bytes_per_agent": bf, "achieved_GBs": achieved, "frac": achieved / HBM_PEAK_GBS,
+                          "note": "every candidate record charged to HBM (the kernel reads them from LDS)"},
             "hbm_traffic_frac": (traffic / (t_force_ms * 1e-3) / 1e9 / HBM_PEAK_GBS) if traffic else None,
+            "source": PROFILES[args.workload] + " (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated; "
+                      "SQ_INSTS_VALU for the VALU issue fraction)" if args.workload in PROFILES else None,
             "step_model_bytes_per_agent": step_bytes,
             "step_model_frac": (step_bytes * value / 1e9 / HBM_PEAK_GBS) if step_bytes else None,
         },
