@@ -1586,9 +1586,9 @@ __global__ __launch_bounds__(kBlock) void k_gather_integrate_point(Arrays A, Sta
     double x[D], v[D], f[D], nx[D], nv[D], dx[D], dv[D];
 #pragma unroll
     for (int k = 0; k < D; k++) {
-        x[k] = src.pos[k * N + o];
-        v[k] = src.vel[k * N + o];
-        f[k] = A.force[k * N + o];
+        x[k] = __builtin_nontemporal_load(&src.pos[k * N + o]);
+        v[k] = __builtin_nontemporal_load(&src.vel[k * N + o]);
+        f[k] = __builtin_nontemporal_load(&A.force[k * N + o]);
     }
     // the ring before this step (min(t - birth, cap) entries: with cap <= 2 the
     // slot not overwritten now, once filled), in registers, except the slot this
@@ -1601,8 +1601,8 @@ __global__ __launch_bounds__(kBlock) void k_gather_integrate_point(Arrays A, Sta
 #pragma unroll
         for (int k = 0; k < D; k++) {
             const uint64_t e = ((uint64_t)h * D + k) * N + o;
-            hx[h][k] = live ? src.hdx[e] : 0.0;
-            hv[h][k] = live ? src.hdv[e] : 0.0;
+            hx[h][k] = live ? __builtin_nontemporal_load(&src.hdx[e]) : 0.0;
+            hv[h][k] = live ? __builtin_nontemporal_load(&src.hdv[e]) : 0.0;
         }
     }
     auto hist = [&](int i, int k, bool is_dv) {
@@ -1613,8 +1613,8 @@ __global__ __launch_bounds__(kBlock) void k_gather_integrate_point(Arrays A, Sta
     if (out) raise_error(A.ctrl, CRM_BOUNDARY_ERROR, t, id);
 #pragma unroll
     for (int k = 0; k < D; k++) {
-        dst.pos[k * N + s] = nx[k];
-        dst.vel[k * N + s] = nv[k];
+        __builtin_nontemporal_store(nx[k], &dst.pos[k * N + s]);
+        __builtin_nontemporal_store(nv[k], &dst.vel[k * N + s]);
     }
     // ring: the slot written this step gets (dx, dv), the other slot is carried over
     if (rg.cap > 0) {
@@ -1629,8 +1629,8 @@ __global__ __launch_bounds__(kBlock) void k_gather_integrate_point(Arrays A, Sta
                 }
                 if (keep) {
                     const uint64_t e = ((uint64_t)h * D + k) * N + s;
-                    dst.hdx[e] = hx[h][k];
-                    dst.hdv[e] = hv[h][k];
+                    __builtin_nontemporal_store(hx[h][k], &dst.hdx[e]);
+                    __builtin_nontemporal_store(hv[h][k], &dst.hdv[e]);
                 }
             }
         }
