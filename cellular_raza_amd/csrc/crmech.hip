@@ -1681,11 +1681,14 @@ crm_status crm_create(const crm_config* cfg, const crm_instance* instances, uint
             P.dx[a] = g.dx[a];
             P.dxf[a] = (float)g.dx[a];
             P.qinv[a] = 1073741824.0 / g.dx[a];
+            P.inv_dx[a] = 1.0 / g.dx[a];
             P.qunit[a] = (float)(g.dx[a] / 1073741824.0);
             P.n[a] = (uint32_t)g.n[a];
         }
         P.voxel_offset = (uint32_t)voff;
         P.n_voxels = (uint32_t)g.V;
+        P.inv_n2 = 1.0 / (double)P.n[2];
+        P.inv_n12 = 1.0 / ((double)P.n[1] * (double)P.n[2]);
         if (cfg->n_subdomains > 1 && cfg->weighting == CRM_WEIGHT_REFERENCE) {
             // create_subdomains (cartesian_cuboid_n.rs:493-584): lexicographic chunks
             uint64_t nf = 0, avg = 0;

@@ -407,4 +407,34 @@ __device__ __forceinline__ bool axis_index(double p, double mn, double dx, uint3
     return true;
 }
 
+// axis_index with the quotient estimated by a multiplication: (p - mn) * inv_dx
+// is within a few units in the last place of the correctly rounded (p - mn) / dx
+// (3e-16 relative, below 2e-6 absolute for indices under 2^32), so where its
+// fractional part is further than 1e-4 from an integer both truncate to the same
+// index; near integers, at negative or huge values and for NaN the division
+// decides.  Same result as axis_index for every input.
+__device__ __forceinline__ bool axis_index_fast(double p, double mn, double dx, double inv_dx, uint32_t n,
+                                                uint32_t& out) {
+    const double y = (p - mn) * inv_dx;
+    const double fl = floor(y);
+    const double fr = y - fl;
+    if (y >= 0.0 && y < 4294967296.0 && fr > 1e-4 && fr < 1.0 - 1e-4) {
+        const uint32_t i = (uint32_t)fl;
+        if (i >= n) return false;
+        out = i;
+        return true;
+    }
+    return axis_index(p, mn, dx, n, out);
+}
+
+// a / d for a, d < 2^32 with inv_d = 1 / d in double: the estimate is off by at
+// most one, which one remainder test corrects.
+__device__ __forceinline__ uint32_t udiv_by_inv(uint32_t a, uint32_t d, double inv_d) {
+    uint32_t q = (uint32_t)((double)a * inv_d);
+    const int64_t r = (int64_t)a - (int64_t)q * (int64_t)d;
+    if (r < 0) q -= 1u;
+    else if (r >= (int64_t)d) q += 1u;
+    return q;
+}
+
 }  // namespace crm

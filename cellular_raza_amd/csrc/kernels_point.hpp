@@ -1257,7 +1257,7 @@ __global__ __launch_bounds__(kBlock) void k_key_point(Arrays A, StateBuf S, Step
 #pragma unroll
     for (int k = 0; k < D; k++) {
         const int ax = k + 3 - D;
-        ok = ok && axis_index(nx[k], P.min[ax], P.dx[ax], P.n[ax], idx[ax]);
+        ok = ok && axis_index_fast(nx[k], P.min[ax], P.dx[ax], P.inv_dx[ax], P.n[ax], idx[ax]);
     }
     uint32_t key;
     if (ok) {
@@ -1448,9 +1448,10 @@ __device__ __forceinline__ void write_record(const Arrays& A, const InstParams& 
     if constexpr (MIXED) {
         const uint32_t lv = key - P.voxel_offset;
         uint32_t idx[3];
-        idx[2] = lv % P.n[2];
-        idx[1] = (lv / P.n[2]) % P.n[1];
-        idx[0] = lv / (P.n[2] * P.n[1]);
+        const uint32_t row = udiv_by_inv(lv, P.n[2], P.inv_n2);  // lv / n2
+        idx[2] = lv - row * P.n[2];
+        idx[0] = udiv_by_inv(lv, P.n[2] * P.n[1], P.inv_n12);
+        idx[1] = row - idx[0] * P.n[1];
         int q[3] = {0, 0, 0};
         uint32_t outside = 0;
 #pragma unroll
@@ -1490,7 +1491,7 @@ __device__ __forceinline__ void bin_ahead(const Arrays& A, const InstParams& P, 
 #pragma unroll
     for (int k = 0; k < D; k++) {
         const int ax = k + 3 - D;
-        ok = ok && axis_index(nx[k], P.min[ax], P.dx[ax], P.n[ax], idx[ax]);
+        ok = ok && axis_index_fast(nx[k], P.min[ax], P.dx[ax], P.inv_dx[ax], P.n[ax], idx[ax]);
     }
     uint32_t key = vox;
     if (ok) key = P.voxel_offset + (idx[0] * P.n[1] + idx[1]) * P.n[2] + idx[2];

@@ -28,6 +28,8 @@ struct InstParams {
     // grid, 3-D layout (axis 0 has extent 1 in 2-D)
     double min[3], max[3], dx[3];
     double qinv[3];   // 2^30 / dx: fixed-point scale of voxel-relative positions
+    double inv_dx[3]; // 1 / dx (voxel index estimate; see axis_index_fast)
+    double inv_n2, inv_n12;  // 1 / n[2], 1 / (n[1] n[2]) (plain-index decomposition)
     float qunit[3];   // dx / 2^30
     float dxf[3];
     uint32_t n[3];
