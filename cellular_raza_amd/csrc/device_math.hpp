@@ -396,7 +396,7 @@ __device__ __forceinline__ void gaussian_pair(uint64_t seed, uint64_t agent, uin
 // ---------------------------------------------------------------------------
 
 // num-traits to_usize on (p - min) / dx: ok iff -1 < x (NaN fails); trunc.
-__device__ __forceinline__ bool axis_index(double p, double mn, double dx, uint32_t n, uint32_t& out) {
+__host__ __device__ __forceinline__ bool axis_index(double p, double mn, double dx, uint32_t n, uint32_t& out) {
     double x = (p - mn) / dx;
     if (!(x > -1.0)) return false;
     if (!(x < 4294967296.0)) return false;
@@ -413,7 +413,7 @@ __device__ __forceinline__ bool axis_index(double p, double mn, double dx, uint3
 // fractional part is further than 1e-4 from an integer both truncate to the same
 // index; near integers, at negative or huge values and for NaN the division
 // decides.  Same result as axis_index for every input.
-__device__ __forceinline__ bool axis_index_fast(double p, double mn, double dx, double inv_dx, uint32_t n,
+__host__ __device__ __forceinline__ bool axis_index_fast(double p, double mn, double dx, double inv_dx, uint32_t n,
                                                 uint32_t& out) {
     const double y = (p - mn) * inv_dx;
     const double fl = floor(y);
@@ -429,7 +429,7 @@ __device__ __forceinline__ bool axis_index_fast(double p, double mn, double dx, 
 
 // a / d for a, d < 2^32 with inv_d = 1 / d in double: the estimate is off by at
 // most one, which one remainder test corrects.
-__device__ __forceinline__ uint32_t udiv_by_inv(uint32_t a, uint32_t d, double inv_d) {
+__host__ __device__ __forceinline__ uint32_t udiv_by_inv(uint32_t a, uint32_t d, double inv_d) {
     uint32_t q = (uint32_t)((double)a * inv_d);
     const int64_t r = (int64_t)a - (int64_t)q * (int64_t)d;
     if (r < 0) q -= 1u;
