@@ -170,6 +170,8 @@ enum { kNumKernelClasses = CRM_KERNEL_COUNT };
 constexpr uint64_t kNoAllocMaxAgents = 1ull << 22;
 constexpr uint32_t kNoAllocMaxInstances = 1u << 16;
 constexpr size_t kNoAllocScanReserve = (size_t)1 << 20;
+// Steps per long graph replay (even, so that a replay keeps the buffer parity).
+constexpr uint32_t kLongGraphSteps = 16;
 constexpr size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 constexpr size_t kNoAllocScratchBytes = align256(kNoAllocMaxAgents) + 4 * align256(4 * (kNoAllocMaxAgents + 1)) +
                                         align256(4 * (size_t)kNoAllocMaxInstances) + kNoAllocScanReserve;
@@ -266,8 +268,9 @@ struct crm_ctx {
     bool halted = false;
     uint64_t step = 0;
 
-    // graphs for the two buffer parities
-    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    // graphs for the two buffer parities: [parity] replays two steps, [2 + parity]
+    // kLongGraphSteps steps
+    hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
 
     // timing
     hipEvent_t ev_a = nullptr, ev_b = nullptr;
@@ -864,14 +867,14 @@ crm_status download_ids(crm_ctx* ctx, std::vector<uint32_t>& id) {
     return CRM_OK;
 }
 
-crm_status build_graph(crm_ctx* ctx, int parity) {
-    // Two steps return the buffers to the same parity, so one captured
-    // two-step sequence replays for any even number of steps.
+crm_status build_graph(crm_ctx* ctx, int parity, bool long_graph) {
+    // An even number of steps returns the buffers to the same parity, so one
+    // captured sequence of two (or kLongGraphSteps) steps replays from either parity.
     int saved = ctx->cur;
     ctx->cur = parity;
     CRM_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    crm_status st = launch_step(ctx, nullptr);
-    if (st == CRM_OK) st = launch_step(ctx, nullptr);
+    crm_status st = CRM_OK;
+    for (uint32_t k = 0; k < (long_graph ? kLongGraphSteps : 2u) && st == CRM_OK; k++) st = launch_step(ctx, nullptr);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(ctx->stream, &g);
     ctx->cur = saved;
@@ -883,7 +886,7 @@ crm_status build_graph(crm_ctx* ctx, int parity) {
         ctx->err = std::string("HIP error: ") + hipGetErrorString(e) + " at hipStreamEndCapture";
         return CRM_DEVICE_ERROR;
     }
-    e = hipGraphInstantiate(&ctx->graph[parity], g, nullptr, nullptr, 0);
+    e = hipGraphInstantiate(&ctx->graph[(long_graph ? 2 : 0) + parity], g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) {
         ctx->err = std::string("HIP error: ") + hipGetErrorString(e) + " at hipGraphInstantiate";
@@ -1979,9 +1982,14 @@ crm_status crm_step(crm_ctx* ctx, uint32_t n_steps, const double* noise) {
         ctx->graphs_stale = false;
     }
     uint32_t s = 0;
-    if (graphs) {
+    if (graphs) {  // each graph launch costs the device a few microseconds
+        while (s + kLongGraphSteps <= n_steps) {
+            if (!ctx->graph[2 + ctx->cur]) CRM_TRY(build_graph(ctx, ctx->cur, true));
+            CRM_HIP(hipGraphLaunch(ctx->graph[2 + ctx->cur], ctx->stream));
+            s += kLongGraphSteps;
+        }
         while (s + 2 <= n_steps) {
-            if (!ctx->graph[ctx->cur]) CRM_TRY(build_graph(ctx, ctx->cur));
+            if (!ctx->graph[ctx->cur]) CRM_TRY(build_graph(ctx, ctx->cur, false));
             CRM_HIP(hipGraphLaunch(ctx->graph[ctx->cur], ctx->stream));
             s += 2;
         }

@@ -296,6 +296,7 @@ def test_graph_and_eager_bitwise_equal_and_deterministic():
                              precision="mixed", graph=graph)
         g.set_agents(wl.pos, wl.vel, wl.species)
         g.step(7)
+        g.step(37)  # two long replays, two two-step replays and a single step
         runs.append(g.get_state())
         g.close()
     for r in runs[1:]:
@@ -334,14 +335,14 @@ def test_boundary_error_same_step_as_oracle():
     assert inf.step == 1  # the two steps after the failing one did not run
 
 
-@pytest.mark.parametrize("graph", [True, False])
+@pytest.mark.parametrize("graph,n_steps", [(True, 10), (True, 40), (False, 10)])
 @pytest.mark.parametrize("case", ["newton", "brownian"])
-def test_error_halts_at_the_failing_step(case, graph):
+def test_error_halts_at_the_failing_step(case, graph, n_steps):
     """The step loop breaks at the first step that fails (run_sim.rs:380-403): a
-    10-step call whose step 4 (NewtonDamped, anti-damped so the speed grows; agents
-    binned ahead by the gather) or step 3 (Brownian, binned after the forces) leaves
-    the domain runs no later step, in graph replays of two steps as in single
-    launches.  The failing step is the oracle's, the step count stops at its end, and
+    10- or 40-step call whose step 4 (NewtonDamped, anti-damped so the speed grows;
+    agents binned ahead by the gather) or step 3 (Brownian, binned after the forces)
+    leaves the domain runs no later step, in graph replays of two or sixteen steps
+    as in single launches.  The failing step is the oracle's, the step count stops at its end, and
     the state equals that of a context stepped one step at a time up to the failure."""
     dom = CartesianCuboid.from_boundaries_and_interaction_range([0.0, 0.0], [10.0, 10.0], 2.0)
     rng = np.random.default_rng(1)
@@ -361,7 +362,7 @@ def test_error_halts_at_the_failing_step(case, graph):
     g = MechanicsBackend(spec, dt=0.1, mechanics_solver_order=2, graph=graph, **kw)
     g.set_agents(pos, vel)
     with pytest.raises(SimulationError) as e:
-        g.step(10)
+        g.step(n_steps)
     assert e.value.kind == eo.value.kind == "BoundaryError"
     inf = g.info()
     assert inf.error_step_valid and inf.error_step == t_err and inf.step == t_err + 1
