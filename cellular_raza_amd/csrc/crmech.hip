@@ -209,7 +209,8 @@ struct crm_ctx {
     Control* d_ctrl = nullptr;
     uint32_t* d_start = nullptr;
     uint32_t* d_count = nullptr;
-    uint32_t* d_bsum = nullptr;
+    uint32_t* d_start_local = nullptr;  // [Vt] voxel offsets inside their scan tile
+    uint32_t* d_bsum = nullptr;         // [n_tiles] scan tile sums, then offsets
     uint32_t n_tiles = 0;
     uint32_t* d_key = nullptr;
     uint32_t* d_rank = nullptr;
@@ -311,8 +312,8 @@ void free_device(crm_ctx* ctx) {
             g = nullptr;
         }
     ctx->graphs_stale = false;
-    void* ptrs[] = {ctx->d_work, ctx->d_tiles, ctx->d_start, ctx->d_count, ctx->d_bsum, ctx->d_key, ctx->d_rank,
-                    ctx->d_perm, ctx->d_force, ctx->d_rec64, ctx->d_recq, ctx->d_noise, ctx->d_recrod, ctx->d_bnd,
+    void* ptrs[] = {ctx->d_work, ctx->d_tiles, ctx->d_start, ctx->d_count, ctx->d_start_local, ctx->d_bsum,
+                    ctx->d_key, ctx->d_rank, ctx->d_perm, ctx->d_force, ctx->d_rec64, ctx->d_recq, ctx->d_noise, ctx->d_recrod, ctx->d_bnd,
                     ctx->d_sense, ctx->d_snap, ctx->d_snap_sp, ctx->d_snap_rod, ctx->d_snap_id, ctx->d_snap_st, ctx->d_divflag, ctx->d_drank, ctx->d_dctr,
                     ctx->d_ndiv, ctx->d_dbefore, ctx->d_vcount, ctx->d_scan_tmp, ctx->d_inst_div, ctx->d_rm};
     for (void* p : ptrs)
@@ -326,7 +327,7 @@ void free_device(crm_ctx* ctx) {
     ctx->d_work = nullptr;
     ctx->d_tiles = nullptr;
     ctx->tiles_cap = 0;
-    ctx->d_start = ctx->d_count = ctx->d_bsum = ctx->d_key = ctx->d_rank = ctx->d_perm = nullptr;
+    ctx->d_start = ctx->d_count = ctx->d_start_local = ctx->d_bsum = ctx->d_key = ctx->d_rank = ctx->d_perm = nullptr;
     ctx->d_force = nullptr;
     ctx->d_rec64 = nullptr;
     ctx->d_recq = nullptr;
@@ -364,6 +365,7 @@ Arrays arrays(crm_ctx* ctx) {
     A.work = ctx->d_work;
     A.ctrl = ctx->d_ctrl;
     A.start = ctx->d_start;
+    A.start_local = ctx->d_start_local;
     A.tile_off = ctx->d_bsum;
     A.count = ctx->d_count;
     A.stay = ctx->d_count + stay_offset(ctx);
@@ -627,17 +629,21 @@ crm_status launch_sort(crm_ctx* ctx, bool advance, const StepArgs* sa) {
     const uint32_t Vt = (uint32_t)ctx->Vt;
     CRM_TRY(t_begin(ctx));
     hipLaunchKernelGGL(k_scan_local, dim3(ctx->n_tiles), dim3(kBlock), 0, ctx->stream, ctx->d_count,
-                       ctx->d_count + stay_offset(ctx), ctx->d_start,
-                       ctx->d_bsum, Vt, ctx->d_ctrl);
+                       ctx->d_count + stay_offset(ctx), ctx->d_start_local, ctx->d_bsum, Vt, ctx->d_ctrl);
     hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->d_bsum, ctx->n_tiles, ctx->d_start,
                        Vt, ctx->d_ctrl, (uint32_t)(advance ? 1 : 0));
     CRM_TRY(t_end(ctx, CRM_KERNEL_SCAN));
     CRM_TRY(t_begin(ctx));
-    hipLaunchKernelGGL(k_scatter, dim3((uint32_t)((ctx->N + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream, A);
+    const uint64_t n_scatter = std::max<uint64_t>(ctx->N, Vt);
+    hipLaunchKernelGGL(k_scatter, dim3((uint32_t)((n_scatter + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream,
+                       A);
     CRM_TRY(t_end(ctx, CRM_KERNEL_SCATTER));
-    CRM_TRY(t_begin(ctx));
-    hipLaunchKernelGGL(k_fixup, dim3((Vt + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, A);
-    CRM_TRY(t_end(ctx, CRM_KERNEL_FIXUP));
+    // the point gather of a step orders each voxel's arrivals itself
+    if (is_rod(ctx) || !advance) {
+        CRM_TRY(t_begin(ctx));
+        hipLaunchKernelGGL(k_fixup, dim3((Vt + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, A);
+        CRM_TRY(t_end(ctx, CRM_KERNEL_FIXUP));
+    }
     CRM_TRY(t_begin(ctx));
     StateBuf src = ctx->buf[ctx->cur], dst = ctx->buf[1 - ctx->cur];
     const uint32_t cap = ctx->cfg.solver_order;
@@ -1812,6 +1818,7 @@ crm_status crm_set_agents(crm_ctx* ctx, const uint64_t* counts, const double* po
     }
     CRM_TRY(dalloc(ctx, ctx->d_start, Vt + 1));
     CRM_TRY(dalloc(ctx, ctx->d_count, stay_offset(ctx) + Vt));  // arrivals, then stayers
+    CRM_TRY(dalloc(ctx, ctx->d_start_local, Vt));
     CRM_TRY(dalloc(ctx, ctx->d_bsum, ctx->n_tiles));
     // N-sized arrays hold `C` agents: the rods of a cycle grow up to max_agents
     const uint64_t C = (is_rod(ctx) && ctx->cycle_on) ? std::max<uint64_t>(N, ctx->cyc.max_agents) : N;

@@ -7,11 +7,12 @@
 //                      (cartesian_cuboid_n.rs:469-491); NewtonDamped and
 //                      Langevin get it from the previous gather instead
 //   k_rank_ahead       ranks: stayers in order, arrivals atomic         } integer counting sort:
-//   k_scan_local/tiles exclusive scan of the voxel histogram            } stayers keep their order,
-//   k_scatter          slot -> sorted position                          } movers appended by
-//   k_fixup            arrivals by previous slot, histogram reset       } previous slot (sort_cells_in_voxels,
-//   k_gather_integrate_point  the full mechanics update, written into   }  update_mechanics.rs:418-513)
-//                      the agent's new sorted slot, and the force records
+//   k_scan_local/tiles exclusive scan of the voxel histogram, its reset } stayers keep their order,
+//   k_scatter          slot -> sorted position, final voxel starts      } movers appended by
+//   k_gather_integrate_point  arrivals put in order of previous slot,   } previous slot (sort_cells_in_voxels,
+//                      the full mechanics update written into the       }  update_mechanics.rs:418-513)
+//                      agent's new sorted slot, and the force records
+//   (set-up sorts and rods order the arrivals with k_fixup)
 //
 // Agents live in SoA arrays sorted by voxel; a voxel's agents are the slots
 // [start[v], start[v+1]).
@@ -47,8 +48,9 @@ struct Arrays {
     const InstParams* ip;
     const BlockWork* work;
     Control* ctrl;
-    uint32_t* start;  // [Vt + 1] (offsets inside the scan tile between k_scan_local and k_fixup)
-    uint32_t* tile_off;  // [n_tiles] offset of each scan tile (k_scan_tiles)
+    uint32_t* start;  // [Vt + 1]
+    uint32_t* start_local;  // [Vt] offset of each voxel inside its scan tile (k_scan_local)
+    uint32_t* tile_off;     // [n_tiles] offset of each scan tile (k_scan_tiles)
     uint32_t* count;  // [Vt] arrivals per next voxel (atomic ranks)
     uint32_t* stay;   // [Vt] agents binned ahead that stay in their voxel (zero otherwise)
     uint32_t* key;    // [N] next voxel
@@ -1317,12 +1319,14 @@ constexpr int kScanItems = 8;
 constexpr int kScanTile = kBlock * kScanItems;
 
 
-// Each thread scans kScanItems consecutive voxels; whole groups move as two
-// 16-byte loads per array and two 16-byte stores (count, stay and start are
-// 256-byte aligned and a group starts at a multiple of 8 voxels).
-__global__ __launch_bounds__(kBlock) void k_scan_local(const uint32_t* __restrict__ count,
+// Each thread scans kScanItems consecutive voxels of count + stay into their
+// offsets inside the scan tile (start_local) and clears count for the next step's
+// ranks; whole groups move as two 16-byte loads per array and two 16-byte stores
+// (count, stay and start_local are 256-byte aligned and a group starts at a
+// multiple of 8 voxels).  bsum[tile] = the tile's sum.
+__global__ __launch_bounds__(kBlock) void k_scan_local(uint32_t* __restrict__ count,
                                                         const uint32_t* __restrict__ stay,
-                                                        uint32_t* __restrict__ start, uint32_t* __restrict__ bsum,
+                                                        uint32_t* __restrict__ start_local, uint32_t* __restrict__ bsum,
                                                         uint32_t Vt, const Control* ctrl) {
     if (halted(ctrl, ctrl->step)) return;
     static_assert(kScanItems == 8, "two uint4 per group");
@@ -1330,7 +1334,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_local(const uint32_t* __restric
     const bool whole = base + kScanItems <= Vt;
     uint32_t vals[kScanItems];
     if (whole) {
-        const uint4* c4 = reinterpret_cast<const uint4*>(count + base);
+        uint4* c4 = reinterpret_cast<uint4*>(count + base);
         const uint4* s4 = reinterpret_cast<const uint4*>(stay + base);
         const uint4 c0 = c4[0], c1 = c4[1], s0 = s4[0], s1 = s4[1];
         vals[0] = c0.x + s0.x;
@@ -1341,11 +1345,14 @@ __global__ __launch_bounds__(kBlock) void k_scan_local(const uint32_t* __restric
         vals[5] = c1.y + s1.y;
         vals[6] = c1.z + s1.z;
         vals[7] = c1.w + s1.w;
+        c4[0] = make_uint4(0u, 0u, 0u, 0u);
+        c4[1] = make_uint4(0u, 0u, 0u, 0u);
     } else {
 #pragma unroll
         for (int i = 0; i < kScanItems; i++) {
             const uint64_t q = base + i;
             vals[i] = q < Vt ? count[q] + stay[q] : 0u;
+            if (q < Vt) count[q] = 0u;
         }
     }
     uint32_t sum = 0;
@@ -1360,20 +1367,20 @@ __global__ __launch_bounds__(kBlock) void k_scan_local(const uint32_t* __restric
         run += vals[i];
     }
     if (whole) {
-        uint4* o4 = reinterpret_cast<uint4*>(start + base);
+        uint4* o4 = reinterpret_cast<uint4*>(start_local + base);
         o4[0] = make_uint4(out[0], out[1], out[2], out[3]);
         o4[1] = make_uint4(out[4], out[5], out[6], out[7]);
     } else {
 #pragma unroll
         for (int i = 0; i < kScanItems; i++)
-            if (base + i < Vt) start[base + i] = out[i];
+            if (base + i < Vt) start_local[base + i] = out[i];
     }
     if (threadIdx.x == 0) bsum[blockIdx.x] = total;
 }
 
-// Single block: exclusive scan of the tile sums in place; start[Vt] = total.
-// advance != 0 also completes the step counter (every kernel that reads it for
-// this step has run; k_gather_point reads the new value).
+// Single block: exclusive scan of the tile sums in place (the tile offsets);
+// start[Vt] = total.  advance != 0 also completes the step counter (every kernel
+// that reads it for this step has run; the gather reads the new value).
 __global__ __launch_bounds__(kBlock) void k_scan_tiles(uint32_t* __restrict__ bsum, uint32_t n_tiles,
                                                         uint32_t* __restrict__ start, uint32_t Vt, Control* ctrl,
                                                         uint32_t advance) {
@@ -1396,36 +1403,34 @@ __global__ __launch_bounds__(kBlock) void k_scan_tiles(uint32_t* __restrict__ bs
     }
 }
 
-// Between k_scan_local and k_fixup, start[v] holds v's offset inside its scan
-// tile and tile_off[v / kScanTile] (scanned by k_scan_tiles) the tile's offset.
+// Thread g writes the final start of voxel g (tile-local offset + tile offset) and
+// places agent g in the slot list of its next voxel: a stayer at its rank, an
+// arrival (kMoved) after the voxel's stayers in atomic order.  Launched over
+// max(N, Vt) threads.
 __global__ __launch_bounds__(kBlock) void k_scatter(Arrays A) {
     if (A.ctrl->skip) return;
-    const uint64_t s = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (s >= A.N) return;
-    const uint32_t k = A.key[s], r = A.rank[s];
+    const uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (g < A.Vt) A.start[g] = A.start_local[g] + A.tile_off[g / kScanTile];
+    if (g >= A.N) return;
+    const uint32_t k = A.key[g], r = A.rank[g];
     const uint32_t pos = (r & kMoved) ? A.stay[k] + (r & ~kMoved) : r;
-    A.perm[A.start[k] + A.tile_off[k / kScanTile] + pos] = (uint32_t)s;
+    A.perm[A.start_local[k] + A.tile_off[k / kScanTile] + pos] = (uint32_t)g;
 }
 
-// Writes the final voxel starts (tile-local offset + tile offset; the end from
-// the voxel's size, so no thread reads a start another one writes), orders each
-// voxel's arrivals by previous slot (atomic arrival order is not deterministic)
-// and clears the histogram for the next step.  The resulting list is the
-// reference's: the cells that stay keep their order and the movers are appended
-// in order of old voxel, then old position (sort_cells_in_voxels_step_1/2,
-// update_mechanics.rs:418-513).  Set-up sorts rank every agent atomically with
-// stay = 0 and get the whole voxel in order of previous slot.
+// Orders each voxel's arrivals by previous slot (atomic arrival order is not
+// deterministic).  The resulting list is the reference's: the cells that stay keep
+// their order and the movers are appended in order of old voxel, then old position
+// (sort_cells_in_voxels_step_1/2, update_mechanics.rs:418-513).  Set-up sorts rank
+// every agent atomically with stay = 0 and get the whole voxel in order of previous
+// slot.  The point gather of a step orders the arrivals itself (arrival_slot), so
+// the step's sort launches this kernel only for rods.
 __global__ __launch_bounds__(kBlock) void k_fixup(Arrays A) {
     if (A.ctrl->skip) return;
     const uint64_t v = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     if (v >= A.Vt) return;
-    const uint32_t stay = A.stay[v];
-    const uint32_t b = A.start[v] + A.tile_off[v / kScanTile], e = b + A.count[v] + stay;
-    A.start[v] = b;
-    A.count[v] = 0u;
     // the stayers hold [b, b + stay) in their old order; the arrivals follow, put in
     // order of previous slot, i.e. of (old voxel, old position)
-    const uint32_t m = b + stay;
+    const uint32_t m = A.start[v] + A.stay[v], e = A.start[v + 1];
     for (uint32_t i = m + 1; i < e; i++) {
         uint32_t x = A.perm[i];
         uint32_t j = i;
@@ -1437,6 +1442,18 @@ __global__ __launch_bounds__(kBlock) void k_fixup(Arrays A) {
         }
         A.perm[j] = x;
     }
+}
+
+// Slot of the agent from previous slot o that k_scatter put at slot i of voxel key:
+// an arrival (kMoved rank) belongs after the voxel's stayers at the number of the
+// voxel's arrivals with a smaller previous slot, the order k_fixup gives.
+__device__ __forceinline__ uint32_t arrival_slot(const Arrays& A, uint32_t i, uint32_t o, uint32_t key) {
+    if (!(A.rank[o] & kMoved)) return i;
+    const uint32_t m = A.start[key] + A.stay[key], e = A.start[key + 1];
+    if (e - m < 2u) return i;
+    uint32_t r = 0;
+    for (uint32_t j = m; j < e; j++) r += A.perm[j] < o ? 1u : 0u;
+    return m + r;
 }
 
 // Force record of slot s: absolute double coordinates (FP64), or 30-bit fixed
@@ -1565,22 +1582,25 @@ __global__ __launch_bounds__(kBlock) void k_gather_point(Arrays A, StateBuf src,
     if constexpr (AHEAD) bin_ahead<D, MECH>(A, P, sa, t, id, s, p, v, r0, r1, key, birth);
 }
 
-// Gather + integrate: the agent that lands in sorted slot s comes from slot
-// o = perm[s]; its update (the same point_step as in k_key_point, so the same
-// position) is computed here and the new state written straight to slot s,
-// together with the force record for the next step.  k_scan_tiles has already
-// advanced the step counter, so this step started after step - 1 steps.
+// Gather + integrate: thread i moves the agent from slot o = perm[i] to its sorted
+// slot s (i itself, or arrival_slot's place among the voxel's arrivals); its
+// update (the same point_step as in k_key_point, so the same position) is computed
+// here and the new state written straight to slot s, together with the force
+// record for the next step.  k_scan_tiles has already advanced the step counter,
+// so this step started after step - 1 steps.
 template <int D, bool MIXED, int MECH>
 __global__ __launch_bounds__(kBlock) void k_gather_integrate_point(Arrays A, StateBuf src, StateBuf dst,
                                                                      StepArgs sa) {
     if (A.ctrl->skip) return;
     const BlockWork w = A.work[blockIdx.x];
     if (threadIdx.x >= w.count) return;
-    const uint64_t s = w.first + threadIdx.x;
+    const uint32_t i = (uint32_t)(w.first + threadIdx.x);
     const InstParams& P = A.ip[w.inst];
     const uint64_t N = A.N;
     const uint64_t t = A.ctrl->step - 1;
-    const uint32_t o = A.perm[s];
+    const uint32_t o = A.perm[i];
+    const uint32_t key = A.key[o];
+    const uint64_t s = arrival_slot(A, i, o, key);
     const uint32_t id = src.id[o];
     const uint64_t birth = src.birth ? src.birth[o] : 0u;
     const Ring rg = ring_agent(t, birth, sa.cap, sa.newest_first);
@@ -1639,7 +1659,6 @@ __global__ __launch_bounds__(kBlock) void k_gather_integrate_point(Arrays A, Sta
     const int32_t sp = src.species[o];
     dst.species[s] = sp;
     dst.id[s] = id;
-    const uint32_t key = A.key[o];
     dst.vox[s] = key;
     write_record<D, MIXED>(A, P, s, key, nx, sp);
     if (dst.birth) dst.birth[s] = (uint32_t)birth;
