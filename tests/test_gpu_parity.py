@@ -203,6 +203,41 @@ def test_voxels_and_neighbour_sets_bit_exact():
         o.step(3)
 
 
+@pytest.mark.parametrize("mech", ["brownian", "langevin"])
+@pytest.mark.parametrize("graph", [True, False])
+def test_crowded_arrivals_keep_previous_slot_order(mech, graph):
+    """Many agents per voxel and large moves: every step brings several agents into
+    the same voxel, which the gather must put after the voxel's stayers in order of
+    old voxel, then old position (sort_cells_in_voxels_step_2,
+    update_mechanics.rs:470-513).  Voxel lists bit-exact with the oracle every step,
+    Brownian (binned after the forces) and Langevin (binned ahead)."""
+    dom = CartesianCuboid.from_boundaries_and_interaction_range([0.0, 0.0], [20.0, 20.0], 2.0)
+    pot = MorsePotential(1.0, 0.2, 2.0, 0.05)
+    m = Brownian(2.0, 1.0) if mech == "brownian" else Langevin(1.0, 0.5, 20.0)
+    spec = [InstanceSpec(dom, pot, m)]
+    rng = np.random.default_rng(61)
+    pos = rng.uniform(0.5, 19.5, size=(1500, 2))
+    vel = rng.normal(0.0, 4.0, size=(1500, 2))
+    g = MechanicsBackend(spec, dt=0.05, mechanics_solver_order=2, noise="philox", seed=9, graph=graph)
+    o = OracleSim(spec, dt=0.05, mechanics_solver_order=2, noise="philox", seed=9)
+    g.set_agents(pos, vel)
+    o.set_agents(pos, vel)
+    most = 0
+    for _ in range(6):
+        before = g.get_voxels()[0]
+        g.step(2)
+        o.step(2)
+        after = g.get_voxels()[0]
+        moved = after[before != after]
+        if moved.size:
+            most = max(most, int(np.bincount(moved).max()))
+        assert np.array_equal(g.voxel_order(), o.get_voxels()[2].astype(np.uint32))
+        pg, po = g.get_state()[0], o.get_state()[0]
+        assert rel_err(pg, po) <= TRAJ_TOL
+    assert most >= 3  # voxels received several agents in the same two steps
+    g.close()
+
+
 def test_mixed_precision_force_error():
     wl = workloads.c2_sorting_3d(n=20000, seed=13, steps=10)
     g, o = pair(wl, precision="mixed")
