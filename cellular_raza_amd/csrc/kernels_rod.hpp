@@ -725,12 +725,16 @@ __global__ __launch_bounds__(kBlock) void k_gather_rod(Arrays A, StateBuf src, S
     __shared__ double aheadpos[AHEAD ? kRodLdsK : 1];
     if (A.ctrl->skip) return;
     const BlockWork w = A.work[blockIdx.x];
-    uint64_t s;
-    uint32_t v;
-    const bool active = rod_lane(w, R.nv, s, v);
+    const uint32_t nv = R.nv, K = nv * D;
+    // rods fastest across the lanes: thread t takes vertex row t / rpb of the
+    // block's rod t % rpb, so that a wave's loads and stores of one component row
+    // cover consecutive slots.  lt = rod * nv + vertex addresses the LDS rows.
+    const uint32_t rr = threadIdx.x % R.rpb, v = threadIdx.x / R.rpb;
+    const uint64_t s = w.first + rr;
+    const bool active = v < nv && rr < w.count;
+    const uint32_t lt = rr * nv + v;
     const InstParams& P = A.ip[w.inst];
     const uint64_t N = A.N;
-    const uint32_t nv = R.nv, K = nv * D;
     double nx[D], nvel[D];
     if constexpr (INTEGRATE) {  // bending window v of the rod, shared with the next two lanes
         if (active && v + 2 < nv) {
@@ -739,7 +743,7 @@ __global__ __launch_bounds__(kBlock) void k_gather_rod(Arrays A, StateBuf src, S
             double F[D];
             rod_bend_window<D>(P, pos, (int)v, F);
 #pragma unroll
-            for (int k = 0; k < D; k++) win[threadIdx.x * 3 + k] = F[k];
+            for (int k = 0; k < D; k++) win[lt * 3 + k] = F[k];
         }
         __syncthreads();
     }
@@ -758,7 +762,7 @@ __global__ __launch_bounds__(kBlock) void k_gather_rod(Arrays A, StateBuf src, S
                 vv[k] = vel((int)v, k);
                 f[k] = A.force[(uint64_t)(v * D + k) * N + o];
             }
-            auto wn = [&](int i, int k) { return win[(threadIdx.x - v + (uint32_t)i) * 3 + k]; };
+            auto wn = [&](int i, int k) { return win[(lt - v + (uint32_t)i) * 3 + k]; };
             rod_row_increment<D>(P, src.spring[o], nv, v, pos, vel, wn, f, dx, dv);
             rod_row_noise<D>(P, sa, id, t, nv, v, dxr);
             auto hist = [&](int i, int k, bool is_dv) {
@@ -775,7 +779,7 @@ __global__ __launch_bounds__(kBlock) void k_gather_rod(Arrays A, StateBuf src, S
                     const uint32_t sl = rg1.it[i];
                     return sl == rg.wslot ? dx[k] : src.hdx[((uint64_t)sl * K + v * D + k) * N + o];
                 };
-                rod_ahead<D>(P, sa, t + 1, id, nv, v, nx, nvel, hist1, aheadpos + threadIdx.x * 3);
+                rod_ahead<D>(P, sa, t + 1, id, nv, v, nx, nvel, hist1, aheadpos + lt * 3);
             }
             if (rg.cap > 0) {
                 const uint32_t n_hist = (uint32_t)min<uint64_t>(t + 1, rg.cap);
@@ -810,15 +814,14 @@ __global__ __launch_bounds__(kBlock) void k_gather_rod(Arrays A, StateBuf src, S
                 auto hist1 = [&](int i, int k, bool) {
                     return src.hdx[((uint64_t)rg1.it[i] * K + v * D + k) * N + o];
                 };
-                rod_ahead<D>(P, sa, t1, id, nv, v, nx, nvel, hist1, aheadpos + threadIdx.x * 3);
+                rod_ahead<D>(P, sa, t1, id, nv, v, nx, nvel, hist1, aheadpos + lt * 3);
             }
         }
 #pragma unroll
         for (int k = 0; k < D; k++) {
             dst.pos[(uint64_t)(v * D + k) * N + s] = nx[k];
             dst.vel[(uint64_t)(v * D + k) * N + s] = nvel[k];
-            R.rec[s * K + v * D + k] = nx[k];
-            newpos[threadIdx.x * 3 + k] = nx[k];
+            newpos[lt * 3 + k] = nx[k];
         }
         if (v == 0) {
             dst.species[s] = src.species[o];
@@ -828,12 +831,18 @@ __global__ __launch_bounds__(kBlock) void k_gather_rod(Arrays A, StateBuf src, S
         }
     }
     __syncthreads();
+    // the force records rec[slot][K] of the block's rods, a contiguous range,
+    // written element by element from LDS
+    for (uint32_t e = threadIdx.x; e < w.count * K; e += kBlock) {
+        const uint32_t r = e / K, c = e - r * K, vq = c / D;
+        R.rec[w.first * K + e] = newpos[(r * nv + vq) * 3 + (c - vq * D)];
+    }
     // bounding ball of the rod (first lane): vertex mean and the largest distance to it
     if (active && v == 0) {
         double c[3] = {0.0, 0.0, 0.0};
         for (uint32_t q = 0; q < nv; q++) {
 #pragma unroll
-            for (int k = 0; k < D; k++) c[k] = c[k] + newpos[(threadIdx.x + q) * 3 + k];
+            for (int k = 0; k < D; k++) c[k] = c[k] + newpos[(lt + q) * 3 + k];
         }
 #pragma unroll
         for (int k = 0; k < D; k++) c[k] = c[k] / (double)nv;
@@ -842,7 +851,7 @@ __global__ __launch_bounds__(kBlock) void k_gather_rod(Arrays A, StateBuf src, S
             double d2 = 0.0;
 #pragma unroll
             for (int k = 0; k < D; k++) {
-                const double d = newpos[(threadIdx.x + q) * 3 + k] - c[k];
+                const double d = newpos[(lt + q) * 3 + k] - c[k];
                 d2 = d2 + d * d;
             }
             r2 = fmax(r2, d2);
@@ -855,7 +864,7 @@ __global__ __launch_bounds__(kBlock) void k_gather_rod(Arrays A, StateBuf src, S
             bool same = true;
 #pragma unroll
             for (int k = 0; k < D; k++)
-                same = same && newpos[(threadIdx.x + q) * 3 + k] == newpos[(threadIdx.x + q + 1) * 3 + k];
+                same = same && newpos[(lt + q) * 3 + k] == newpos[(lt + q + 1) * 3 + k];
             degenerate = degenerate || same;
         }
         // radius rounded up by a relative 1e-12 so that it covers every vertex
@@ -868,7 +877,7 @@ __global__ __launch_bounds__(kBlock) void k_gather_rod(Arrays A, StateBuf src, S
             for (int k = 0; k < D; k++) c1[k] = 0.0;
             for (uint32_t q = 0; q < nv; q++) {
 #pragma unroll
-                for (int k = 0; k < D; k++) c1[k] = c1[k] + aheadpos[(threadIdx.x + q) * 3 + k];
+                for (int k = 0; k < D; k++) c1[k] = c1[k] + aheadpos[(lt + q) * 3 + k];
             }
             uint32_t idx[3] = {0, 0, 0};
             bool ok = true;
