@@ -224,6 +224,7 @@ struct crm_ctx {
     size_t noise_cap = 0;
     double* d_recrod = nullptr;  // rods: AoS vertex positions [N][K]
     double4* d_bnd = nullptr;    // rods: bounding balls [N]
+    float4* d_seg = nullptr;     // rods: segment balls [N][nv - 1]
     uint32_t rods_per_block = 0;
     // neighbour sensing: counts by slot ([N][nv] for rods) of the last force
     // evaluation, which read state buffer sense_parity (-1: none since enabled)
@@ -314,7 +315,7 @@ void free_device(crm_ctx* ctx) {
     ctx->graphs_stale = false;
     void* ptrs[] = {ctx->d_work, ctx->d_tiles, ctx->d_start, ctx->d_count, ctx->d_start_local, ctx->d_bsum,
                     ctx->d_key, ctx->d_rank, ctx->d_perm, ctx->d_force, ctx->d_rec64, ctx->d_recq, ctx->d_noise, ctx->d_recrod, ctx->d_bnd,
-                    ctx->d_sense, ctx->d_snap, ctx->d_snap_sp, ctx->d_snap_rod, ctx->d_snap_id, ctx->d_snap_st, ctx->d_divflag, ctx->d_drank, ctx->d_dctr,
+                    ctx->d_seg, ctx->d_sense, ctx->d_snap, ctx->d_snap_sp, ctx->d_snap_rod, ctx->d_snap_id, ctx->d_snap_st, ctx->d_divflag, ctx->d_drank, ctx->d_dctr,
                     ctx->d_ndiv, ctx->d_dbefore, ctx->d_vcount, ctx->d_scan_tmp, ctx->d_inst_div, ctx->d_rm};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -334,6 +335,7 @@ void free_device(crm_ctx* ctx) {
     ctx->d_noise = nullptr;
     ctx->d_recrod = nullptr;
     ctx->d_bnd = nullptr;
+    ctx->d_seg = nullptr;
     ctx->d_sense = nullptr;
     ctx->sense_parity = -1;
     ctx->d_snap = nullptr;
@@ -391,6 +393,7 @@ RodArgs rod_args(crm_ctx* ctx) {
     RodArgs R;
     R.rec = ctx->d_recrod;
     R.bnd = ctx->d_bnd;
+    R.seg = ctx->d_seg;
     R.nv = ctx->nv;
     R.rpb = ctx->rods_per_block;
     return R;
@@ -1830,6 +1833,7 @@ crm_status crm_set_agents(crm_ctx* ctx, const uint64_t* counts, const double* po
     if (is_rod(ctx)) {
         CRM_TRY(dalloc(ctx, ctx->d_recrod, (size_t)K * C));
         CRM_TRY(dalloc(ctx, ctx->d_bnd, C));
+        CRM_TRY(dalloc(ctx, ctx->d_seg, (size_t)(ctx->nv - 1) * C));
     }
     else if (ctx->cfg.precision == CRM_PRECISION_MIXED) CRM_TRY(dalloc(ctx, ctx->d_recq, N));
     else CRM_TRY(dalloc(ctx, ctx->d_rec64, N));

@@ -22,6 +22,9 @@ namespace crm {
 struct RodArgs {
     double* rec;   // [N][K] AoS vertex positions (force records)
     double4* bnd;  // [N] bounding ball of each rod: vertex mean (x, y, z), radius
+    float4* seg;   // [N][nv - 1] ball of each segment: midpoint minus the rod's vertex
+                   // mean (x, y, z), half length with margin (infinite for a
+                   // zero-length segment)
     uint32_t nv;   // vertices per rod
     uint32_t rpb;  // rods per workgroup
 };
@@ -221,14 +224,31 @@ __global__ __launch_bounds__(kBlock) void k_force_rod(Arrays A, StateBuf S, RodA
         for (uint32_t c = 0; c < per; c++) {
             const uint32_t j = cur.j;
             if (j != s) {
+                // v relative to j's vertex mean in float, then to each segment ball
+                // (their offsets from the mean are small, so the float differences
+                // stay far inside the reach margin)
                 const double4 bj = R.bnd[j];
                 const double cj[3] = {bj.x, bj.y, bj.z};
-                if (!out_of_reach(pv, cj, (float)bj.w * (1.0f + 1e-6f))) {
-                    const double* ext = R.rec + (uint64_t)j * K;
+                float dc[D], d2 = 0.0f;
+#pragma unroll
+                for (int k = 0; k < D; k++) {
+                    dc[k] = (float)(pv[k] - cj[k]);
+                    d2 = fmaf(dc[k], dc[k], d2);
+                }
+                const float lim = (float)bj.w * (1.0f + 1e-6f) + reach;
+                if (!(d2 > lim * lim)) {
+                    const float4* sg = R.seg + (uint64_t)j * nseg;
                     for (uint32_t q = 0; q < nseg; q++) {
-                        double m[D];
-                        const float h = seg_ball(ext + q * D, ext + (q + 1) * D, m);
-                        if (!out_of_reach(pv, m, h)) mo |= 1ull << (c * nseg + q);
+                        const float4 g = sg[q];
+                        const float go[3] = {g.x, g.y, g.z};
+                        float e2 = 0.0f;
+#pragma unroll
+                        for (int k = 0; k < D; k++) {
+                            const float e = dc[k] - go[k];
+                            e2 = fmaf(e, e, e2);
+                        }
+                        const float l = g.w + reach;
+                        if (!(e2 > l * l)) mo |= 1ull << (c * nseg + q);
                     }
                 }
             }
@@ -837,15 +857,39 @@ __global__ __launch_bounds__(kBlock) void k_gather_rod(Arrays A, StateBuf src, S
         const uint32_t r = e / K, c = e - r * K, vq = c / D;
         R.rec[w.first * K + e] = newpos[(r * nv + vq) * 3 + (c - vq * D)];
     }
-    // bounding ball of the rod (first lane): vertex mean and the largest distance to it
-    if (active && v == 0) {
-        double c[3] = {0.0, 0.0, 0.0};
+    // the rod's vertex mean (every lane of the rod computes the same)
+    auto mean = [&](double* c) {
+        const uint32_t l0 = rr * nv;
+        c[0] = c[1] = c[2] = 0.0;
         for (uint32_t q = 0; q < nv; q++) {
 #pragma unroll
-            for (int k = 0; k < D; k++) c[k] = c[k] + newpos[(lt + q) * 3 + k];
+            for (int k = 0; k < D; k++) c[k] = c[k] + newpos[(l0 + q) * 3 + k];
         }
 #pragma unroll
         for (int k = 0; k < D; k++) c[k] = c[k] / (double)nv;
+    };
+    // ball of segment (v, v + 1): midpoint relative to the mean, half length with
+    // margin (the pass-1 reach tests of the force kernel)
+    if (active && v + 1 < nv) {
+        double c[3];
+        mean(c);
+        float o[3] = {0.0f, 0.0f, 0.0f}, h2 = 0.0f;
+        bool zero = true;
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+            const double a = newpos[lt * 3 + k], b = newpos[(lt + 1) * 3 + k];
+            o[k] = (float)((a + b) * 0.5 - c[k]);
+            const float d = (float)(b - a);
+            h2 = fmaf(d, d, h2);
+            zero = zero && a == b;
+        }
+        const float h = zero ? __builtin_inff() : 0.5f * sqrtf(h2) * (1.0f + 1e-6f) + 1e-6f;
+        R.seg[s * (nv - 1) + v] = make_float4(o[0], o[1], o[2], h);
+    }
+    // bounding ball of the rod (first lane): vertex mean and the largest distance to it
+    if (active && v == 0) {
+        double c[3];
+        mean(c);
         double r2 = 0.0;
         for (uint32_t q = 0; q < nv; q++) {
             double d2 = 0.0;
