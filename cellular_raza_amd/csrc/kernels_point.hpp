@@ -767,6 +767,7 @@ __global__ __launch_bounds__(kBlock, MIXED ? 5 : 4) void k_force_tiled(Arrays A,
     using Rec = typename RecOf<MIXED>::type;
     constexpr int CAP = MIXED ? kLdsRecordsMixed : kLdsRecordsFP64;
     constexpr uint32_t NP = D == 3 ? 3u : 1u;         // staged planes
+    const ForceTile T = tiles[xcd_block(blockIdx.x, gridDim.x)];  // loaded beside the control block
     if (halted(A.ctrl, A.ctrl->step)) return;
     constexpr uint32_t PA_OWN = D == 3 ? 1u : 0u;     // the tile's own plane among them
     __shared__ Rec lds[CAP];
@@ -778,7 +779,6 @@ __global__ __launch_bounds__(kBlock, MIXED ? 5 : 4) void k_force_tiled(Arrays A,
     __shared__ OwnEntry own_tab[kBlock];             // own agent lookups (sorted case)
     __shared__ uint16_t lsub[CAP];                   // chili subdomain of each staged record (sub_on only)
     const Rec* __restrict__ rec = rec_array(A, std::integral_constant<bool, MIXED>());
-    const ForceTile T = tiles[xcd_block(blockIdx.x, gridDim.x)];
     const InstParams& P = A.ip[T.inst];
     const uint32_t n0 = P.n[0], n1 = P.n[1], n2 = P.n[2], voff = P.voxel_offset;
     Region G;
@@ -808,13 +808,15 @@ __global__ __launch_bounds__(kBlock, MIXED ? 5 : 4) void k_force_tiled(Arrays A,
     // order, the others an atomic arrival rank in their next voxel.  The force
     // computation reads none of key, rank, stay or count, so each wave does this
     // after its agents' forces, beside the waves still computing theirs.
-    auto rank_tile = [&]() {
+    // staged: the own voxels' slot ranges are in LDS (own_gs, own_pre) already.
+    auto rank_tile = [&](bool staged) {
         if (!rank) return;
         const float inv_w = 1.0f / (float)W;
         for (uint32_t u = threadIdx.x; u < n_ownv; u += kBlock) {
             const uint32_t r = small_div(u, inv_w), c = u - r * W;
             const uint32_t g = voff + (T.a0 * n1 + T.b0 + r) * n2 + T.c0 + c;
-            const uint32_t b = A.start[g], e = A.start[g + 1];
+            const uint32_t b = staged ? own_gs[u] : A.start[g];
+            const uint32_t e = staged ? b + ((uint32_t)own_pre[u + 1] - (uint32_t)own_pre[u]) : A.start[g + 1];
             uint32_t n = 0;
             for (uint32_t s = b; s < e; s++) {
                 const uint32_t k = A.key[s];
@@ -868,6 +870,15 @@ __global__ __launch_bounds__(kBlock, MIXED ? 5 : 4) void k_force_tiled(Arrays A,
                 }
             }
         }
+        // the first kPreRecs records of each voxel are loaded before the scan, so
+        // that their latency overlaps its barriers
+        constexpr uint32_t kPreRecs = 2;
+        Rec pre[3][kPreRecs];
+#pragma unroll
+        for (int q = 0; q < 3; q++)
+#pragma unroll
+            for (uint32_t k = 0; k < kPreRecs; k++)
+                if (k < cnt[q]) pre[q][k] = rec[gs[q] + k];
         uint32_t total;
         uint32_t run = block_exclusive_scan(sum, total);
         const bool use_lds = total <= (uint32_t)CAP;  // uniform
@@ -879,7 +890,7 @@ __global__ __launch_bounds__(kBlock, MIXED ? 5 : 4) void k_force_tiled(Arrays A,
                     vst[v] = (uint16_t)run;
                     const uint32_t pa = (uint32_t)pa_q[q], rb = (uint32_t)rb_q[q], c = c_q[q];
                     for (uint32_t k = 0; k < cnt[q]; k++) {
-                        const Rec r = rec[gs[q] + k];
+                        const Rec r = k < kPreRecs ? pre[q][k] : rec[gs[q] + k];
                         if constexpr (MIXED) lds[run + k] = to_region<D>(r, (int)pa, (int)rb, G);
                         else lds[run + k] = r;
                     }
@@ -993,7 +1004,7 @@ __global__ __launch_bounds__(kBlock, MIXED ? 5 : 4) void k_force_tiled(Arrays A,
                                                       physical, symmetric, src, unit, MULTI ? threadIdx.x & gmask : 0u,
                                                       lg);
                 }
-                rank_tile();
+                rank_tile(true);
                 return;
             }
             for (uint32_t t = threadIdx.x; t < (n_own << lg); t += kBlock) {
@@ -1004,7 +1015,7 @@ __global__ __launch_bounds__(kBlock, MIXED ? 5 : 4) void k_force_tiled(Arrays A,
                 agent_force<D, POT, MIXED, SENSE>(A, S, P, s, self, lds[self], ov, w, physical, symmetric, src, unit,
                                                   MULTI ? t & gmask : 0u, lg);
             }
-            rank_tile();
+            rank_tile(true);
             return;
         }
     }
@@ -1037,7 +1048,7 @@ __global__ __launch_bounds__(kBlock, MIXED ? 5 : 4) void k_force_tiled(Arrays A,
             agent_force<D, POT, MIXED, SENSE>(A, S, P, s, s, src.record(s, wm), ov, w, physical, symmetric, src, unit);
         }
     }
-    rank_tile();
+    rank_tile(false);
 }
 
 // ---------------------------------------------------------------------------
