@@ -537,7 +537,8 @@ def main():
         bf = force_bytes_per_agent(cbar, rec_bytes, D, 3 ** D, 8 * nv * D)
     else:
         rec_bytes = (4 * D + 4) if has_mixed else 8 * D + 8
-        bf = force_bytes_per_agent(cbar, rec_bytes, D, 3 ** D, 8 * D)
+        # mixed point forces are stored in float, FP64 ones in double
+        bf = force_bytes_per_agent(cbar, rec_bytes, D, 3 ** D, (4 if has_mixed else 8) * D)
     t_force_ms = tim_m["force"]["total_ms"] / max(tim_m["force"]["launches"], 1)
     achieved = bf * N / (t_force_ms * 1e-3) / 1e9
     flops_agent = None

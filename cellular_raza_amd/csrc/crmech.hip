@@ -361,6 +361,8 @@ void free_device(crm_ctx* ctx) {
 // the stayers per voxel.
 uint64_t stay_offset(const crm_ctx* ctx) { return (ctx->Vt + 63) & ~(uint64_t)63; }
 
+bool force_f32(const crm_ctx* ctx);
+
 Arrays arrays(crm_ctx* ctx) {
     Arrays A;
     A.ip = ctx->d_ip;
@@ -378,6 +380,7 @@ Arrays arrays(crm_ctx* ctx) {
     A.key_next = ctx->d_key + (size_t)(1 - ctx->cur) * ctx->N;
     A.perm = ctx->d_perm;
     A.force = ctx->d_force;
+    A.forcef = force_f32(ctx) ? reinterpret_cast<float*>(ctx->d_force) : nullptr;
     A.rec64 = ctx->d_rec64;
     A.recq = ctx->d_recq;
     A.sense = ctx->sense_on ? ctx->d_sense : nullptr;
@@ -388,6 +391,10 @@ Arrays arrays(crm_ctx* ctx) {
 }
 
 bool is_rod(const crm_ctx* ctx) { return ctx->mech_kind == CRM_MECHANICS_ROD; }
+
+// Point agents in mixed precision keep their forces in float (the pair sums are
+// float already); the buffer keeps its double size.
+bool force_f32(const crm_ctx* ctx) { return !is_rod(ctx) && ctx->cfg.precision == CRM_PRECISION_MIXED; }
 
 RodArgs rod_args(crm_ctx* ctx) {
     RodArgs R;
@@ -2034,7 +2041,13 @@ crm_status crm_compute_forces(crm_ctx* ctx, double* force) {
     if (st) return st;
     std::vector<uint32_t> id;
     CRM_TRY(download_ids(ctx, id));
-    return download_by_id(ctx, ctx->d_force, ctx->K, force, id);
+    if (!force_f32(ctx)) return download_by_id(ctx, ctx->d_force, ctx->K, force, id);
+    std::vector<float> tmp((size_t)ctx->K * ctx->N);
+    CRM_HIP(hipMemcpyAsync(tmp.data(), ctx->d_force, tmp.size() * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    CRM_HIP(hipStreamSynchronize(ctx->stream));
+    for (uint64_t s = 0; s < ctx->N; s++)
+        for (uint32_t k = 0; k < ctx->K; k++) force[(uint64_t)id[s] * ctx->K + k] = (double)tmp[(uint64_t)k * ctx->N + s];
+    return CRM_OK;
 }
 
 crm_status crm_get_state(crm_ctx* ctx, double* pos, double* vel) {

@@ -58,6 +58,7 @@ struct Arrays {
     uint32_t* key_next;   // [N] next voxel of the following step (binned ahead by the gather)
     uint32_t* perm;   // [N] sorted slot -> previous slot
     double* force;    // [K][N]
+    float* forcef;    // [K][N] in the same memory: point forces of the mixed path, stored in float
     double4* rec64;   // [N] (x, y, z, species) absolute, FP64 path
     int4* recq;       // [N] (qx, qy, qz, species | outside << 31) fixed point in the voxel, mixed path
     uint32_t* sense;  // [N] (rods: [N][nv]) neighbour counts by slot; null when sensing is off
@@ -689,7 +690,8 @@ __device__ __forceinline__ void agent_force(const Arrays& A, const StateBuf& S, 
     for (int k = 0; k < D; k++) {
         double f = fsum[k];
         if (spring) f = f + (-P.dk[k]) * (S.pos[(uint64_t)k * N + s] - P.dc[k]);
-        A.force[(uint64_t)k * N + s] = f;
+        if constexpr (MIXED) A.forcef[(uint64_t)k * N + s] = (float)f;  // the pair sums are float already
+        else A.force[(uint64_t)k * N + s] = f;
     }
     if constexpr (SENSE) A.sense[s] = n_sense;
 }
@@ -1260,7 +1262,7 @@ __global__ __launch_bounds__(kBlock) void k_key_point(Arrays A, StateBuf S, Step
     for (int k = 0; k < D; k++) {
         x[k] = S.pos[k * N + s];
         v[k] = MECH == CRM_MECHANICS_BROWNIAN ? 0.0 : S.vel[k * N + s];
-        f[k] = MECH == CRM_MECHANICS_BROWNIAN ? A.force[k * N + s] : 0.0;
+        f[k] = MECH != CRM_MECHANICS_BROWNIAN ? 0.0 : A.forcef ? (double)A.forcef[k * N + s] : A.force[k * N + s];
     }
     auto hist = [&](int i, int k, bool) { return S.hdx[((uint64_t)rg.it[i] * D + k) * N + s]; };
     const bool out = point_step<D, MECH, false>(P, sa, rg, t, id, x, v, f, hist, nx, nv, dx, dv);
@@ -1620,7 +1622,8 @@ __global__ __launch_bounds__(kBlock) void k_gather_integrate_point(Arrays A, Sta
     for (int k = 0; k < D; k++) {
         x[k] = __builtin_nontemporal_load(&src.pos[k * N + o]);
         v[k] = __builtin_nontemporal_load(&src.vel[k * N + o]);
-        f[k] = __builtin_nontemporal_load(&A.force[k * N + o]);
+        if constexpr (MIXED) f[k] = (double)__builtin_nontemporal_load(&A.forcef[k * N + o]);
+        else f[k] = __builtin_nontemporal_load(&A.force[k * N + o]);
     }
     // the ring before this step (min(t - birth, cap) entries: with cap <= 2 the
     // slot not overwritten now, once filled), in registers, except the slot this
