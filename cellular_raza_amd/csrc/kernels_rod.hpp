@@ -57,17 +57,122 @@ __device__ __forceinline__ bool rod_lane(const BlockWork& w, uint32_t nv, uint64
 // order; the ext parts follow the own parts of all candidates.
 // ---------------------------------------------------------------------------
 // Mixed: 5 waves per SIMD (at most 102 VGPRs) hide more of the candidates' read
-// latency than the 4 its 112 VGPRs would give (C3: 447 -> 417 us).
+// latency than the 4 its 112 VGPRs would give.
+//
+// Staging: the block's rods lie in at most two rows of voxels (own rows; the slots
+// are in voxel order), and every candidate of a rod lies in one of the 3^(D-1)
+// stencil rows of its own row, within the columns of the own row's rods +- 1.  Those
+// rows' rods (records, bounding and segment balls, species) are copied to LDS, up
+// to kRodLdsBytes, so that the candidate reads of the walks below come from LDS; a
+// block with more own rows or more rods reads global memory (same order, same
+// arithmetic).
+constexpr uint32_t kRodLdsBytes = 36 * 1024;
 template <int D, int POT, bool MIXED>
 __global__ __launch_bounds__(kBlock, MIXED ? 5 : 4) void k_force_rod(Arrays A, StateBuf S, RodArgs R, uint32_t physical) {
     if (halted(A.ctrl, A.ctrl->step)) return;
     const BlockWork w = A.work[blockIdx.x];
     uint64_t s;
     uint32_t v;
-    if (!rod_lane(w, R.nv, s, v)) return;
+    const bool active = rod_lane(w, R.nv, s, v);
     const InstParams& P = A.ip[w.inst];
-    const PotParams<double>& PP = P.pd;
     const uint32_t nv = R.nv, K = nv * D, nseg = nv - 1;
+    __shared__ __align__(16) char lbuf[kRodLdsBytes];
+    __shared__ uint32_t rg_slot[18], rg_cnt[18], rg_off[19];
+    __shared__ uint32_t s_row[2], s_cmin[2], s_cmax[2], s_bad;
+    const uint32_t n1 = P.n[1], n2 = P.n[2], voff = P.voxel_offset;
+    const uint32_t per_rod = 8u * K + 32u + 16u * nseg + 4u;
+    const uint32_t max_rods = (kRodLdsBytes - 64u) / per_rod;  // room for the sections' 16-byte alignment
+    // LDS layout, each section 16-byte aligned: bnd [max_rods] double4, rec
+    // [max_rods][K] doubles, seg [max_rods][nseg] float4, species [max_rods] int
+    auto up16 = [](uint32_t b) { return (b + 15u) & ~15u; };
+    const uint32_t o_rec = up16(32u * max_rods), o_seg = up16(o_rec + 8u * K * max_rods);
+    const uint32_t o_sp = up16(o_seg + 16u * nseg * max_rods);
+    double4* lbnd = reinterpret_cast<double4*>(lbuf);
+    double* lrec = reinterpret_cast<double*>(lbuf + o_rec);
+    float4* lseg = reinterpret_cast<float4*>(lbuf + o_seg);
+    int* lsp = reinterpret_cast<int*>(lbuf + o_sp);
+    if (threadIdx.x == 0) {
+        s_row[0] = (S.vox[w.first] - voff) / n2;
+        s_row[1] = (S.vox[w.first + w.count - 1] - voff) / n2;
+        s_cmin[0] = s_cmin[1] = 0xffffffffu;
+        s_cmax[0] = s_cmax[1] = 0u;
+        s_bad = 0u;
+    }
+    __syncthreads();
+    if (threadIdx.x < w.count) {  // own rows and their column ranges
+        const uint32_t lv = S.vox[w.first + threadIdx.x] - voff, row = lv / n2, c = lv - row * n2;
+        const uint32_t q = row == s_row[0] ? 0u : (row == s_row[1] ? 1u : 2u);
+        if (q == 2u) s_bad = 1u;
+        else {
+            atomicMin(&s_cmin[q], c);
+            atomicMax(&s_cmax[q], c);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 18) {  // range (own row q, stencil row (da, db))
+        const uint32_t q = threadIdx.x / 9u, t9 = threadIdx.x - q * 9u;
+        const int da = (int)(t9 / 3u) - 1, db = (int)(t9 % 3u) - 1;
+        uint32_t slot = 0, cnt = 0;
+        if (q == 0 || s_row[1] != s_row[0]) {
+            const uint32_t row = s_row[q], a = row / n1, b = row - a * n1;
+            const int aa = (int)a + da, bb = (int)b + db;
+            if (aa >= 0 && aa < (int)P.n[0] && bb >= 0 && bb < (int)n1) {
+                const uint32_t clo = s_cmin[q] == 0 ? 0u : s_cmin[q] - 1u, chi = min(s_cmax[q] + 2u, n2);
+                const uint32_t g = voff + ((uint32_t)aa * n1 + (uint32_t)bb) * n2;
+                slot = A.start[g + clo];
+                cnt = A.start[g + chi] - slot;
+            }
+        }
+        rg_slot[threadIdx.x] = slot;
+        rg_cnt[threadIdx.x] = cnt;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t o = 0;
+        for (int r = 0; r < 18; r++) {
+            rg_off[r] = o;
+            o += rg_cnt[r];
+        }
+        rg_off[18] = o;
+        if (o > max_rods) s_bad = 1u;
+    }
+    __syncthreads();
+    const bool staged = s_bad == 0u;  // uniform
+    if (staged) {
+        const uint32_t total = rg_off[18];
+        for (uint32_t e = threadIdx.x; e < total; e += kBlock) {  // thread per staged rod
+            uint32_t r = 0;
+            while (rg_off[r + 1] <= e) r++;
+            const uint32_t j = rg_slot[r] + (e - rg_off[r]);
+            lbnd[e] = R.bnd[j];
+            lsp[e] = S.species[j];
+            for (uint32_t k = 0; k < K; k++) lrec[e * K + k] = R.rec[(uint64_t)j * K + k];
+            for (uint32_t q = 0; q < nseg; q++) lseg[e * nseg + q] = R.seg[(uint64_t)j * nseg + q];
+        }
+    }
+    __syncthreads();
+    if (!active) return;
+    // this lane's own row among the block's (0 or 1)
+    const uint32_t orow = ((S.vox[s] - voff) / n2) == s_row[0] ? 0u : 1u;
+    auto work = [&](auto stg) {
+        constexpr bool STG = decltype(stg)::value;
+        auto bnd_of = [&](uint32_t j, int32_t base) -> double4 {
+            if constexpr (STG) return lbnd[(int32_t)j + base];
+            else return R.bnd[j];
+        };
+        auto seg_of = [&](uint32_t j, int32_t base) -> const float4* {
+            if constexpr (STG) return lseg + (uint32_t)((int32_t)j + base) * nseg;
+            else return R.seg + (uint64_t)j * nseg;
+        };
+        auto rec_of = [&](uint32_t j, int32_t base) -> const double* {
+            if constexpr (STG) return lrec + (uint32_t)((int32_t)j + base) * K;
+            else return R.rec + (uint64_t)j * K;
+        };
+        auto sp_of = [&](uint32_t j, int32_t base) -> int {
+            if constexpr (STG) return lsp[(int32_t)j + base];
+            else return S.species[j];
+        };
+    const PotParams<double>& PP = P.pd;
     const double* own = R.rec + s * K;
     double pv[D];
 #pragma unroll
@@ -180,6 +285,13 @@ __global__ __launch_bounds__(kBlock, MIXED ? 5 : 4) void k_force_rod(Arrays A, S
 
     struct RodWalk {
         uint32_t ra, rb, j, je;
+        int32_t base;  // staged: LDS index of slot j = j + base in this stencil row
+    };
+    // the staged range of stencil row (ra, rb) of this lane's own row
+    auto row_base = [&](uint32_t ra, uint32_t rb) -> int32_t {
+        if constexpr (!STG) return 0;
+        const uint32_t r = orow * 9u + (ra + 1u - idx[0]) * 3u + (rb + 1u - idx[1]);
+        return (int32_t)rg_off[r] - (int32_t)rg_slot[r];
     };
     // candidates: stencil rows (ra, rb) ascending, each the rods of columns [lo2, hi2)
     auto settle = [&](RodWalk& wk) {
@@ -192,6 +304,7 @@ __global__ __launch_bounds__(kBlock, MIXED ? 5 : 4) void k_force_rod(Arrays A, S
             const uint32_t row = P.voxel_offset + (wk.ra * P.n[1] + wk.rb) * P.n[2];
             wk.j = A.start[row + lo[2]] + over;
             wk.je = A.start[row + hi[2]];
+            wk.base = row_base(wk.ra, wk.rb);
         }
         return true;
     };
@@ -201,10 +314,11 @@ __global__ __launch_bounds__(kBlock, MIXED ? 5 : 4) void k_force_rod(Arrays A, S
         const uint32_t row = P.voxel_offset + (wk.ra * P.n[1] + wk.rb) * P.n[2];
         wk.j = A.start[row + lo[2]];
         wk.je = A.start[row + hi[2]];
+        wk.base = row_base(wk.ra, wk.rb);
         return settle(wk);
     };
-    auto strengths = [&](uint32_t j, bool& same, double& s_own, double& s_ext) {
-        const int sp_j = S.species[j];
+    auto strengths = [&](uint32_t j, int32_t base, bool& same, double& s_own, double& s_ext) {
+        const int sp_j = sp_of(j, base);
         same = sp_i == sp_j;
         s_own = s_ext = PP.strength;
         if constexpr (POT == CRM_POTENTIAL_MORSE_SPECIES) {
@@ -229,7 +343,7 @@ __global__ __launch_bounds__(kBlock, MIXED ? 5 : 4) void k_force_rod(Arrays A, S
                 // v relative to j's vertex mean in float, then to each segment ball
                 // (their offsets from the mean are small, so the float differences
                 // stay far inside the reach margin)
-                const double4 bj = R.bnd[j];
+                const double4 bj = bnd_of(j, cur.base);
                 const double cj[3] = {bj.x, bj.y, bj.z};
                 float dc[D], d2 = 0.0f;
 #pragma unroll
@@ -239,7 +353,7 @@ __global__ __launch_bounds__(kBlock, MIXED ? 5 : 4) void k_force_rod(Arrays A, S
                 }
                 const float lim = (float)bj.w * (1.0f + 1e-6f) + reach;
                 if (!(d2 > lim * lim)) {
-                    const float4* sg = R.seg + (uint64_t)j * nseg;
+                    const float4* sg = seg_of(j, cur.base);
                     for (uint32_t q = 0; q < nseg; q++) {
                         const float4 g = sg[q];
                         const float go[3] = {g.x, g.y, g.z};
@@ -284,8 +398,8 @@ __global__ __launch_bounds__(kBlock, MIXED ? 5 : 4) void k_force_rod(Arrays A, S
                 wk.j += c - pos;
                 pos = c;
                 settle(wk);
-                ext = R.rec + (uint64_t)wk.j * K;
-                strengths(wk.j, same, s_own, s_ext);
+                ext = rec_of(wk.j, wk.base);
+                strengths(wk.j, wk.base, same, s_own, s_ext);
             }
             double t, f[D];
             calc_err |= pair(pv, ext + q * D, ext + (q + 1) * D, same, s_own, f, t);
@@ -310,6 +424,7 @@ __global__ __launch_bounds__(kBlock, MIXED ? 5 : 4) void k_force_rod(Arrays A, S
         ew.ra = ew.rb = 0;
         ew.j = cb;
         ew.je = ce;
+        ew.base = row_base(idx[0], idx[1]);
         live = POT != CRM_POTENTIAL_NONE && ew.j < ew.je;
     }
     auto esettle = [&](RodWalk& wk) { return physical ? settle(wk) : wk.j < wk.je; };
@@ -319,10 +434,10 @@ __global__ __launch_bounds__(kBlock, MIXED ? 5 : 4) void k_force_rod(Arrays A, S
         for (uint32_t c = 0; c < per_e; c++) {
             const uint32_t j = ew.j;
             if (j != s) {
-                const double4 bj = R.bnd[j];
+                const double4 bj = bnd_of(j, ew.base);
                 const double cj[3] = {bj.x, bj.y, bj.z};
                 if (!out_of_reach(ci, cj, ri + (float)bj.w * (1.0f + 1e-6f))) {
-                    const double* ext = R.rec + (uint64_t)j * K;
+                    const double* ext = rec_of(j, ew.base);
                     for (uint32_t u = 0; u < nv; u++) {
                         const double* pu = ext + u * D;
                         if (h0 >= 0.0f && !out_of_reach(pu, m0, h0)) me |= 1ull << (c * 2 * nv + 2 * u);
@@ -358,8 +473,8 @@ __global__ __launch_bounds__(kBlock, MIXED ? 5 : 4) void k_force_rod(Arrays A, S
                 wk.j += c - pos;
                 pos = c;
                 if (physical) settle(wk);
-                ext = R.rec + (uint64_t)wk.j * K;
-                strengths(wk.j, same, s_own, s_ext);
+                ext = rec_of(wk.j, wk.base);
+                strengths(wk.j, wk.base, same, s_own, s_ext);
             }
             // side 0: segment (v - 1, v), v is its second end (weight t)
             // side 1: segment (v, v + 1), v is its first end (weight 1 - t)
@@ -417,7 +532,7 @@ __global__ __launch_bounds__(kBlock, MIXED ? 5 : 4) void k_force_rod(Arrays A, S
         while (more) {
             const uint32_t j = sw.j;
             if (j != s) {
-                const double4 bj = R.bnd[j];
+                const double4 bj = bnd_of(j, sw.base);
                 const float lim = (float)bj.w * (1.0f + 1e-6f) + sreach;
                 float d2 = 0.0f;
                 const double cj[3] = {bj.x, bj.y, bj.z};
@@ -427,7 +542,7 @@ __global__ __launch_bounds__(kBlock, MIXED ? 5 : 4) void k_force_rod(Arrays A, S
                     d2 = fmaf(d, d, d2);
                 }
                 if (!(d2 > lim * lim)) {
-                    const double* ext = R.rec + (uint64_t)j * K;
+                    const double* ext = rec_of(j, sw.base);
                     for (uint32_t u = 0; u < nv; u++) {
                         double z[D];
 #pragma unroll
@@ -441,6 +556,9 @@ __global__ __launch_bounds__(kBlock, MIXED ? 5 : 4) void k_force_rod(Arrays A, S
         }
         A.sense[s * nv + v] = n;
     }
+    };
+    if (staged) work(std::true_type{});
+    else work(std::false_type{});
 }
 
 // Normal k of a rod's K normals for (agent, step): Philox block k / 2, element k % 2.
