@@ -56,19 +56,18 @@ __device__ __forceinline__ bool rod_lane(const BlockWork& w, uint32_t nv, uint64
 // evaluates only those.  Own parts are summed per candidate in the reference's
 // order; the ext parts follow the own parts of all candidates.
 // ---------------------------------------------------------------------------
-// Mixed: 5 waves per SIMD (at most 102 VGPRs) hide more of the candidates' read
-// latency than the 4 its 112 VGPRs would give.
-//
 // Staging: the block's rods lie in at most two rows of voxels (own rows; the slots
 // are in voxel order), and every candidate of a rod lies in one of the 3^(D-1)
 // stencil rows of its own row, within the columns of the own row's rods +- 1.  Those
 // rows' rods (records, bounding and segment balls, species) are copied to LDS, up
 // to kRodLdsBytes, so that the candidate reads of the walks below come from LDS; a
 // block with more own rows or more rods reads global memory (same order, same
-// arithmetic).
+// arithmetic).  36 KiB hold the 2-D 8-vertex rods' stencil rows of nearly every
+// block (C3: 32 KiB left too many blocks on global memory, 418 -> 409 us; 36 KiB
+// 348 us; 40 KiB 347 us) and leave 4 workgroups per CU.
 constexpr uint32_t kRodLdsBytes = 36 * 1024;
 template <int D, int POT, bool MIXED>
-__global__ __launch_bounds__(kBlock, MIXED ? 5 : 4) void k_force_rod(Arrays A, StateBuf S, RodArgs R, uint32_t physical) {
+__global__ __launch_bounds__(kBlock, 4) void k_force_rod(Arrays A, StateBuf S, RodArgs R, uint32_t physical) {
     if (halted(A.ctrl, A.ctrl->step)) return;
     const BlockWork w = A.work[blockIdx.x];
     uint64_t s;
