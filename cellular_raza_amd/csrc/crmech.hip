@@ -484,24 +484,28 @@ crm_status launch_force_rod_dm(crm_ctx* ctx, StateBuf S) {
     Arrays A = arrays(ctx);
     const RodArgs R = rod_args(ctx);
     const uint32_t phys = ctx->cfg.weighting == CRM_WEIGHT_PHYSICAL;
+    const uint32_t no_lds = (ctx->cfg.flags & CRM_FLAG_NO_LDS) ? 1u : 0u;
     const dim3 g(ctx->n_work), b(kBlock);
     switch (ctx->pot_kind) {
         case CRM_POTENTIAL_MORSE:
-            hipLaunchKernelGGL((k_force_rod<D, CRM_POTENTIAL_MORSE, MIXED>), g, b, 0, ctx->stream, A, S, R, phys);
+            hipLaunchKernelGGL((k_force_rod<D, CRM_POTENTIAL_MORSE, MIXED>), g, b, 0, ctx->stream, A, S, R, phys,
+                               no_lds);
             return CRM_OK;
         case CRM_POTENTIAL_MORSE_SPECIES:
             hipLaunchKernelGGL((k_force_rod<D, CRM_POTENTIAL_MORSE_SPECIES, MIXED>), g, b, 0, ctx->stream, A, S, R,
-                               phys);
+                               phys, no_lds);
             return CRM_OK;
         case CRM_POTENTIAL_MIE:
-            hipLaunchKernelGGL((k_force_rod<D, CRM_POTENTIAL_MIE, MIXED>), g, b, 0, ctx->stream, A, S, R, phys);
+            hipLaunchKernelGGL((k_force_rod<D, CRM_POTENTIAL_MIE, MIXED>), g, b, 0, ctx->stream, A, S, R, phys,
+                               no_lds);
             return CRM_OK;
         case CRM_POTENTIAL_BOUND_LENNARD_JONES:
             hipLaunchKernelGGL((k_force_rod<D, CRM_POTENTIAL_BOUND_LENNARD_JONES, MIXED>), g, b, 0, ctx->stream, A, S,
-                               R, phys);
+                               R, phys, no_lds);
             return CRM_OK;
         case CRM_POTENTIAL_NONE:
-            hipLaunchKernelGGL((k_force_rod<D, CRM_POTENTIAL_NONE, MIXED>), g, b, 0, ctx->stream, A, S, R, phys);
+            hipLaunchKernelGGL((k_force_rod<D, CRM_POTENTIAL_NONE, MIXED>), g, b, 0, ctx->stream, A, S, R, phys,
+                               no_lds);
             return CRM_OK;
     }
     return CRM_INVALID_ARGUMENT;

@@ -67,7 +67,8 @@ __device__ __forceinline__ bool rod_lane(const BlockWork& w, uint32_t nv, uint64
 // 348 us; 40 KiB 347 us) and leave 4 workgroups per CU.
 constexpr uint32_t kRodLdsBytes = 36 * 1024;
 template <int D, int POT, bool MIXED>
-__global__ __launch_bounds__(kBlock, 4) void k_force_rod(Arrays A, StateBuf S, RodArgs R, uint32_t physical) {
+__global__ __launch_bounds__(kBlock, 4) void k_force_rod(Arrays A, StateBuf S, RodArgs R, uint32_t physical,
+                                                          uint32_t no_lds) {
     if (halted(A.ctrl, A.ctrl->step)) return;
     const BlockWork w = A.work[blockIdx.x];
     uint64_t s;
@@ -95,7 +96,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_force_rod(Arrays A, StateBuf S, R
         s_row[1] = (S.vox[w.first + w.count - 1] - voff) / n2;
         s_cmin[0] = s_cmin[1] = 0xffffffffu;
         s_cmax[0] = s_cmax[1] = 0u;
-        s_bad = 0u;
+        s_bad = no_lds;
     }
     __syncthreads();
     if (threadIdx.x < w.count) {  // own rows and their column ranges
