@@ -247,7 +247,8 @@ crm_status crm_step(crm_ctx* ctx, uint32_t n_steps, const double* noise);
 
 /* Interaction force accumulated for the current state, by agent index
  * (AuxStorageMechanics::current_force before integration, aux_storage.rs:41-130).
- * Does not advance the simulation. */
+ * Does not advance the simulation.  Point agents in mixed precision hold their
+ * forces in float (the values returned, and the ones the integrator uses). */
 crm_status crm_compute_forces(crm_ctx* ctx, double* force);
 
 crm_status crm_get_state(crm_ctx* ctx, double* pos, double* vel);
