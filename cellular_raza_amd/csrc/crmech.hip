@@ -274,8 +274,13 @@ struct crm_ctx {
     // kLongGraphSteps steps
     hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
 
-    // timing
-    hipEvent_t ev_a = nullptr, ev_b = nullptr;
+    // timing: an event pair around each launch (class t_cls[i]), recorded without
+    // waiting, so that the timed kernels run back to back as in a normal step;
+    // the pairs are read when the pool is full and when the timings are read
+    static constexpr int kTimingPairs = 64;
+    hipEvent_t ev_a[kTimingPairs] = {}, ev_b[kTimingPairs] = {};
+    int t_cls[kTimingPairs] = {};
+    int t_used = 0;
     double t_ms[CRM_KERNEL_COUNT] = {};
     uint64_t t_n[CRM_KERNEL_COUNT] = {};
 };
@@ -408,20 +413,37 @@ RodArgs rod_args(crm_ctx* ctx) {
 
 bool timing_on(crm_ctx* ctx) { return (ctx->cfg.flags & CRM_FLAG_TIMING) != 0; }
 
+// Waits for the recorded event pairs and adds their times to their classes.
+crm_status t_flush(crm_ctx* ctx) {
+    if (ctx->t_used == 0) return CRM_OK;
+    const int n = ctx->t_used;
+    ctx->t_used = 0;
+    CRM_HIP(hipEventSynchronize(ctx->ev_b[n - 1]));
+    for (int i = 0; i < n; i++) {
+        float ms = 0.f;
+        CRM_HIP(hipEventElapsedTime(&ms, ctx->ev_a[i], ctx->ev_b[i]));
+        ctx->t_ms[ctx->t_cls[i]] += ms;
+        ctx->t_n[ctx->t_cls[i]] += 1;
+    }
+    return CRM_OK;
+}
+
 crm_status t_begin(crm_ctx* ctx) {
-    if (timing_on(ctx)) CRM_HIP(hipEventRecord(ctx->ev_a, ctx->stream));
+    if (timing_on(ctx)) {
+        if (ctx->t_used == crm_ctx::kTimingPairs) {
+            const crm_status st = t_flush(ctx);
+            if (st != CRM_OK) return st;
+        }
+        CRM_HIP(hipEventRecord(ctx->ev_a[ctx->t_used], ctx->stream));
+    }
     return CRM_OK;
 }
 
 crm_status t_end(crm_ctx* ctx, int cls) {
     CRM_HIP(hipGetLastError());
     if (timing_on(ctx)) {
-        CRM_HIP(hipEventRecord(ctx->ev_b, ctx->stream));
-        CRM_HIP(hipEventSynchronize(ctx->ev_b));
-        float ms = 0.f;
-        CRM_HIP(hipEventElapsedTime(&ms, ctx->ev_a, ctx->ev_b));
-        ctx->t_ms[cls] += ms;
-        ctx->t_n[cls] += 1;
+        CRM_HIP(hipEventRecord(ctx->ev_b[ctx->t_used], ctx->stream));
+        ctx->t_cls[ctx->t_used++] = cls;
     }
     return CRM_OK;
 }
@@ -1759,8 +1781,11 @@ crm_status crm_create(const crm_config* cfg, const crm_instance* instances, uint
     ctx->inst_host.assign(n_instances, 0);
     e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e != hipSuccess) return fail(CRM_DEVICE_ERROR, std::string("hipStreamCreate: ") + hipGetErrorString(e));
-    if (hipEventCreate(&ctx->ev_a) != hipSuccess || hipEventCreate(&ctx->ev_b) != hipSuccess)
-        return fail(CRM_DEVICE_ERROR, "hipEventCreate failed");
+    if (cfg->flags & CRM_FLAG_TIMING) {
+        for (int i = 0; i < crm_ctx::kTimingPairs; i++)
+            if (hipEventCreate(&ctx->ev_a[i]) != hipSuccess || hipEventCreate(&ctx->ev_b[i]) != hipSuccess)
+                return fail(CRM_DEVICE_ERROR, "hipEventCreate failed");
+    }
     if (hipMalloc((void**)&ctx->d_ip, sizeof(InstParams) * n_instances) != hipSuccess ||
         hipMalloc((void**)&ctx->d_ctrl, sizeof(Control)) != hipSuccess)
         return fail(CRM_DEVICE_ERROR, "hipMalloc failed");
@@ -2665,6 +2690,8 @@ crm_status crm_get_info(crm_ctx* ctx, crm_info* info) {
 
 crm_status crm_get_timing(crm_ctx* ctx, crm_timing* timing) {
     if (!ctx || !timing) return CRM_INVALID_ARGUMENT;
+    CRM_HIP(hipSetDevice(ctx->device));
+    CRM_TRY(t_flush(ctx));
     for (int i = 0; i < CRM_KERNEL_COUNT; i++) {
         timing->total_ms[i] = ctx->t_ms[i];
         timing->launches[i] = ctx->t_n[i];
@@ -2674,6 +2701,8 @@ crm_status crm_get_timing(crm_ctx* ctx, crm_timing* timing) {
 
 crm_status crm_reset_timing(crm_ctx* ctx) {
     if (!ctx) return CRM_INVALID_ARGUMENT;
+    CRM_HIP(hipSetDevice(ctx->device));
+    CRM_TRY(t_flush(ctx));  // pairs still in flight belong to the timings cleared here
     for (int i = 0; i < CRM_KERNEL_COUNT; i++) {
         ctx->t_ms[i] = 0.0;
         ctx->t_n[i] = 0;
@@ -2700,8 +2729,10 @@ void crm_destroy(crm_ctx* ctx) {
     free_device(ctx);
     if (ctx->d_ip) (void)hipFree(ctx->d_ip);
     if (ctx->d_ctrl) (void)hipFree(ctx->d_ctrl);
-    if (ctx->ev_a) (void)hipEventDestroy(ctx->ev_a);
-    if (ctx->ev_b) (void)hipEventDestroy(ctx->ev_b);
+    for (int i = 0; i < crm_ctx::kTimingPairs; i++) {
+        if (ctx->ev_a[i]) (void)hipEventDestroy(ctx->ev_a[i]);
+        if (ctx->ev_b[i]) (void)hipEventDestroy(ctx->ev_b[i]);
+    }
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }

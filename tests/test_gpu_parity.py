@@ -338,6 +338,32 @@ def test_graph_and_eager_bitwise_equal_and_deterministic():
         assert np.array_equal(r[0], runs[0][0]) and np.array_equal(r[1], runs[0][1])
 
 
+def test_kernel_timing_leaves_results_alone():
+    """CRM_FLAG_TIMING brackets every launch with HIP events (read back in batches of
+    64 pairs): one launch per class and step, positive times, the same state as an
+    untimed eager run, and crm_reset_timing clears the counts."""
+    wl = workloads.c2_sorting_3d(n=6000, seed=17, steps=30)
+    states, tim = [], None
+    for timing in (False, True):
+        g = MechanicsBackend(wl.instances, dt=wl.dt, mechanics_solver_order=2, noise="philox", seed=5,
+                             precision="mixed", graph=False, timing=timing)
+        g.set_agents(wl.pos, wl.vel, wl.species)
+        if timing:
+            g.reset_timing()
+            assert all(v["launches"] == 0 for v in g.timing().values())
+        g.step(30)  # 120 timed launches: the 64-pair pool is read back mid-run
+        if timing:
+            tim = g.timing()
+            g.reset_timing()
+            assert all(v["launches"] == 0 and v["total_ms"] == 0.0 for v in g.timing().values())
+        states.append(g.get_state())
+        g.close()
+    for cls in ("force", "scan", "scatter", "gather"):
+        assert tim[cls]["launches"] == 30, (cls, tim[cls])
+        assert tim[cls]["total_ms"] > 0.0, (cls, tim[cls])
+    assert np.array_equal(states[0][0], states[1][0]) and np.array_equal(states[0][1], states[1][1])
+
+
 def test_initial_placement_errors():
     dom = CartesianCuboid.from_boundaries_and_interaction_range([0.0, 0.0], [10.0, 10.0], 2.0)
     spec = [InstanceSpec(dom, MorsePotential(1.0, 1.0, 2.0, 1.0), NewtonDamped(1.0, 1.0))]
