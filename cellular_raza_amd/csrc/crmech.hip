@@ -663,17 +663,23 @@ bool keys_ahead(const crm_ctx* ctx);
 crm_status launch_sort(crm_ctx* ctx, bool advance, const StepArgs* sa) {
     Arrays A = arrays(ctx);
     const uint32_t Vt = (uint32_t)ctx->Vt;
-    CRM_TRY(t_begin(ctx));
-    hipLaunchKernelGGL(k_scan_local, dim3(ctx->n_tiles), dim3(kBlock), 0, ctx->stream, ctx->d_count,
-                       ctx->d_count + stay_offset(ctx), ctx->d_start_local, ctx->d_bsum, Vt, ctx->d_ctrl);
-    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->d_bsum, ctx->n_tiles, ctx->d_start,
-                       Vt, ctx->d_ctrl, (uint32_t)(advance ? 1 : 0));
-    CRM_TRY(t_end(ctx, CRM_KERNEL_SCAN));
-    CRM_TRY(t_begin(ctx));
-    const uint64_t n_scatter = std::max<uint64_t>(ctx->N, Vt);
-    hipLaunchKernelGGL(k_scatter, dim3((uint32_t)((n_scatter + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream,
-                       A);
-    CRM_TRY(t_end(ctx, CRM_KERNEL_SCATTER));
+    if (ctx->n_tiles == 1 && ctx->N <= kSmallSortAgents) {  // one workgroup does all three (timed as scan)
+        CRM_TRY(t_begin(ctx));
+        hipLaunchKernelGGL(k_sort_small, dim3(1), dim3(kBlock), 0, ctx->stream, A, (uint32_t)(advance ? 1 : 0));
+        CRM_TRY(t_end(ctx, CRM_KERNEL_SCAN));
+    } else {
+        CRM_TRY(t_begin(ctx));
+        hipLaunchKernelGGL(k_scan_local, dim3(ctx->n_tiles), dim3(kBlock), 0, ctx->stream, ctx->d_count,
+                           ctx->d_count + stay_offset(ctx), ctx->d_start_local, ctx->d_bsum, Vt, ctx->d_ctrl);
+        hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->d_bsum, ctx->n_tiles,
+                           ctx->d_start, Vt, ctx->d_ctrl, (uint32_t)(advance ? 1 : 0));
+        CRM_TRY(t_end(ctx, CRM_KERNEL_SCAN));
+        CRM_TRY(t_begin(ctx));
+        const uint64_t n_scatter = std::max<uint64_t>(ctx->N, Vt);
+        hipLaunchKernelGGL(k_scatter, dim3((uint32_t)((n_scatter + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                           ctx->stream, A);
+        CRM_TRY(t_end(ctx, CRM_KERNEL_SCATTER));
+    }
     // the point gather of a step orders each voxel's arrivals itself
     if (is_rod(ctx) || !advance) {
         CRM_TRY(t_begin(ctx));

@@ -12,7 +12,8 @@
 //   k_gather_integrate_point  arrivals put in order of previous slot,   } previous slot (sort_cells_in_voxels,
 //                      the full mechanics update written into the       }  update_mechanics.rs:418-513)
 //                      agent's new sorted slot, and the force records
-//   (set-up sorts and rods order the arrivals with k_fixup)
+//   (set-up sorts and rods order the arrivals with k_fixup; systems whose voxels
+//   fit one scan tile run k_sort_small, the scans and the scatter in one workgroup)
 //
 // Agents live in SoA arrays sorted by voxel; a voxel's agents are the slots
 // [start[v], start[v+1]).
@@ -1425,6 +1426,50 @@ __global__ __launch_bounds__(kBlock) void k_scatter(Arrays A) {
     const uint32_t k = A.key[g], r = A.rank[g];
     const uint32_t pos = (r & kMoved) ? A.stay[k] + (r & ~kMoved) : r;
     A.perm[A.start_local[k] + A.tile_off[k / kScanTile] + pos] = (uint32_t)g;
+}
+
+// Small systems (every voxel in one scan tile, at most kSmallSortAgents agents):
+// k_scan_local, k_scan_tiles and k_scatter in one workgroup, with barriers in place
+// of the launches between them; the voxel starts stay in LDS for the scatter.  The
+// same skip flag, count reset, starts, start[Vt], step advance and slot lists.
+constexpr uint32_t kSmallSortAgents = 16384;
+__global__ __launch_bounds__(kBlock) void k_sort_small(Arrays A, uint32_t advance) {
+    static_assert(kScanItems == 8, "eight voxels per thread");
+    __shared__ uint32_t s_start[kScanTile];
+    const bool skip = halted(A.ctrl, A.ctrl->step);
+    __syncthreads();  // every thread has read the control block before thread 0 writes it
+    if (threadIdx.x == 0) A.ctrl->skip = skip ? 1u : 0u;
+    if (skip) return;
+    const uint32_t Vt = A.Vt, base = threadIdx.x * kScanItems;
+    uint32_t vals[kScanItems], sum = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; i++) {
+        const uint32_t q = base + i;
+        vals[i] = q < Vt ? A.count[q] + A.stay[q] : 0u;
+        if (q < Vt) A.count[q] = 0u;
+        sum += vals[i];
+    }
+    uint32_t total;
+    uint32_t run = block_exclusive_scan(sum, total);
+#pragma unroll
+    for (int i = 0; i < kScanItems; i++) {
+        const uint32_t q = base + i;
+        if (q < Vt) {
+            s_start[q] = run;
+            A.start[q] = run;
+        }
+        run += vals[i];
+    }
+    if (threadIdx.x == 0) {
+        A.start[Vt] = total;
+        if (advance) A.ctrl->step += 1;
+    }
+    __syncthreads();
+    for (uint32_t g = threadIdx.x; g < (uint32_t)A.N; g += kBlock) {
+        const uint32_t k = A.key[g], r = A.rank[g];
+        const uint32_t pos = (r & kMoved) ? A.stay[k] + (r & ~kMoved) : r;
+        A.perm[s_start[k] + pos] = g;
+    }
 }
 
 // Orders each voxel's arrivals by previous slot (atomic arrival order is not
