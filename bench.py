@@ -38,7 +38,7 @@ FLOPS_PER_CANDIDATE = 8
 FLOPS_PER_PAIR = 30
 # committed rocprofv3 summaries of the workloads that have one
 PROFILES = {"c2": "profiles/c2_cell_sorting_1M.md", "c3": "profiles/c3_rods_2d_200k.md",
-            "c5": "profiles/c5_mie_3d_4M.md"}
+            "c4": "profiles/c4_ensemble_512.md", "c5": "profiles/c5_mie_3d_4M.md"}
 
 
 def candidates_per_agent(wl, start):
