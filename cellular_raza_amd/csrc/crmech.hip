@@ -148,6 +148,8 @@ static void fill_pot(PotParams<T>& q, const crm_potential& p) {
     // C = en/(en-em) * (en/em)^(em/(en-em)) (interaction.rs:363-367,390-392)
     q.mie_sigma = (T)(std::pow(p.em / p.en, 1.0 / (p.en - p.em)) * R2);
     q.mie_constant = (T)(p.en / (p.en - p.em) * std::pow(p.en / p.em, p.em / (p.en - p.em)));
+    q.mie_en_s = q.en / q.mie_sigma;
+    q.mie_em_s = q.em / q.mie_sigma;
     q.epsilon = (T)p.epsilon;
     q.sigma = (T)p.sigma;
     q.rel_range = (T)p.relative_interaction_range;

@@ -17,6 +17,7 @@ template <typename T>
 struct PotParams {
     T radius, two_radius, stiffness, cutoff, strength, bound;
     T en, em, mie_sigma, mie_constant;
+    T mie_en_s, mie_em_s;  // en / mie_sigma, em / mie_sigma (pair-independent)
     T epsilon, sigma;
     T rel_range, sign;
     T species_matrix[CRM_MAX_SPECIES * CRM_MAX_SPECIES];

@@ -116,7 +116,7 @@ def test_mixed_force_error_along_the_mixed_trajectory(make, steps):
 
 
 def test_c5_mixed_drift_500_steps():
-    """500-step position drift of FP32 pair forces + Kahan sums on
+    """500-step position drift of FP32 pair forces and sums on
     FP64 state against the FP64 path, max and RMS, on the 4M-agent lattice.  The
     relaxing jittered lattice amplifies perturbations at a few agents (near unstable
     equilibria): the FP64 path itself, started from positions moved by 2e-9 (1e-9 of
