@@ -215,8 +215,59 @@ __device__ __forceinline__ void bound_lj(const T* z, const PotParams<T>& P, int 
 // CellSpecificInteraction, benchmarks/src/cell_sorting.rs:31-94 (sign +1) and
 // examples/cell_sorting/src/main.rs:32-103 (sign -1).  `same` = equal species.
 // Its discontinuity is the signum at relative_interaction_range * (R + R).
+// Float path: 1/|z| from v_rsq_f32 and one Newton step gives the direction and
+// (outside the core) 1 / sigma = 2R / r, in place of sqrtf and five divisions.
+template <int D>
+__device__ __forceinline__ void cell_sorting_fast(const float* z, bool same, const PotParams<float>& P, int cut,
+                                                  float* f) {
+    const float mrd = 0.3162277660168379f;
+    float d2 = 0.0f;
+    bool zero = true;
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+        d2 = d2 + z[k] * z[k];
+        zero = zero && (z[k] == 0.0f);
+    }
+    if (zero) {
+#pragma unroll
+        for (int k = 0; k < D; k++) f[k] = 0.0f;
+        return;
+    }
+    const float y0 = __builtin_amdgcn_rsqf(d2);
+    const float y = fmaf(0.5f * y0, fmaf(-(d2 * y0), y0, 1.0f), y0);  // 1 / |z|
+    const float nz = d2 * y;
+    const float core = P.radius * mrd;
+    const bool in_core = nz < core;
+    const float r = in_core ? core : nz;
+    const float inv = P.two_radius * (in_core ? __builtin_amdgcn_rcpf(core) : y);  // 1 / sigma
+    const float bound = 4.0f + inv;
+    float spatial_cutoff;
+    if (cut < 0) {
+        const float sgn_arg = P.rel_range * P.two_radius - r;
+        const float signum = (sgn_arg != sgn_arg) ? sgn_arg : copysignf(1.0f, sgn_arg);
+        spatial_cutoff = (1.0f + signum) * 0.5f;
+    } else {
+        spatial_cutoff = cut ? 1.0f : 0.0f;
+    }
+    const float inv2 = inv * inv;
+    const float strength = P.strength * fmaxf(fminf(inv2 - inv2 * inv2, bound), -bound);
+    const float smax = fmaxf(strength, 0.0f), smin = fminf(strength, 0.0f);
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+        const float dir = z[k] * y;
+        const float att = (dir * smax) * spatial_cutoff;
+        const float rep = (dir * smin) * spatial_cutoff;
+        const float o = same ? rep + att : rep;
+        f[k] = P.sign < 0.0f ? -o : o;
+    }
+}
+
 template <typename T, int D>
 __device__ __forceinline__ void cell_sorting(const T* z, bool same, const PotParams<T>& P, int cut, T* f) {
+    if constexpr (std::is_same<T, float>::value) {
+        cell_sorting_fast<D>(z, same, P, cut, f);
+        return;
+    }
     const T mrd = T(0.3162277660168379);
     T d2 = T(0);
 #pragma unroll

@@ -252,7 +252,9 @@ __global__ __launch_bounds__(kBlock, 4) void k_force_rod(Arrays A, StateBuf S, R
                 den = den + ab[k] * ab[k];
                 span = span + av[k] * av[k] + bk * bk;
             }
-            float tt = num / den;
+            // (v_rcp_f32: 1 ulp on a parameter clamped to [0, 1]; den == 0 gives a
+            // non-finite tt and the FP64 path below, as a division would)
+            float tt = num * __builtin_amdgcn_rcpf(den);
             tt = tt < 0.0f ? 0.0f : (tt > 1.0f ? 1.0f : tt);
             float z[D], d2 = 0.0f;
 #pragma unroll

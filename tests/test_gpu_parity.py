@@ -108,6 +108,19 @@ def test_forces_match_oracle(name, weighting):
     assert rel_err(fg, fo) <= FORCE_TOL, (name, rel_err(fg, fo))
 
 
+@pytest.mark.parametrize("name", sorted(WL))
+@pytest.mark.parametrize("weighting", ["reference", "physical"])
+def test_mixed_forces_match_oracle(name, weighting):
+    """The float pair path of every potential (the Morse, Mie and cell-sorting paths
+    take 1/r from v_rsq_f32) within the mixed-precision bound of the FP64 oracle."""
+    wl = WL[name]
+    g, o = pair(wl, weighting=weighting, precision="mixed")
+    fg, fo = g.compute_forces(), o.compute_forces()
+    assert np.abs(fo).max() > 0
+    assert rel_err(fg, fo) <= MIXED_TOL, (name, rel_err(fg, fo))
+    g.close()
+
+
 @pytest.mark.parametrize("name", ["species3d", "mie3d"])
 def test_step_and_trajectory(name):
     wl = WL[name]
