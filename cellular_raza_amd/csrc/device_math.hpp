@@ -195,9 +195,33 @@ __device__ __forceinline__ bool mie(const T* z, const PotParams<T>& P, int cut, 
     return false;
 }
 
+// Float path: 1/r from v_rsq_f32 and one Newton step for sigma / r, 4 epsilon / r,
+// bound / r and the direction (in place of sqrtf and six divisions).
+template <int D>
+__device__ __forceinline__ void bound_lj_fast(const float* z, const PotParams<float>& P, int cut, float* f) {
+    float d2 = 0.0f;
+#pragma unroll
+    for (int k = 0; k < D; k++) d2 = d2 + z[k] * z[k];
+    const float y0 = __builtin_amdgcn_rsqf(d2);
+    const float y = fmaf(0.5f * y0, fmaf(-(d2 * y0), y0, 1.0f), y0);  // 1 / r
+    const float r = d2 * y;
+    const float sr = P.sigma * y;
+    const float val = (4.0f * P.epsilon * y) * (12.0f * ipow<float>(sr, 11) - 6.0f * ipow<float>(sr, 5));
+    const float mx = P.bound * y;
+    const bool inside = cut < 0 ? (P.cutoff >= r) : (cut == 1);
+    const float q = inside ? 1.0f : 0.0f;
+    const float m = fminf(mx, val);
+#pragma unroll
+    for (int k = 0; k < D; k++) f[k] = ((-(z[k] * y)) * q) * m;
+}
+
 // BoundLennardJones::calculate_force_between, interaction.rs:122-138
 template <typename T, int D>
 __device__ __forceinline__ void bound_lj(const T* z, const PotParams<T>& P, int cut, T* f) {
+    if constexpr (std::is_same<T, float>::value) {
+        bound_lj_fast<D>(z, P, cut, f);
+        return;
+    }
     T d2 = T(0);
 #pragma unroll
     for (int k = 0; k < D; k++) d2 = d2 + z[k] * z[k];
