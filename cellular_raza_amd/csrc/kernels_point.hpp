@@ -414,8 +414,9 @@ __device__ __forceinline__ void agent_force(const Arrays& A, const StateBuf& S, 
     }
     // per-agent sums: double in FP64 mode; float in mixed mode, whose forces are
     // stored as float (a compensated Kahan sum changed the stored forces by
-    // nothing measurable -- C2 error 4.29e-7 either way, C5 1.91e-6 against
-    // 1.93e-6 -- and cost 5 % of the kernel; a double accumulator was slower)
+    // nothing measurable -- 1M-agent 3-D sorting error 4.29e-7 either way, 4M-agent
+    // Mie lattice 1.91e-6 against 1.93e-6 -- and cost 5 % of the kernel; a double
+    // accumulator was slower)
     double accd[D];
     float ks[D];
 #pragma unroll
@@ -870,8 +871,9 @@ __global__ __launch_bounds__(kBlock, MIXED ? 5 : 4) void k_force_tiled(Arrays A,
             }
         }
         // the first kPreRecs records of each voxel are loaded before the scan, so
-        // that their latency overlaps its barriers (mixed: 4 int4 records, C2 force
-        // kernel 86.7 -> 84.6 us against 2; FP64 keeps 2 double4, more would spill)
+        // that their latency overlaps its barriers (mixed: 4 int4 records, 1M-agent
+        // 3-D sorting force kernel 86.7 -> 84.6 us against 2; FP64 keeps 2 double4,
+        // more would spill)
         constexpr uint32_t kPreRecs = MIXED ? 4 : 2;
         Rec pre[3][kPreRecs];
 #pragma unroll

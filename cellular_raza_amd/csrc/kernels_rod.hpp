@@ -63,8 +63,8 @@ __device__ __forceinline__ bool rod_lane(const BlockWork& w, uint32_t nv, uint64
 // to kRodLdsBytes, so that the candidate reads of the walks below come from LDS; a
 // block with more own rows or more rods reads global memory (same order, same
 // arithmetic).  36 KiB hold the 2-D 8-vertex rods' stencil rows of nearly every
-// block (C3: 32 KiB left too many blocks on global memory, 418 -> 409 us; 36 KiB
-// 348 us; 40 KiB 347 us) and leave 4 workgroups per CU.
+// block (200,000 2-D rods: 32 KiB left too many blocks on global memory, 418 ->
+// 409 us; 36 KiB 348 us; 40 KiB 347 us) and leave 4 workgroups per CU.
 constexpr uint32_t kRodLdsBytes = 36 * 1024;
 template <int D, int POT, bool MIXED>
 __global__ __launch_bounds__(kBlock, 4) void k_force_rod(Arrays A, StateBuf S, RodArgs R, uint32_t physical,
@@ -872,7 +872,7 @@ __global__ __launch_bounds__(kBlock) void k_gather_rod(Arrays A, StateBuf src, S
     // block's rod t % rpb, so that a wave's loads and stores of one component row
     // cover consecutive slots.  lt = rod * nv + vertex addresses the LDS rows.  The
     // streams read or written once (forces, the new state and ring) bypass the
-    // caches' retention (nontemporal; C3 125 -> 118 us); positions stay cached, the
+    // caches' retention (nontemporal; 200,000 2-D rods 125 -> 118 us); positions stay cached, the
     // neighbouring vertex rows read them again.
     const uint32_t rr = threadIdx.x % R.rpb, v = threadIdx.x / R.rpb;
     const uint64_t s = w.first + rr;
