@@ -819,13 +819,17 @@ crm_status reset_control(crm_ctx* ctx, uint64_t step) {
 // Tiling policies the autotuner chooses from: the fraction of the workgroup's
 // lanes the expected own agents may fill, and the density margin on the staged
 // region.  Policy 0 is the default.
+// own_frac: the tile's expected own agents per lane of the workgroup (above 1, a
+// lane walks several own agents one after another, which fills the lanes of
+// tiles cut from small grids)
 struct TilePolicy {
     double own_frac, margin;
     uint32_t lanes_log2;  // lanes per own agent: 1, 2 or 4 (agents per wavefront 64, 32, 16)
 };
 constexpr TilePolicy kTilePolicies[] = {{0.94, 1.2, 0}, {0.94, 1.1, 0}, {0.94, 1.4, 0}, {0.80, 1.2, 0},
                                         {0.70, 1.2, 0}, {0.55, 1.2, 0}, {0.40, 1.2, 0}, {0.70, 1.4, 0},
-                                        {0.94, 1.2, 1}, {0.70, 1.2, 1}, {0.94, 1.2, 2}, {0.70, 1.2, 2}};
+                                        {0.94, 1.2, 1}, {0.70, 1.2, 1}, {0.94, 1.2, 2}, {0.70, 1.2, 2},
+                                        {1.50, 1.2, 0}, {1.90, 1.2, 0}};
 constexpr uint32_t kNumTilePolicies = sizeof(kTilePolicies) / sizeof(kTilePolicies[0]);
 
 void build_tiles(const InstParams& P, uint32_t dim, uint32_t inst, uint64_t n_agents, uint32_t cap,

@@ -19,8 +19,10 @@ from oracle.oracle import OracleSim
 
 pytestmark = pytest.mark.gpu
 
-# policy index -> lanes per agent (the library's table: 0-7 one lane, 8-9 two, 10-11 four)
+# policy index -> lanes per agent (the library's table: 0-7 one lane, 8-9 two, 10-11 four,
+# 12-13 one lane with up to 1.5 or 1.9 own agents per lane)
 POLICIES = {0: 1, 8: 2, 9: 2, 10: 4, 11: 4}
+CROWDED = (12, 13)
 
 
 def rel_err(a, b):
@@ -107,3 +109,28 @@ def test_lanes_per_agent_counts_and_trajectory(lanes_policy, tmp_path):
     o.step(20)
     assert rel_err(g.get_state()[0], o.get_state()[0]) <= 1e-9
     g.close()
+
+
+@pytest.mark.parametrize("name", sorted(WORKLOADS))
+def test_tiles_with_more_agents_than_lanes(name, tmp_path):
+    """Policies whose tiles hold more own agents than the workgroup has lanes (a
+    lane walks several agents in turn): FP64 forces bitwise those of policy 0
+    (the candidate order of an agent does not depend on the tiling), mixed within
+    the mixed bar of the oracle."""
+    wl = WORKLOADS[name]()
+    o = OracleSim(wl.instances, dt=wl.dt, mechanics_solver_order=wl.solver_order)
+    o.set_agents(wl.pos, wl.vel, wl.species, wl.counts)
+    f_o = o.compute_forces()
+    base = pinned(name, wl, "fp64", 0, tmp_path / "p0_fp64.json")
+    f0 = base.compute_forces()
+    base.close()
+    for policy in CROWDED:
+        for precision in ("fp64", "mixed"):
+            g = pinned(name, wl, precision, policy, tmp_path / f"p{policy}_{precision}.json")
+            assert g.tiling()["lanes_per_agent"] == 1
+            f = g.compute_forces()
+            if precision == "fp64":
+                assert np.array_equal(f, f0), policy
+            else:
+                assert rel_err(f, f_o) <= 1e-5, (policy, rel_err(f, f_o))
+            g.close()
