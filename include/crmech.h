@@ -484,6 +484,12 @@ typedef struct crm_json_cells {
 } crm_json_cells;
 crm_status crm_save_cells_json(crm_ctx* ctx, const char* path, uint64_t iteration, const crm_json_cells* spec);
 
+/* Kernel timing of a context created with CRM_FLAG_TIMING (no counterpart in the
+ * reference).  The event pairs around the launches are recorded without waiting;
+ * crm_get_timing waits for those still in flight and adds them in, and
+ * crm_reset_timing does the same before clearing the sums.  A class may cover
+ * several launches per step (the scans, or all three sort kernels when the voxels
+ * fit one scan tile, count as CRM_KERNEL_SCAN). */
 crm_status crm_get_timing(crm_ctx* ctx, crm_timing* timing);
 crm_status crm_reset_timing(crm_ctx* ctx);
 
