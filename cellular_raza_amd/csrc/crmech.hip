@@ -824,12 +824,12 @@ crm_status reset_control(crm_ctx* ctx, uint64_t step) {
 // tiles cut from small grids)
 struct TilePolicy {
     double own_frac, margin;
-    uint32_t lanes_log2;  // lanes per own agent: 1, 2 or 4 (agents per wavefront 64, 32, 16)
+    uint32_t lanes_log2;  // lanes per own agent: 1, 2, 4 or 8 (agents per wavefront 64, 32, 16, 8)
 };
 constexpr TilePolicy kTilePolicies[] = {{0.94, 1.2, 0}, {0.94, 1.1, 0}, {0.94, 1.4, 0}, {0.80, 1.2, 0},
                                         {0.70, 1.2, 0}, {0.55, 1.2, 0}, {0.40, 1.2, 0}, {0.70, 1.4, 0},
                                         {0.94, 1.2, 1}, {0.70, 1.2, 1}, {0.94, 1.2, 2}, {0.70, 1.2, 2},
-                                        {1.50, 1.2, 0}, {1.90, 1.2, 0}};
+                                        {1.50, 1.2, 0}, {1.90, 1.2, 0}, {0.94, 1.2, 3}, {0.70, 1.2, 3}};
 constexpr uint32_t kNumTilePolicies = sizeof(kTilePolicies) / sizeof(kTilePolicies[0]);
 
 void build_tiles(const InstParams& P, uint32_t dim, uint32_t inst, uint64_t n_agents, uint32_t cap,

@@ -20,8 +20,8 @@ from oracle.oracle import OracleSim
 pytestmark = pytest.mark.gpu
 
 # policy index -> lanes per agent (the library's table: 0-7 one lane, 8-9 two, 10-11 four,
-# 12-13 one lane with up to 1.5 or 1.9 own agents per lane)
-POLICIES = {0: 1, 8: 2, 9: 2, 10: 4, 11: 4}
+# 12-13 one lane with up to 1.5 or 1.9 own agents per lane, 14-15 eight)
+POLICIES = {0: 1, 8: 2, 9: 2, 10: 4, 11: 4, 14: 8, 15: 8}
 CROWDED = (12, 13)
 
 
@@ -86,7 +86,7 @@ def test_lanes_per_agent_forces_match_oracle(name, tmp_path):
             g.close()
 
 
-@pytest.mark.parametrize("lanes_policy", [8, 10])
+@pytest.mark.parametrize("lanes_policy", [8, 10, 14])
 def test_lanes_per_agent_counts_and_trajectory(lanes_policy, tmp_path):
     """Neighbour counts (integers, summed over the lanes of a group) equal the
     oracle's; the FP64 trajectory follows the oracle for 20 steps."""

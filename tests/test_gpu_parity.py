@@ -554,7 +554,7 @@ def test_autotune_keeps_results_and_caches(tmp_path):
         assert tuned.tiling_policy is not None and path.exists()
         assert f"precision={1 if precision == 'mixed' else 0}" in path.read_text()
         t = tuned.tiling()
-        assert t["rows"] >= 1 and t["cols"] >= 1 and t["tiles"] >= 1 and t["lanes_per_agent"] in (1, 2, 4)
+        assert t["rows"] >= 1 and t["cols"] >= 1 and t["tiles"] >= 1 and t["lanes_per_agent"] in (1, 2, 4, 8)
         one_lane = t["lanes_per_agent"] == 1
         f_tuned, f_ref = tuned.compute_forces(), ref.compute_forces()
         if precision == "fp64":
