@@ -277,10 +277,11 @@ crm_status crm_get_info(crm_ctx* ctx, crm_info* info);
 /* Tiling autotuner (no counterpart in the reference, whose CPU backend has no
  * work tiles): times the force kernel on the current agents under each of the
  * library's tiling policies and keeps the fastest.  A policy sets the voxel tile
- * shape (through the share of a 256-lane workgroup the tile's own agents may
- * fill and the density margin of the staged region) and the lanes per agent
- * (1, 2 or 4: 64, 32 or 16 agents per wavefront; the lanes of an agent split its
- * candidate stream and add their partial forces with wavefront shuffles).  The
+ * shape (through the own agents per lane of a 256-lane workgroup, up to 1.9 with
+ * a lane walking several agents in turn, and the density margin of the staged
+ * region) and the lanes per agent (1, 2, 4 or 8: 64, 32, 16 or 8 agents per
+ * wavefront; the lanes of an agent split its candidate stream and add their
+ * partial forces with wavefront shuffles).  The
  * choice is cached per workload key (dimension, precision, potential, weighting,
  * instance count, agent count, grid) for the process and, when cache_path is not
  * NULL, in that JSON file, which is read first and rewritten.  FP64 forces do
