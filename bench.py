@@ -7,7 +7,7 @@ forces -> integrator -> boundary -> counting-sort re-binning of all agents,
 inputs resident in HBM.
 
 Prints one JSON line.  `value` is the mixed-precision path (float pair forces
-with compensated sums, double state) where the workload has one, else FP64; the
+and sums, double state) where the workload has one, else FP64; the
 FP64 path, the force error of the mixed path against FP64, the force kernel's
 roofline figures and the CPU baseline (the oracle's chili restatement, threaded)
 are reported beside it.  `--workload` selects the other configurations:
@@ -33,7 +33,7 @@ FP32_PEAK_TFLOPS = 157.3  # MI355X vector FP32
 # Useful FP32 work of the force kernel (FMA = 2 flops; sqrt, reciprocal, exp = 1):
 # per candidate the displacement and squared distance (3 sub, 1 mul, 2 FMA), per
 # pair inside the cutoff about 30 more (root and reciprocal, the potential's
-# exponential or powers, direction and weight, the compensated sum of 3 components).
+# exponential or powers, direction and weight, the sum of 3 components).
 FLOPS_PER_CANDIDATE = 8
 FLOPS_PER_PAIR = 30
 # committed rocprofv3 summaries of the workloads that have one

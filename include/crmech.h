@@ -84,8 +84,8 @@ typedef enum crm_mechanics_kind {
 
 typedef enum crm_precision {
     CRM_PRECISION_FP64 = 0,   /* everything in double */
-    CRM_PRECISION_MIXED = 1   /* float pair forces on voxel-relative coordinates with
-                                 compensated per-agent sums; double state and integrator */
+    CRM_PRECISION_MIXED = 1   /* float pair forces on voxel-relative coordinates and
+                                 float per-agent sums; double state and integrator */
 } crm_precision;
 
 typedef enum crm_weighting {
